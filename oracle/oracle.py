@@ -1,0 +1,301 @@
+"""ctypes binding of the WCSPH oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker or as the timed CPU baseline. It restates
+the reference CPU path (crates/vxl-phys-fluid); see wcsph_oracle.h.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+class OrConfig(C.Structure):
+    _fields_ = [
+        ("rest_density", C.c_float),
+        ("smoothing_radius", C.c_float),
+        ("xsph_viscosity", C.c_float),
+        ("sound_speed", C.c_float),
+        ("gamma_tait", C.c_float),
+        ("substeps", C.c_uint32),
+        ("threads", C.c_uint32),
+        ("tensile_clamp", C.c_uint32),
+        ("boundary_layers", C.c_uint32),
+        ("gravity", C.c_float * 3),
+        ("max_speed_frac", C.c_float),
+        ("artificial_viscosity", C.c_float),
+        ("max_bins", C.c_uint64),
+    ]
+
+
+class OrBoxBody(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32),
+        ("half", C.c_float * 3),
+        ("pos", C.c_float * 3),
+        ("rot", C.c_float * 4),
+        ("linvel", C.c_float * 3),
+        ("angvel", C.c_float * 3),
+    ]
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        fp = C.POINTER(C.c_float)
+        up = C.POINTER(C.c_uint32)
+        L.or_default_config.argtypes = [C.POINTER(OrConfig)]
+        L.or_new_lattice.restype = P
+        L.or_new_lattice.argtypes = [C.POINTER(OrConfig), fp, up, C.c_float]
+        L.or_new_particles.restype = P
+        L.or_new_particles.argtypes = [C.POINTER(OrConfig), C.c_uint32, fp, fp, C.c_float, C.c_float]
+        L.or_free.argtypes = [P]
+        L.or_set_boundary_boxes.restype = C.c_uint32
+        L.or_set_boundary_boxes.argtypes = [P, C.c_uint32, C.POINTER(OrBoxBody)]
+        L.or_set_boundary_raw.argtypes = [P, C.c_uint32, fp, fp, fp, C.c_uint32, up, fp, up, up]
+        L.or_set_velocities.argtypes = [P, C.c_uint32, fp]
+        L.or_set_threads.argtypes = [P, C.c_uint32]
+        L.or_step.argtypes = [P, C.c_float]
+        for name in ("or_n_fluid", "or_n_total", "or_n_spans"):
+            getattr(L, name).restype = C.c_uint32
+            getattr(L, name).argtypes = [P]
+        L.or_mass.restype = C.c_float
+        L.or_mass.argtypes = [P]
+        L.or_constants.argtypes = [P, fp]
+        for name in ("or_get_pos", "or_get_vel", "or_get_pmass", "or_get_dens", "or_get_press",
+                     "or_get_acc", "or_get_xsph", "or_get_bforce"):
+            getattr(L, name).argtypes = [P, fp]
+        L.or_get_ncount.argtypes = [P, up]
+        L.or_get_reactions.argtypes = [P, up, fp]
+        L.or_grid_info.argtypes = [P, fp, fp, fp, up]
+        L.or_grid_total.restype = C.c_uint64
+        L.or_grid_total.argtypes = [P]
+        L.or_grid_tables.argtypes = [P, up, up, up]
+        L.or_grid_hash.restype = C.c_uint64
+        L.or_grid_hash.argtypes = [P]
+        L.or_state_hash.restype = C.c_uint64
+        L.or_state_hash.argtypes = [P]
+        L.or_grid_box.argtypes = [fp, fp, C.c_float, C.c_uint64, fp, fp, up]
+        L.or_lattice_box.restype = C.c_uint32
+        L.or_lattice_box.argtypes = [fp, C.c_float, C.c_float, C.c_uint32, fp, C.c_uint32, up, fp]
+        L.or_boundary_self_sum_bruteforce.restype = C.c_float
+        L.or_boundary_self_sum_bruteforce.argtypes = [fp, C.c_uint32, C.c_float]
+        L.or_boundary_self_sum_grid.restype = C.c_float
+        L.or_boundary_self_sum_grid.argtypes = [fp, C.c_uint32, C.c_float]
+        L.or_neighbors.restype = C.c_uint32
+        L.or_neighbors.argtypes = [P, C.c_uint32, up, C.c_uint32]
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def default_config(**kw):
+    c = OrConfig()
+    lib().or_default_config(C.byref(c))
+    for k, v in kw.items():
+        if k == "gravity":
+            c.gravity = (C.c_float * 3)(*v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def grid_box(lo, hi, h, max_bins):
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    mn = np.zeros(3, np.float32)
+    b = np.zeros(1, np.float32)
+    d = np.zeros(3, np.uint32)
+    lib().or_grid_box(_f(lo), _f(hi), C.c_float(h), max_bins, _f(mn), _f(b), _u(d))
+    return mn, float(b[0]), [int(x) for x in d]
+
+
+def lattice_box(half, s, h, layers):
+    half = np.asarray(half, np.float32)
+    n0 = np.zeros(1, np.uint32)
+    vol = np.zeros(1, np.float32)
+    n = lib().or_lattice_box(_f(half), C.c_float(s), C.c_float(h), layers, None, 0, None, None)
+    pts = np.zeros((max(n, 1), 3), np.float32)
+    lib().or_lattice_box(_f(half), C.c_float(s), C.c_float(h), layers, _f(pts), n, _u(n0), _f(vol))
+    return pts[:n], int(n0[0]), np.float32(vol[0])
+
+
+def self_sum_bruteforce(pts, h):
+    pts = np.ascontiguousarray(pts, np.float32)
+    return np.float32(lib().or_boundary_self_sum_bruteforce(_f(pts), len(pts), C.c_float(h)))
+
+
+def self_sum_grid(pts, h):
+    pts = np.ascontiguousarray(pts, np.float32)
+    return np.float32(lib().or_boundary_self_sum_grid(_f(pts), len(pts), C.c_float(h)))
+
+
+class System:
+    """FluidSystem restated (fluid_access.rs, fluid_step.rs)."""
+
+    def __init__(self, ptr):
+        self.p = C.c_void_p(ptr)
+
+    @classmethod
+    def lattice(cls, cfg, origin, dims, spacing):
+        o = np.asarray(origin, np.float32)
+        d = np.asarray(dims, np.uint32)
+        return cls(lib().or_new_lattice(C.byref(cfg), _f(o), _u(d), C.c_float(spacing)))
+
+    @classmethod
+    def particles(cls, cfg, pos, vel, mass, spacing):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        velp = None
+        if vel is not None:
+            vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+            velp = _f(vel)
+        return cls(lib().or_new_particles(C.byref(cfg), len(pos), _f(pos), velp,
+                                          C.c_float(mass), C.c_float(spacing)))
+
+    def __del__(self):
+        if getattr(self, "p", None) is not None and _lib is not None:
+            _lib.or_free(self.p)
+            self.p = None
+
+    def set_boundary_boxes(self, bodies):
+        arr = (OrBoxBody * len(bodies))()
+        for k, b in enumerate(bodies):
+            arr[k].id = b["id"]
+            arr[k].half = (C.c_float * 3)(*b["half"])
+            arr[k].pos = (C.c_float * 3)(*b["pos"])
+            arr[k].rot = (C.c_float * 4)(*b.get("rot", (0.0, 0.0, 0.0, 1.0)))
+            arr[k].linvel = (C.c_float * 3)(*b.get("linvel", (0.0, 0.0, 0.0)))
+            arr[k].angvel = (C.c_float * 3)(*b.get("angvel", (0.0, 0.0, 0.0)))
+        return lib().or_set_boundary_boxes(self.p, len(bodies), arr)
+
+    def set_boundary_raw(self, pos, vel, pmass, spans):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        pmass = np.ascontiguousarray(pmass, np.float32)
+        body = np.array([s[0] for s in spans], np.uint32)
+        org = np.array([s[1] for s in spans], np.float32).reshape(-1, 3)
+        st = np.array([s[2] for s in spans], np.uint32)
+        en = np.array([s[3] for s in spans], np.uint32)
+        lib().or_set_boundary_raw(self.p, len(pos), _f(pos), _f(vel), _f(pmass), len(spans),
+                                  _u(body), _f(org), _u(st), _u(en))
+
+    def set_velocities(self, vel):
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        lib().or_set_velocities(self.p, len(vel), _f(vel))
+
+    def set_threads(self, t):
+        lib().or_set_threads(self.p, t)
+
+    def step(self, dt_tick):
+        lib().or_step(self.p, C.c_float(dt_tick))
+
+    @property
+    def n_fluid(self):
+        return lib().or_n_fluid(self.p)
+
+    @property
+    def n_total(self):
+        return lib().or_n_total(self.p)
+
+    @property
+    def mass(self):
+        return np.float32(lib().or_mass(self.p))
+
+    def constants(self):
+        o = np.zeros(8, np.float32)
+        lib().or_constants(self.p, _f(o))
+        return dict(zip(["h", "h2", "k6", "ks", "w0", "b_tait", "mass", "alpha_c"], o))
+
+    def _vec(self, fn, n):
+        o = np.zeros((n, 3), np.float32)
+        fn(self.p, _f(o))
+        return o
+
+    def _sc(self, fn, n, dt=np.float32):
+        o = np.zeros(n, dt)
+        fn(self.p, _f(o) if dt == np.float32 else _u(o))
+        return o
+
+    def pos(self):
+        return self._vec(lib().or_get_pos, self.n_total)
+
+    def vel(self):
+        return self._vec(lib().or_get_vel, self.n_total)
+
+    def acc(self):
+        return self._vec(lib().or_get_acc, self.n_fluid)
+
+    def xsph(self):
+        return self._vec(lib().or_get_xsph, self.n_fluid)
+
+    def bforce(self):
+        return self._vec(lib().or_get_bforce, self.n_total - self.n_fluid)
+
+    def pmass(self):
+        return self._sc(lib().or_get_pmass, self.n_total)
+
+    def dens(self):
+        return self._sc(lib().or_get_dens, self.n_total)
+
+    def press(self):
+        return self._sc(lib().or_get_press, self.n_total)
+
+    def ncount(self):
+        return self._sc(lib().or_get_ncount, self.n_total, np.uint32)
+
+    def reactions(self):
+        n = lib().or_n_spans(self.p)
+        body = np.zeros(max(n, 1), np.uint32)
+        ft = np.zeros((max(n, 1), 6), np.float32)
+        lib().or_get_reactions(self.p, _u(body), _f(ft))
+        return body[:n], ft[:n]
+
+    def grid_info(self):
+        mn = np.zeros(3, np.float32)
+        inv = np.zeros(1, np.float32)
+        b = np.zeros(1, np.float32)
+        d = np.zeros(3, np.uint32)
+        lib().or_grid_info(self.p, _f(mn), _f(inv), _f(b), _u(d))
+        return mn, inv[0], b[0], [int(x) for x in d]
+
+    def grid_tables(self):
+        tot = lib().or_grid_total(self.p)
+        start = np.zeros(tot + 1, np.uint32)
+        items = np.zeros(self.n_total, np.uint32)
+        keys = np.zeros(self.n_total, np.uint32)
+        lib().or_grid_tables(self.p, _u(start), _u(items), _u(keys))
+        return start, items, keys
+
+    def grid_hash(self):
+        return int(lib().or_grid_hash(self.p))
+
+    def state_hash(self):
+        return int(lib().or_state_hash(self.p))
+
+    def neighbors(self, i, cap=4096):
+        o = np.zeros(cap, np.uint32)
+        n = lib().or_neighbors(self.p, i, _u(o), cap)
+        return o[:min(n, cap)]
