@@ -1,0 +1,186 @@
+/*
+ * vxl_sph.h — C ABI of the MI355X (gfx950) WCSPH fluid step.
+ *
+ * Drop-in boundary for the reference engine's GPU fluid path:
+ *   - PhysGpuBackend  (crates/vxl-phys-gpu/src/lib.rs:20-29)      -> vxl_sph_backend_*
+ *   - FluidStepper    (crates/vxl-phys-core/src/interop.rs:293-316) -> vxl_sph_stepper_*
+ *     semantics of GpuFluidStepper (crates/vxl-phys-gpu/src/pipeline/stepper.rs:70-147)
+ *   - FluidSystem step/accessors (crates/vxl-phys-fluid/src/fluid_step.rs:13-147,
+ *     fluid_access.rs:78-186) over a pool of scenes               -> vxl_sph_engine_*, vxl_sph_scene_*
+ *   - boundary::lattice (crates/vxl-phys-fluid/src/boundary.rs:67-88) -> vxl_sph_box_lattice
+ * Conventions follow crates/vxl-phys-ffi/src/lib.rs:1-41: a versioned ABI,
+ * every configuration block starts with {size, version}, batch calls only,
+ * errors are status codes (exceptions never cross the boundary).
+ *
+ * Vec3 arrays are 3 floats per particle (the repr(C) layout of &[Vec3]).
+ * Handles are not thread-safe: one host thread per handle.
+ */
+#ifndef VXL_SPH_H
+#define VXL_SPH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ffi/src/lib.rs:13 — bump on any incompatible change. */
+#define VXL_SPH_ABI_VERSION 1u
+
+/* ffi/src/lib.rs:24-34 (codes 0-5 keep the reference's meaning). */
+typedef enum VxlSphStatus {
+    VXL_SPH_OK = 0,
+    VXL_SPH_NULL_POINTER = 1,
+    VXL_SPH_VERSION_MISMATCH = 2,
+    VXL_SPH_SIZE_MISMATCH = 3,
+    VXL_SPH_NOT_READY = 4,
+    VXL_SPH_PANIC_CAUGHT = 5,
+    VXL_SPH_INVALID_ARGUMENT = 16,
+    VXL_SPH_UNSUPPORTED = 17,
+    VXL_SPH_DEVICE_ERROR = 18,
+    VXL_SPH_OUT_OF_MEMORY = 19,
+    VXL_SPH_CAPACITY = 20,
+    VXL_SPH_NOT_FOUND = 21
+} VxlSphStatus;
+
+/* ffi/src/lib.rs:17-22 */
+typedef struct VxlSphHeader {
+    uint32_t size;    /* sizeof the whole block */
+    uint32_t version; /* VXL_SPH_ABI_VERSION */
+} VxlSphHeader;
+
+/* FluidConfig (fluid/src/config.rs:6-69) + the values a restated scene needs. */
+typedef struct VxlSphFluidConfig {
+    VxlSphHeader hdr;
+    float rest_density;         /* rho0 */
+    float smoothing_radius;     /* h */
+    float xsph_viscosity;       /* eps */
+    float sound_speed;          /* c */
+    float gamma_tait;           /* only 7 is supported (fluid_force.rs:17-21) */
+    uint32_t substeps;          /* substeps per tick */
+    uint32_t tensile_clamp;     /* tensile_instability_suppression: p < 0 -> 0 */
+    uint32_t boundary_layers;   /* Akinci layers of vxl_sph_box_lattice users */
+    float gravity[3];
+    float max_speed_frac;       /* CFL clamp |v| <= frac * h / dt */
+    float artificial_viscosity; /* Monaghan alpha */
+    float particle_mass;        /* FluidSystem::mass; 0 = lattice calibration (fluid_access.rs:27-42) */
+    float spacing;              /* lattice spacing (mass calibration, Akinci sampling) */
+    uint64_t max_bins;          /* grid budget GRID_MAX_BINS (core/grid.rs:20); 0 = 1 << 21 */
+    float dt_tick;              /* tick length used by vxl_sph_engine_step */
+    uint32_t reserved;
+} VxlSphFluidConfig;
+
+/* Boundary span (body, origin, [start, end)) — FluidSystem::boundary_spans (fluid_boundary.rs:129-135). */
+typedef struct VxlSphSpan {
+    uint32_t body;
+    float origin[3];
+    uint32_t start;
+    uint32_t end;
+} VxlSphSpan;
+
+/* (body, force, torque about the body origin) — FluidSystem::boundary_reactions (fluid_access.rs:113-118). */
+typedef struct VxlSphReaction {
+    uint32_t body;
+    float force[3];
+    float torque[3];
+} VxlSphReaction;
+
+/* Grid of the last rebuild — FluidSystem::neighbor_grid (fluid_access.rs:128-140). */
+typedef struct VxlSphGridInfo {
+    float min[3];
+    float inv;
+    float bin;
+    uint32_t dims[3];
+    uint32_t total;
+} VxlSphGridInfo;
+
+typedef struct VxlSphEngine VxlSphEngine;
+typedef struct VxlSphStepper VxlSphStepper;
+
+/* ---- library / backend (PhysGpuBackend, gpu/src/lib.rs:20-29) ---- */
+uint32_t vxl_sph_abi_version(void);
+const char *vxl_sph_backend_name(void);                 /* "hip-gfx950" */
+int32_t vxl_sph_backend_available(void);                /* 1 when a gfx950 device is visible */
+int32_t vxl_sph_backend_submit(const char *tag);        /* 1 for "fluid" when available */
+VxlSphStatus vxl_sph_device_count(int32_t *out);
+const char *vxl_sph_last_error(void);                   /* thread-local message of the last failure */
+VxlSphStatus vxl_sph_default_config(VxlSphFluidConfig *out); /* FluidConfig::default (config.rs:51-69) */
+
+/* ---- engine: a pool of independent scenes on one device ---- */
+VxlSphStatus vxl_sph_engine_create(uint32_t abi_version, int32_t device, VxlSphEngine **out);
+VxlSphStatus vxl_sph_engine_destroy(VxlSphEngine *e);
+/* Enable per-particle diagnostics (neighbour counts, acceleration, XSPH). */
+VxlSphStatus vxl_sph_engine_set_diagnostics(VxlSphEngine *e, int32_t on);
+/* Replay each substep as one hipGraph (default on). */
+VxlSphStatus vxl_sph_engine_set_graphs(VxlSphEngine *e, int32_t on);
+/* Advance every scene by `ticks` ticks (enqueue only; see vxl_sph_engine_sync). */
+VxlSphStatus vxl_sph_engine_step(VxlSphEngine *e, uint32_t ticks);
+VxlSphStatus vxl_sph_engine_sync(VxlSphEngine *e);
+VxlSphStatus vxl_sph_engine_device_bytes(VxlSphEngine *e, uint64_t *out);
+
+/* Add a scene: all particles (fluid prefix, then boundary) in index order,
+ * FluidSystem::raw_particles layout (fluid_access.rs:147-155). vel / pmass
+ * may be NULL (zero velocity / cfg mass). */
+VxlSphStatus vxl_sph_scene_add(VxlSphEngine *e, const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid,
+                               const float *pos, const float *vel, const float *pmass, const VxlSphSpan *spans,
+                               uint32_t nspans, uint32_t *out_scene);
+VxlSphStatus vxl_sph_scene_remove(VxlSphEngine *e, uint32_t scene);
+/* Replace the boundary segment's positions/velocities (index order, nb = n - n_fluid);
+ * GpuFluidStepper's per-tick boundary upload (pipeline/coupling.rs:20-50). */
+VxlSphStatus vxl_sph_scene_upload_boundary(VxlSphEngine *e, uint32_t scene, uint32_t nb, const float *pos,
+                                           const float *vel);
+/* Positions / velocities of all n particles, index order (either may be NULL). */
+VxlSphStatus vxl_sph_scene_read_state(VxlSphEngine *e, uint32_t scene, uint32_t n, float *pos, float *vel);
+/* dens/press: n; ncount: n (diagnostics); acc/xsph: 3*n_fluid (diagnostics). Any may be NULL. */
+VxlSphStatus vxl_sph_scene_read_fields(VxlSphEngine *e, uint32_t scene, uint32_t n, float *dens, float *press,
+                                       uint32_t *ncount, float *acc, float *xsph);
+/* start: total+1 entries, items: n entries (scene-local indices), keys: n entries (cell per particle). */
+VxlSphStatus vxl_sph_scene_read_grid(VxlSphEngine *e, uint32_t scene, VxlSphGridInfo *info, uint32_t start_cap,
+                                     uint32_t *start, uint32_t n, uint32_t *items, uint32_t *keys);
+/* AABB of all particles at the next grid rebuild (FluidStepper::bounds). */
+VxlSphStatus vxl_sph_scene_read_bounds(VxlSphEngine *e, uint32_t scene, float lo[3], float hi[3]);
+/* FNV-1a-64 over fluid pos then vel f32 bits per particle, index order
+ * (crates/vxl-phys-fluid/examples/sph_scale.rs:52-63). */
+VxlSphStatus vxl_sph_scene_state_hash(VxlSphEngine *e, uint32_t scene, uint64_t *out);
+
+/* ---- FluidStepper drop-in (single scene; GpuFluidStepper semantics) ---- */
+VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,
+                                    VxlSphStepper **out);
+VxlSphStatus vxl_sph_stepper_destroy(VxlSphStepper *s);
+/* FluidStepper::step: pos/vel/pmass are all particles (fluid prefix + boundary
+ * segment). The first call uploads the whole scene; later calls upload only the
+ * boundary segment (fluid state is device-resident) and length-check pmass. */
+VxlSphStatus vxl_sph_stepper_step(VxlSphStepper *s, float dt_tick, uint32_t n_fluid, uint32_t n, const float *pos,
+                                  const float *vel, const float *pmass, const VxlSphSpan *spans, uint32_t nspans);
+VxlSphStatus vxl_sph_stepper_reactions(VxlSphStepper *s, uint32_t cap, VxlSphReaction *out, uint32_t *count);
+VxlSphStatus vxl_sph_stepper_bounds(VxlSphStepper *s, float lo[3], float hi[3]);
+VxlSphStatus vxl_sph_stepper_read_state(VxlSphStepper *s, uint32_t n, float *pos, float *vel);
+
+/* ---- scene construction ---- */
+/* boundary::lattice for Shape::Box (boundary.rs:67-88, :208-232): writes up to cap
+ * points (3 floats each), returns the count, points per layer and V_b. */
+VxlSphStatus vxl_sph_box_lattice(const float half[3], float s, float h, uint32_t layers, uint32_t cap,
+                                 float *out_pts, uint32_t *out_n, uint32_t *out_n0, float *out_volume);
+/* FluidSystem::new constants (fluid_access.rs:8-42): h, h2, k6, ks, w0, b_tait, mass, alpha_c. */
+VxlSphStatus vxl_sph_fluid_constants(const VxlSphFluidConfig *cfg, float out[8]);
+
+/* Seeded dam-break scene (tank [0,L]^3 open at the top, fluid block in the
+ * corner, 5 Akinci Box bodies of thickness 2d). Call with pos == NULL to
+ * query n / n_fluid / nspans, then again with buffers of those sizes. */
+typedef struct VxlSphDamBreak {
+    VxlSphHeader hdr;
+    float tank[3];
+    uint32_t block[3];
+    float d;
+    float jitter;      /* jitter amplitude as a fraction of d */
+    uint64_t seed;
+    uint32_t walls;    /* 0 = no tank particles */
+    uint32_t reserved;
+} VxlSphDamBreak;
+VxlSphStatus vxl_sph_dam_break(const VxlSphDamBreak *spec, const VxlSphFluidConfig *cfg, uint32_t *n,
+                               uint32_t *n_fluid, uint32_t *nspans, float *pos, float *vel, float *pmass,
+                               VxlSphSpan *spans);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

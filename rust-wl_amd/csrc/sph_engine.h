@@ -1,0 +1,135 @@
+// Host runtime of the WCSPH pool: device buffers, scene table, the substep
+// launch sequence (captured once as a hipGraph and replayed), readbacks.
+#ifndef VXL_SPH_ENGINE_H
+#define VXL_SPH_ENGINE_H
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sph_kernels.h"
+#include "sph_scene.h"
+
+namespace vxl {
+
+struct DeviceError : std::runtime_error {
+    hipError_t code;
+    DeviceError(hipError_t c, const std::string &w) : std::runtime_error(w), code(c) {}
+};
+struct CapacityError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// One scene as the host knows it.
+struct SceneSlot {
+    uint32_t id = 0;
+    SceneParams params{};  // pbase/alloc assigned by the layout
+    uint32_t substeps = 1;
+    std::vector<Span> spans;
+    // host staging for a scene that is not on the device yet
+    bool resident = false;
+    std::vector<float> pos, vel, pmass;  // AoS 3n / 3n / n
+    uint32_t cells_hint = 0;             // grid cells of the initial box
+    // reactions of the last tick (tick average), per span: F[3], tau[3]
+    std::vector<float> reactions;
+    bool want_reactions = false;
+};
+
+class Engine {
+public:
+    explicit Engine(int device);
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // Scene management (takes effect at the next step or readback).
+    uint32_t add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos,
+                       const float *vel, const float *pmass, const Span *spans, uint32_t nspans,
+                       bool reactions = true);
+    void remove_scene(uint32_t id);
+    bool has_scene(uint32_t id) const;
+    const SceneSlot &scene(uint32_t id) const;
+    size_t scene_count() const { return scenes_.size(); }
+
+    // Overwrite the boundary segment pos/vel of a resident scene (index order).
+    void upload_boundary(uint32_t id, uint32_t nb, const float *pos, const float *vel);
+
+    // Advance every scene by `ticks` ticks of its own substeps (enqueue only).
+    void step(uint32_t ticks);
+    // Wait for the device and surface deferred errors.
+    void sync();
+
+    // Readbacks (synchronous), index order.
+    void read_state(uint32_t id, float *pos, float *vel);
+    void read_fields(uint32_t id, float *dens, float *press, uint32_t *ncount, float *acc, float *xsph);
+    SceneGrid read_grid(uint32_t id, std::vector<uint32_t> *start, std::vector<uint32_t> *items);
+    void read_bounds(uint32_t id, float lo[3], float hi[3]);
+    // Tick-averaged (F, tau) per span of the scene: 6 floats each.
+    std::vector<float> read_reactions(uint32_t id);
+
+    void set_diagnostics(bool on);
+    void set_graphs(bool on) { use_graph_ = on; invalidate_graph(); }
+    hipStream_t stream() const { return stream_; }
+    uint64_t substeps_done() const { return substeps_done_; }
+
+    // Launch geometry knobs (autotuned). 0 = default.
+    struct Tuning {
+        uint32_t persist_blocks = 0;  // grid-stride grid size cap for slot kernels
+    };
+    void set_tuning(const Tuning &t) { tuning_ = t; invalidate_graph(); }
+    uint64_t device_bytes() const { return bytes_; }
+
+private:
+    void layout();            // assign slot ranges, (re)allocate, move + upload
+    void enqueue_substep();   // the launch sequence of one substep
+    void build_graph();
+    void invalidate_graph();
+    void check_error_state();  // read the control block; grow tables on demand
+    void alloc_pool(uint32_t slots);
+    void alloc_cells(uint32_t ccap);
+    void free_all();
+    template <class T>
+    T *dalloc(size_t n);
+    void dfree(void *p);
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    std::vector<SceneSlot> scenes_;
+    uint32_t next_id_ = 1;
+    bool dirty_ = true;        // layout must be rebuilt
+    bool diagnostics_ = false;
+    bool use_graph_ = true;
+    Tuning tuning_;
+
+    uint32_t slots_ = 0;       // pool slots allocated
+    uint32_t used_slots_ = 0;  // slots assigned to scenes
+    uint32_t ccap_ = 0;
+    uint32_t tiles_cap_ = 0;
+    Pool P_{};
+    std::vector<void *> pool_allocs_;
+    SceneParams *d_params_ = nullptr;
+    SceneGrid *d_grid_ = nullptr;
+    SceneBox *d_box_ = nullptr;
+    Control *d_ctl_ = nullptr;
+    Control *h_ctl_ = nullptr;  // pinned mirror
+    uint32_t *d_block_scene_ = nullptr;
+    uint32_t *d_tile_sums_ = nullptr;
+    SpanDev *d_spans_ = nullptr;
+    float *d_react_ = nullptr;
+    uint32_t nspans_ = 0;
+    std::vector<uint32_t> span_first_;  // first global span of each scene (by scene position)
+    // index-order staging for readbacks
+    float *d_out_f_ = nullptr;  // 6 * slots
+    uint32_t *d_out_u_ = nullptr;
+    float *d_out_fields_ = nullptr;  // dens, press, acc(3), xsph(3) = 8 * slots
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    uint64_t substeps_done_ = 0;
+    uint32_t ctl_step_base_ = 0;
+    uint64_t bytes_ = 0;
+};
+
+}  // namespace vxl
+#endif

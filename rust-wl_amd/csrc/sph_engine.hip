@@ -1,0 +1,572 @@
+// Host runtime of the WCSPH pool (see sph_engine.h).
+#include "sph_engine.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace vxl {
+
+namespace {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(e, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+// Cell-table entries to reserve for a scene: the grid of its initial box
+// with room for the box to grow (splashes raise the free surface), capped by
+// the scene's own budget.
+uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
+    uint64_t want = std::max<uint64_t>((uint64_t)cells_hint * 4ull, 4096ull);
+    want = std::min<uint64_t>(want, max_bins);
+    return (uint32_t)want + 1u;  // + pad cell
+}
+
+}  // namespace
+
+Engine::Engine(int device) : device_(device) {
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipHostMalloc((void **)&h_ctl_, sizeof(Control), hipHostMallocDefault), "hipHostMalloc");
+    std::memset(h_ctl_, 0, sizeof(Control));
+    d_ctl_ = dalloc<Control>(1);
+    hip_check(hipMemcpy(d_ctl_, h_ctl_, sizeof(Control), hipMemcpyHostToDevice), "ctl init");
+}
+
+Engine::~Engine() {
+    try {
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        invalidate_graph();
+        free_all();
+        dfree(d_ctl_);
+        if (h_ctl_) (void)hipHostFree(h_ctl_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    } catch (...) {
+    }
+}
+
+template <class T>
+T *Engine::dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw DeviceError(e, std::string("hipMalloc: ") + hipGetErrorString(e));
+    bytes_ += n * sizeof(T);
+    return (T *)p;
+}
+
+void Engine::dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+void Engine::free_all() {
+    for (void *p : pool_allocs_) dfree(p);
+    pool_allocs_.clear();
+    dfree(d_params_);
+    dfree(d_grid_);
+    dfree(d_box_);
+    dfree(d_block_scene_);
+    dfree(d_tile_sums_);
+    dfree(d_spans_);
+    dfree(d_react_);
+    d_spans_ = nullptr;
+    d_react_ = nullptr;
+    nspans_ = 0;
+    d_params_ = nullptr;
+    d_grid_ = nullptr;
+    d_box_ = nullptr;
+    d_block_scene_ = nullptr;
+    d_tile_sums_ = nullptr;
+    d_out_f_ = nullptr;
+    d_out_u_ = nullptr;
+    d_out_fields_ = nullptr;
+    std::memset(&P_, 0, sizeof(P_));
+    slots_ = 0;
+    ccap_ = 0;
+    bytes_ = sizeof(Control);
+}
+
+void Engine::invalidate_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    gexec_ = nullptr;
+    graph_ = nullptr;
+}
+
+uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos, const float *vel,
+                           const float *pmass, const Span *spans, uint32_t nspans, bool reactions) {
+    if (nf > n) throw std::invalid_argument("n_fluid exceeds particle count");
+    if (n == 0) throw std::invalid_argument("empty scene");
+    if (fabsf(cfg.gamma_tait - 7.0f) >= 1e-6f)
+        throw std::domain_error("only the gamma = 7 Tait branch (fluid_force.rs:17-21) is supported");
+    SceneSlot sc;
+    sc.id = next_id_++;
+    sc.params = scene_params(cfg, n, nf);
+    sc.substeps = std::max<uint32_t>(cfg.substeps, 1);
+    sc.pos.assign(pos, pos + 3ull * n);
+    if (vel)
+        sc.vel.assign(vel, vel + 3ull * n);
+    else
+        sc.vel.assign(3ull * n, 0.0f);
+    if (pmass)
+        sc.pmass.assign(pmass, pmass + n);
+    else
+        sc.pmass.assign(n, sc.params.mass);
+    for (uint32_t k = 0; k < nspans; ++k) {
+        if (spans[k].start < nf || spans[k].end > n || spans[k].start > spans[k].end)
+            throw std::invalid_argument("span outside the boundary segment");
+        sc.spans.push_back(spans[k]);
+    }
+    sc.want_reactions = reactions && nspans > 0;
+    sc.params.react = sc.want_reactions ? 1u : 0u;
+    // Initial grid size (host grid_box over the initial AABB) sizes the tables.
+    float lo[3] = {pos[0], pos[1], pos[2]}, hi[3] = {pos[0], pos[1], pos[2]};
+    for (uint32_t i = 1; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], pos[3 * i + a]);
+            hi[a] = fmaxf(hi[a], pos[3 * i + a]);
+        }
+    float gmin[3], bin;
+    uint32_t dims[3];
+    grid_box(lo, hi, sc.params.h, sc.params.max_bins, gmin, bin, dims);
+    sc.cells_hint = dims[0] * dims[1] * dims[2];
+    scenes_.push_back(std::move(sc));
+    dirty_ = true;
+    return scenes_.back().id;
+}
+
+void Engine::remove_scene(uint32_t id) {
+    auto it = std::find_if(scenes_.begin(), scenes_.end(), [&](const SceneSlot &s) { return s.id == id; });
+    if (it == scenes_.end()) throw std::out_of_range("unknown scene");
+    if (it->resident) {
+        // Only the slot map changes; device state of the remaining scenes is
+        // moved by the next layout.
+    }
+    scenes_.erase(it);
+    dirty_ = true;
+}
+
+bool Engine::has_scene(uint32_t id) const {
+    return std::any_of(scenes_.begin(), scenes_.end(), [&](const SceneSlot &s) { return s.id == id; });
+}
+
+const SceneSlot &Engine::scene(uint32_t id) const {
+    for (const auto &s : scenes_)
+        if (s.id == id) return s;
+    throw std::out_of_range("unknown scene");
+}
+
+void Engine::alloc_pool(uint32_t slots) {
+    slots_ = slots;
+    auto F = [&](float *&p) {
+        p = dalloc<float>(slots);
+        pool_allocs_.push_back(p);
+    };
+    auto U = [&](uint32_t *&p) {
+        p = dalloc<uint32_t>(slots);
+        pool_allocs_.push_back(p);
+    };
+    F(P_.cx); F(P_.cy); F(P_.cz); F(P_.cvx); F(P_.cvy); F(P_.cvz); F(P_.cm); U(P_.cg);
+    F(P_.sx); F(P_.sy); F(P_.sz); F(P_.svx); F(P_.svy); F(P_.svz); F(P_.sm); U(P_.sg);
+    U(P_.key); U(P_.slot); U(P_.items_m); U(P_.items_g);
+    F(P_.dens); F(P_.press); F(P_.c2);
+    if (diagnostics_) {
+        U(P_.ncount);
+        F(P_.ax); F(P_.ay); F(P_.az); F(P_.xx); F(P_.xy); F(P_.xz);
+    }
+}
+
+void Engine::alloc_cells(uint32_t ccap) {
+    ccap_ = ccap;
+    P_.counts = dalloc<uint32_t>(ccap);
+    pool_allocs_.push_back(P_.counts);
+    P_.start = dalloc<uint32_t>((size_t)ccap + 1);
+    pool_allocs_.push_back(P_.start);
+    hip_check(hipMemset(P_.counts, 0, sizeof(uint32_t) * ccap), "memset counts");
+    tiles_cap_ = (ccap + 2047) / 2048 + 1;
+    d_tile_sums_ = dalloc<uint32_t>(tiles_cap_);
+    P_.ccap = ccap;
+}
+
+// Assign slot ranges, (re)allocate, move resident scenes, upload new ones.
+void Engine::layout() {
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    hip_check(hipStreamSynchronize(stream_), "sync before layout");
+    invalidate_graph();
+    // Save resident scenes' device state (cur arrays, this grid order).
+    struct Saved {
+        uint32_t idx, n;
+        std::vector<float> f[7];
+        std::vector<uint32_t> g;
+    };
+    std::vector<Saved> saved;
+    for (uint32_t k = 0; k < scenes_.size(); ++k) {
+        SceneSlot &sc = scenes_[k];
+        if (!sc.resident) continue;
+        Saved sv;
+        sv.idx = k;
+        sv.n = sc.params.n;
+        float *src[7] = {P_.cx, P_.cy, P_.cz, P_.cvx, P_.cvy, P_.cvz, P_.cm};
+        for (int a = 0; a < 7; ++a) {
+            sv.f[a].resize(sv.n);
+            hip_check(hipMemcpy(sv.f[a].data(), src[a] + sc.params.pbase, 4ull * sv.n, hipMemcpyDeviceToHost),
+                      "save scene");
+        }
+        sv.g.resize(sv.n);
+        hip_check(hipMemcpy(sv.g.data(), P_.cg + sc.params.pbase, 4ull * sv.n, hipMemcpyDeviceToHost), "save gid");
+        saved.push_back(std::move(sv));
+    }
+    // Slot and cell reservations.
+    uint32_t used = 0;
+    uint64_t cells = 0;
+    std::vector<uint32_t> block_scene;
+    for (uint32_t k = 0; k < scenes_.size(); ++k) {
+        SceneParams &p = scenes_[k].params;
+        p.pbase = used;
+        p.alloc = round_up(p.n, kBlock);
+        used += p.alloc;
+        for (uint32_t b = 0; b < p.alloc / kBlock; ++b) block_scene.push_back(k);
+        cells += cell_reserve(scenes_[k].cells_hint, p.max_bins);
+    }
+    used_slots_ = used;
+    const uint32_t want_cells = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(cells, ccap_), 0xfffffff0ull);
+    free_all();
+    alloc_pool(std::max<uint32_t>(used, kBlock));
+    alloc_cells(std::max<uint32_t>(want_cells, 4096));
+    d_params_ = dalloc<SceneParams>(std::max<size_t>(scenes_.size(), 1));
+    d_grid_ = dalloc<SceneGrid>(std::max<size_t>(scenes_.size(), 1));
+    d_box_ = dalloc<SceneBox>(2 * std::max<size_t>(scenes_.size(), 1));
+    d_block_scene_ = dalloc<uint32_t>(std::max<size_t>(block_scene.size(), 1));
+    d_out_f_ = dalloc<float>(6ull * slots_);
+    pool_allocs_.push_back(d_out_f_);
+    d_out_fields_ = dalloc<float>(8ull * slots_);
+    pool_allocs_.push_back(d_out_fields_);
+    d_out_u_ = dalloc<uint32_t>(slots_);
+    pool_allocs_.push_back(d_out_u_);
+    // Reaction spans (scenes with reactions only).
+    std::vector<SpanDev> spans;
+    span_first_.assign(scenes_.size(), 0);
+    for (uint32_t k = 0; k < scenes_.size(); ++k) {
+        span_first_[k] = (uint32_t)spans.size();
+        if (!scenes_[k].want_reactions) continue;
+        for (const Span &s : scenes_[k].spans) {
+            SpanDev d{};
+            d.scene = k;
+            d.start = scenes_[k].params.pbase + s.start;
+            d.end = scenes_[k].params.pbase + s.end;
+            d.substeps = scenes_[k].substeps;
+            for (int a = 0; a < 3; ++a) d.origin[a] = s.origin[a];
+            spans.push_back(d);
+        }
+    }
+    nspans_ = (uint32_t)spans.size();
+    if (nspans_) {
+        auto F = [&](float *&p) {
+            p = dalloc<float>(slots_);
+            pool_allocs_.push_back(p);
+        };
+        F(P_.bfx); F(P_.bfy); F(P_.bfz); F(P_.bpx); F(P_.bpy); F(P_.bpz);
+        d_spans_ = dalloc<SpanDev>(nspans_);
+        hip_check(hipMemcpy(d_spans_, spans.data(), sizeof(SpanDev) * nspans_, hipMemcpyHostToDevice), "spans");
+        d_react_ = dalloc<float>(6ull * nspans_);
+        hip_check(hipMemset(d_react_, 0, 24ull * nspans_), "react");
+    }
+    P_.nspans = nspans_;
+    P_.spans = d_spans_;
+    P_.react = d_react_;
+    std::vector<SceneParams> hp;
+    for (auto &sc : scenes_) hp.push_back(sc.params);
+    if (!hp.empty())
+        hip_check(hipMemcpy(d_params_, hp.data(), sizeof(SceneParams) * hp.size(), hipMemcpyHostToDevice), "params");
+    if (!block_scene.empty())
+        hip_check(hipMemcpy(d_block_scene_, block_scene.data(), 4 * block_scene.size(), hipMemcpyHostToDevice),
+                  "block_scene");
+    // Upload state: resident scenes from the saved copies, new scenes from
+    // their staging (index order; the first grid build sorts them).
+    std::vector<float> tmp;
+    float *dst[7] = {P_.cx, P_.cy, P_.cz, P_.cvx, P_.cvy, P_.cvz, P_.cm};
+    for (auto &sv : saved) {
+        SceneSlot &sc = scenes_[sv.idx];
+        for (int a = 0; a < 7; ++a)
+            hip_check(hipMemcpy(dst[a] + sc.params.pbase, sv.f[a].data(), 4ull * sv.n, hipMemcpyHostToDevice),
+                      "restore");
+        hip_check(hipMemcpy(P_.cg + sc.params.pbase, sv.g.data(), 4ull * sv.n, hipMemcpyHostToDevice), "restore gid");
+    }
+    for (auto &sc : scenes_) {
+        if (sc.resident) continue;
+        const uint32_t n = sc.params.n;
+        tmp.resize(n);
+        for (int a = 0; a < 3; ++a) {
+            for (uint32_t i = 0; i < n; ++i) tmp[i] = sc.pos[3 * i + a];
+            hip_check(hipMemcpy(dst[a] + sc.params.pbase, tmp.data(), 4ull * n, hipMemcpyHostToDevice), "pos");
+            for (uint32_t i = 0; i < n; ++i) tmp[i] = sc.vel[3 * i + a];
+            hip_check(hipMemcpy(dst[3 + a] + sc.params.pbase, tmp.data(), 4ull * n, hipMemcpyHostToDevice), "vel");
+        }
+        hip_check(hipMemcpy(P_.cm + sc.params.pbase, sc.pmass.data(), 4ull * n, hipMemcpyHostToDevice), "pmass");
+        std::vector<uint32_t> g(n);
+        for (uint32_t i = 0; i < n; ++i) g[i] = i;
+        hip_check(hipMemcpy(P_.cg + sc.params.pbase, g.data(), 4ull * n, hipMemcpyHostToDevice), "gid");
+        sc.resident = true;
+        sc.pos.clear(); sc.pos.shrink_to_fit();
+        sc.vel.clear(); sc.vel.shrink_to_fit();
+    }
+    P_.params = d_params_;
+    P_.grid = d_grid_;
+    P_.box = d_box_;
+    P_.ctl = d_ctl_;
+    P_.block_scene = d_block_scene_;
+    P_.nblocks = used / kBlock;
+    P_.nscenes = (uint32_t)scenes_.size();
+    // Control block + AABB of the uploaded state into box[step & 1].
+    std::memset(h_ctl_, 0, sizeof(Control));
+    hip_check(hipMemcpy(d_ctl_, h_ctl_, sizeof(Control), hipMemcpyHostToDevice), "ctl");
+    ctl_step_base_ = 0;
+    std::vector<SceneBox> init(2 * std::max<size_t>(scenes_.size(), 1));
+    for (auto &b : init) {
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = 0xffffffffu;
+            b.hi[a] = 0u;
+        }
+    }
+    hip_check(hipMemcpy(d_box_, init.data(), sizeof(SceneBox) * init.size(), hipMemcpyHostToDevice), "box");
+    if (P_.nblocks) {
+        hipLaunchKernelGGL(k_box, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_);
+        hip_check(hipGetLastError(), "k_box");
+    }
+    hip_check(hipStreamSynchronize(stream_), "layout sync");
+    dirty_ = false;
+}
+
+void Engine::enqueue_substep() {
+    const uint32_t nb = P_.nblocks;
+    const uint32_t grid_slots = tuning_.persist_blocks ? std::min(nb, tuning_.persist_blocks) : nb;
+    const uint32_t grid_scan = std::min<uint32_t>(tiles_cap_, 4096);
+    hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream_, P_);
+    hipLaunchKernelGGL(k_keys, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream_, P_, d_tile_sums_);
+    hipLaunchKernelGGL(k_scan_apply, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+    hipLaunchKernelGGL(k_place, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_canon_gather, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_force, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    if (nspans_) {
+        hipLaunchKernelGGL(k_bforce, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(k_reactions, dim3((nspans_ + 63) / 64), dim3(64), 0, stream_, P_);
+    }
+    hip_check(hipGetLastError(), "substep launch");
+}
+
+void Engine::build_graph() {
+    invalidate_graph();
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+    enqueue_substep();
+    hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
+    hip_check(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+}
+
+void Engine::step(uint32_t ticks) {
+    if (dirty_) layout();
+    if (scenes_.empty() || P_.nblocks == 0) return;
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    // All scenes share one substep launch; a tick is `substeps` of them.
+    uint32_t sub = 1;
+    for (auto &s : scenes_) sub = std::max(sub, s.substeps);
+    for (auto &s : scenes_)
+        if (s.substeps != sub)
+            throw std::invalid_argument("scenes of one pool must share the substep count");
+    if (use_graph_ && !gexec_) build_graph();
+    const uint64_t n = (uint64_t)ticks * sub;
+    for (uint64_t k = 0; k < n; ++k) {
+        if (use_graph_)
+            hip_check(hipGraphLaunch(gexec_, stream_), "graph launch");
+        else
+            enqueue_substep();
+    }
+    substeps_done_ += n;
+}
+
+void Engine::check_error_state() {
+    hip_check(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(Control), hipMemcpyDeviceToHost, stream_), "ctl read");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    if (h_ctl_->overflow)
+        throw CapacityError("a boundary particle has more fluid neighbours than the reaction pass holds");
+    if (h_ctl_->err & kErrGrid)
+        throw std::domain_error("a scene's particle box is not finite (NaN or infinite positions)");
+    if (h_ctl_->err & kErrCellCapacity) {
+        // The failed substep and every later one left the state untouched;
+        // grow the cell tables and replay the missing substeps.
+        const uint64_t done = h_ctl_->step - ctl_step_base_;
+        const uint64_t missing = substeps_done_ - done;
+        const uint32_t new_cap = (uint32_t)std::min<uint64_t>((uint64_t)ccap_ * 2ull, 0xfffffff0ull);
+        if (new_cap <= ccap_) throw CapacityError("cell table capacity exhausted");
+        invalidate_graph();
+        dfree(P_.counts);
+        dfree(P_.start);
+        dfree(d_tile_sums_);
+        pool_allocs_.erase(std::remove_if(pool_allocs_.begin(), pool_allocs_.end(),
+                                          [&](void *p) { return p == P_.counts || p == P_.start; }),
+                           pool_allocs_.end());
+        alloc_cells(new_cap);
+        h_ctl_->err = 0;
+        hip_check(hipMemcpy(&d_ctl_->err, &h_ctl_->err, sizeof(uint32_t), hipMemcpyHostToDevice), "ctl reset");
+        substeps_done_ = done;
+        if (missing) {
+            uint32_t sub = 1;
+            for (auto &s : scenes_) sub = std::max(sub, s.substeps);
+            if (use_graph_) build_graph();
+            for (uint64_t k = 0; k < missing; ++k) {
+                if (use_graph_)
+                    hip_check(hipGraphLaunch(gexec_, stream_), "graph launch");
+                else
+                    enqueue_substep();
+            }
+            substeps_done_ += missing;
+        }
+        check_error_state();
+    }
+}
+
+void Engine::sync() {
+    if (dirty_) layout();
+    check_error_state();
+}
+
+void Engine::set_diagnostics(bool on) {
+    if (on == diagnostics_) return;
+    // Diagnostics arrays are part of the pool: force a relayout.
+    diagnostics_ = on;
+    dirty_ = true;
+}
+
+void Engine::upload_boundary(uint32_t id, uint32_t nb, const float *pos, const float *vel) {
+    if (dirty_) layout();
+    sync();
+    const SceneSlot &sc = scene(id);
+    const uint32_t n = sc.params.n, nf = sc.params.nf;
+    if (nb != n - nf) throw std::length_error("boundary segment length differs from the scene's");
+    if (nb == 0) return;
+    // Device holds cur arrays in grid order: scatter through the slot map.
+    const uint32_t pb = sc.params.pbase;
+    std::vector<uint32_t> gid(n);
+    hip_check(hipMemcpy(gid.data(), P_.cg + pb, 4ull * n, hipMemcpyDeviceToHost), "gid");
+    float *dst[6] = {P_.cx, P_.cy, P_.cz, P_.cvx, P_.cvy, P_.cvz};
+    std::vector<float> cur(n);
+    for (int a = 0; a < 6; ++a) {
+        hip_check(hipMemcpy(cur.data(), dst[a] + pb, 4ull * n, hipMemcpyDeviceToHost), "cur");
+        const float *src = a < 3 ? pos : vel;
+        for (uint32_t m = 0; m < n; ++m)
+            if (gid[m] >= nf) cur[m] = src[3 * (gid[m] - nf) + (a % 3)];
+        hip_check(hipMemcpy(dst[a] + pb, cur.data(), 4ull * n, hipMemcpyHostToDevice), "cur");
+    }
+    // Boundary motion changes the AABB of the next grid build.
+    std::vector<SceneBox> boxes(2 * scenes_.size());
+    hip_check(hipMemcpy(boxes.data(), d_box_, sizeof(SceneBox) * boxes.size(), hipMemcpyDeviceToHost), "box");
+    const uint32_t par = h_ctl_->step & 1u;
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    SceneBox &b = boxes[par * scenes_.size() + k];
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = 0xffffffffu;
+        b.hi[a] = 0u;
+    }
+    hip_check(hipMemcpy(d_box_, boxes.data(), sizeof(SceneBox) * boxes.size(), hipMemcpyHostToDevice), "box");
+    hipLaunchKernelGGL(k_box, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_);
+    hip_check(hipGetLastError(), "k_box");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+}
+
+void Engine::read_state(uint32_t id, float *pos, float *vel) {
+    sync();
+    const SceneSlot &sc = scene(id);
+    const uint32_t n = sc.params.n, pb = sc.params.pbase;
+    float *o = d_out_f_;
+    hipLaunchKernelGGL(k_unsort, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, o, o + slots_, o + 2ull * slots_,
+                       o + 3ull * slots_, o + 4ull * slots_, o + 5ull * slots_);
+    hip_check(hipGetLastError(), "k_unsort");
+    std::vector<float> tmp(6ull * n);
+    for (int a = 0; a < 6; ++a)
+        hip_check(hipMemcpyAsync(tmp.data() + (size_t)a * n, o + (size_t)a * slots_ + pb, 4ull * n,
+                                 hipMemcpyDeviceToHost, stream_),
+                  "read");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            if (pos) pos[3 * i + a] = tmp[(size_t)a * n + i];
+            if (vel) vel[3 * i + a] = tmp[(size_t)(3 + a) * n + i];
+        }
+}
+
+void Engine::read_fields(uint32_t id, float *dens, float *press, uint32_t *ncount, float *acc, float *xsph) {
+    sync();
+    if ((ncount || acc || xsph) && !diagnostics_)
+        throw std::logic_error("per-particle diagnostics are off (enable before stepping)");
+    const SceneSlot &sc = scene(id);
+    const uint32_t n = sc.params.n, nf = sc.params.nf, pb = sc.params.pbase;
+    float *f = d_out_fields_;
+    float *od = f, *op = f + slots_, *oa = f + 2ull * slots_, *ox = f + 5ull * slots_;
+    hipLaunchKernelGGL(k_unsort_fields, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, od, op,
+                       diagnostics_ ? d_out_u_ : nullptr, diagnostics_ ? oa : nullptr, diagnostics_ ? ox : nullptr);
+    hip_check(hipGetLastError(), "k_unsort_fields");
+    if (dens) hip_check(hipMemcpyAsync(dens, od + pb, 4ull * n, hipMemcpyDeviceToHost, stream_), "dens");
+    if (press) hip_check(hipMemcpyAsync(press, op + pb, 4ull * n, hipMemcpyDeviceToHost, stream_), "press");
+    if (ncount) hip_check(hipMemcpyAsync(ncount, d_out_u_ + pb, 4ull * n, hipMemcpyDeviceToHost, stream_), "ncount");
+    if (acc) hip_check(hipMemcpyAsync(acc, oa + 3ull * pb, 12ull * nf, hipMemcpyDeviceToHost, stream_), "acc");
+    if (xsph) hip_check(hipMemcpyAsync(xsph, ox + 3ull * pb, 12ull * nf, hipMemcpyDeviceToHost, stream_), "xsph");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+}
+
+SceneGrid Engine::read_grid(uint32_t id, std::vector<uint32_t> *start, std::vector<uint32_t> *items) {
+    sync();
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    const SceneSlot &sc = scenes_[k];
+    SceneGrid g{};
+    hip_check(hipMemcpy(&g, d_grid_ + k, sizeof(SceneGrid), hipMemcpyDeviceToHost), "grid");
+    const uint32_t pb = sc.params.pbase, n = sc.params.n;
+    if (start) {
+        start->resize((size_t)g.total + 1);
+        hip_check(hipMemcpy(start->data(), P_.start + g.cbase, 4ull * (g.total + 1), hipMemcpyDeviceToHost), "start");
+        for (auto &v : *start) v -= pb;
+    }
+    if (items) {
+        items->resize(n);
+        hip_check(hipMemcpy(items->data(), P_.sg + pb, 4ull * n, hipMemcpyDeviceToHost), "items");
+    }
+    return g;
+}
+
+void Engine::read_bounds(uint32_t id, float lo[3], float hi[3]) {
+    sync();
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    SceneBox b{};
+    const uint32_t par = h_ctl_->step & 1u;
+    hip_check(hipMemcpy(&b, d_box_ + par * scenes_.size() + k, sizeof(SceneBox), hipMemcpyDeviceToHost), "box");
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = o2f(b.lo[a]);
+        hi[a] = o2f(b.hi[a]);
+    }
+}
+
+std::vector<float> Engine::read_reactions(uint32_t id) {
+    sync();
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    const SceneSlot &sc = scenes_[k];
+    std::vector<float> out(6 * sc.spans.size(), 0.0f);
+    if (!sc.want_reactions || sc.spans.empty() || substeps_done_ == 0) return out;
+    hip_check(hipMemcpy(out.data(), d_react_ + 6ull * span_first_[k], 4 * out.size(), hipMemcpyDeviceToHost),
+              "reactions");
+    return out;
+}
+
+}  // namespace vxl

@@ -1,0 +1,93 @@
+// Plain-old-data layout shared by the host runtime and the gfx950 kernels.
+//
+// Pool layout (device, SoA f32/u32):
+//   * every scene owns the slot range [pbase, pbase + alloc) of the particle
+//     pool, alloc = n rounded up to kBlock, so a block of kBlock slots always
+//     belongs to exactly one scene (block_scene[] maps block -> scene);
+//   * "cur" arrays hold the state in the grid order of the previous substep;
+//     "srt" arrays hold the state gathered into the current grid order;
+//   * gid is the scene-local original particle index (fluid prefix first),
+//     the key the reference orders every cell by (fluid grid.rs:53-75);
+//   * every scene owns the cell range [cbase, cbase + total + 1) of the cell
+//     tables: total grid cells plus one pad cell that absorbs the unused
+//     slots of the scene, so one global exclusive scan yields per-scene starts
+//     that begin exactly at pbase.
+#ifndef VXL_SPH_LAYOUT_H
+#define VXL_SPH_LAYOUT_H
+#include <stdint.h>
+
+namespace vxl {
+
+constexpr uint32_t kBlock = 256;        // slots per block (4 wave64)
+constexpr uint32_t kGridDimMax = 1u << 14;  // core/grid.rs:8
+
+// Host-computed constants of one scene (FluidSystem::new, fluid_access.rs:8-13,
+// plus the substep constants of fluid_step.rs:110-113). Values are computed
+// once on the host in strict f32 and passed bit-for-bit.
+struct SceneParams {
+    uint32_t pbase;      // first pool slot
+    uint32_t n;          // particles (fluid + boundary)
+    uint32_t nf;         // fluid prefix length
+    uint32_t alloc;      // slots owned (multiple of kBlock)
+    uint64_t max_bins;   // GRID_MAX_BINS budget of the grid rule
+    float h, h2, k6, ks, w0;
+    float mass;          // FluidSystem::mass (fluid density sum scale)
+    float rho0, b_tait, alpha_c, eps;
+    float gx, gy, gz;
+    float dt, vmax, vmax2;
+    uint32_t clamp_neg;  // tensile_instability_suppression
+    uint32_t react;      // compute boundary reactions for this scene's spans
+};
+
+// A boundary span of a scene with reactions (fluid_boundary.rs:10-31):
+// pool slots [start, end) = pbase + local [start, end).
+struct SpanDev {
+    uint32_t scene;
+    uint32_t start, end;
+    uint32_t substeps;   // tick = substeps substeps; reactions are tick averages
+    float origin[3];
+    uint32_t pad;
+};
+
+constexpr uint32_t kReactMaxNeighbours = 96;  // fluid neighbours per boundary particle
+
+// Device-written per-substep grid of one scene (grid_box rule, core/grid.rs:34-51).
+struct SceneGrid {
+    float gmin[3];
+    float inv;
+    float bin;
+    uint32_t dims[3];
+    uint32_t total;      // dims product (<= max_bins)
+    uint32_t cbase;      // first cell of this scene in the global tables
+    uint32_t pad[2];
+};
+
+// Ordered-u32 AABB accumulator (min over lo, max over hi).
+struct SceneBox {
+    uint32_t lo[3];
+    uint32_t hi[3];
+};
+
+// Device-side control block.
+struct Control {
+    uint32_t step;       // substep counter: AABB ping-pong parity
+    uint32_t ctot;       // cells used this substep (sum of total + 1)
+    uint32_t err;        // nonzero: the substep was abandoned (see kErr*)
+    uint32_t overflow;   // reactions: a boundary particle exceeded kReactMaxNeighbours
+};
+
+constexpr uint32_t kErrCellCapacity = 1u;
+constexpr uint32_t kErrGrid = 2u;  // non-finite particle box
+
+// Order-preserving map f32 -> u32 (total order; -0 < +0).
+__host__ __device__ inline uint32_t f2o(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float o2f(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __builtin_bit_cast(float, u);
+}
+
+}  // namespace vxl
+#endif

@@ -1,0 +1,440 @@
+// extern "C" boundary of the WCSPH library (include/vxl_sph.h). Every entry
+// catches C++ exceptions and turns them into status codes; nothing throws
+// across the boundary (ffi/src/lib.rs:1-8).
+#include "../../include/vxl_sph.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "sph_engine.h"
+
+using namespace vxl;
+
+struct VxlSphEngine {
+    Engine eng;
+    explicit VxlSphEngine(int dev) : eng(dev) {}
+};
+
+struct VxlSphStepper {
+    Engine eng;
+    SceneConfig cfg;
+    uint32_t scene = 0;
+    bool started = false;
+    uint32_t n = 0, nf = 0;
+    std::vector<Span> spans;
+    std::vector<VxlSphReaction> reactions;
+    VxlSphStepper(int dev, const SceneConfig &c) : eng(dev), cfg(c) {}
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+VxlSphStatus fail(VxlSphStatus s, const std::string &msg) {
+    g_last_error = msg;
+    return s;
+}
+
+template <class F>
+VxlSphStatus guard(F &&f) {
+    try {
+        f();
+        return VXL_SPH_OK;
+    } catch (const CapacityError &e) {
+        return fail(VXL_SPH_CAPACITY, e.what());
+    } catch (const DeviceError &e) {
+        if (e.code == hipErrorOutOfMemory) return fail(VXL_SPH_OUT_OF_MEMORY, e.what());
+        return fail(VXL_SPH_DEVICE_ERROR, e.what());
+    } catch (const std::length_error &e) {
+        return fail(VXL_SPH_SIZE_MISMATCH, e.what());
+    } catch (const std::out_of_range &e) {
+        return fail(VXL_SPH_NOT_FOUND, e.what());
+    } catch (const std::domain_error &e) {
+        return fail(VXL_SPH_UNSUPPORTED, e.what());
+    } catch (const std::invalid_argument &e) {
+        return fail(VXL_SPH_INVALID_ARGUMENT, e.what());
+    } catch (const std::logic_error &e) {
+        return fail(VXL_SPH_NOT_READY, e.what());
+    } catch (const std::bad_alloc &e) {
+        return fail(VXL_SPH_OUT_OF_MEMORY, e.what());
+    } catch (const std::exception &e) {
+        return fail(VXL_SPH_PANIC_CAUGHT, e.what());
+    } catch (...) {
+        return fail(VXL_SPH_PANIC_CAUGHT, "unknown exception");
+    }
+}
+
+VxlSphStatus check_header(const VxlSphHeader &h, uint32_t size) {
+    if (h.version != VXL_SPH_ABI_VERSION) return fail(VXL_SPH_VERSION_MISMATCH, "block version mismatch");
+    if (h.size != size) return fail(VXL_SPH_SIZE_MISMATCH, "block size mismatch");
+    return VXL_SPH_OK;
+}
+
+SceneConfig to_scene_config(const VxlSphFluidConfig &c) {
+    SceneConfig s;
+    s.rest_density = c.rest_density;
+    s.smoothing_radius = c.smoothing_radius;
+    s.xsph_viscosity = c.xsph_viscosity;
+    s.sound_speed = c.sound_speed;
+    s.gamma_tait = c.gamma_tait;
+    s.substeps = c.substeps;
+    s.tensile_clamp = c.tensile_clamp;
+    s.boundary_layers = c.boundary_layers;
+    for (int a = 0; a < 3; ++a) s.gravity[a] = c.gravity[a];
+    s.max_speed_frac = c.max_speed_frac;
+    s.artificial_viscosity = c.artificial_viscosity;
+    s.particle_mass = c.particle_mass;
+    s.spacing = c.spacing;
+    s.max_bins = c.max_bins ? c.max_bins : (1ull << 21);
+    s.dt_tick = c.dt_tick;
+    return s;
+}
+
+std::vector<Span> to_spans(const VxlSphSpan *spans, uint32_t nspans) {
+    std::vector<Span> v(nspans);
+    for (uint32_t k = 0; k < nspans; ++k) {
+        v[k].body = spans[k].body;
+        for (int a = 0; a < 3; ++a) v[k].origin[a] = spans[k].origin[a];
+        v[k].start = spans[k].start;
+        v[k].end = spans[k].end;
+    }
+    return v;
+}
+
+uint64_t fnv_state(const float *pos, const float *vel, uint32_t nf) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        h ^= (uint64_t)u;
+        h *= 0x00000100000001b3ull;
+    };
+    for (uint32_t i = 0; i < nf; ++i) {
+        mix(pos[3 * i]); mix(pos[3 * i + 1]); mix(pos[3 * i + 2]);
+        mix(vel[3 * i]); mix(vel[3 * i + 1]); mix(vel[3 * i + 2]);
+    }
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t vxl_sph_abi_version(void) { return VXL_SPH_ABI_VERSION; }
+
+const char *vxl_sph_backend_name(void) { return "hip-gfx950"; }
+
+const char *vxl_sph_last_error(void) { return g_last_error.c_str(); }
+
+VxlSphStatus vxl_sph_device_count(int32_t *out) {
+    if (!out) return fail(VXL_SPH_NULL_POINTER, "out is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *out = n;
+    return VXL_SPH_OK;
+}
+
+int32_t vxl_sph_backend_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int32_t vxl_sph_backend_submit(const char *tag) {
+    if (!tag || std::strcmp(tag, "fluid") != 0) return 0;
+    return vxl_sph_backend_available();
+}
+
+VxlSphStatus vxl_sph_default_config(VxlSphFluidConfig *out) {
+    if (!out) return fail(VXL_SPH_NULL_POINTER, "out is null");
+    SceneConfig d;
+    std::memset(out, 0, sizeof(*out));
+    out->hdr.size = sizeof(VxlSphFluidConfig);
+    out->hdr.version = VXL_SPH_ABI_VERSION;
+    out->rest_density = d.rest_density;
+    out->smoothing_radius = d.smoothing_radius;
+    out->xsph_viscosity = d.xsph_viscosity;
+    out->sound_speed = d.sound_speed;
+    out->gamma_tait = d.gamma_tait;
+    out->substeps = d.substeps;
+    out->tensile_clamp = d.tensile_clamp;
+    out->boundary_layers = d.boundary_layers;
+    for (int a = 0; a < 3; ++a) out->gravity[a] = d.gravity[a];
+    out->max_speed_frac = d.max_speed_frac;
+    out->artificial_viscosity = d.artificial_viscosity;
+    out->particle_mass = d.particle_mass;
+    out->spacing = d.spacing;
+    out->max_bins = d.max_bins;
+    out->dt_tick = d.dt_tick;
+    return VXL_SPH_OK;
+}
+
+VxlSphStatus vxl_sph_engine_create(uint32_t abi_version, int32_t device, VxlSphEngine **out) {
+    if (!out) return fail(VXL_SPH_NULL_POINTER, "out is null");
+    *out = nullptr;
+    if (abi_version != VXL_SPH_ABI_VERSION) return fail(VXL_SPH_VERSION_MISMATCH, "ABI version mismatch");
+    return guard([&] { *out = new VxlSphEngine(device); });
+}
+
+VxlSphStatus vxl_sph_engine_destroy(VxlSphEngine *e) {
+    delete e;
+    return VXL_SPH_OK;
+}
+
+VxlSphStatus vxl_sph_engine_set_diagnostics(VxlSphEngine *e, int32_t on) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] { e->eng.set_diagnostics(on != 0); });
+}
+
+VxlSphStatus vxl_sph_engine_set_graphs(VxlSphEngine *e, int32_t on) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] { e->eng.set_graphs(on != 0); });
+}
+
+VxlSphStatus vxl_sph_engine_step(VxlSphEngine *e, uint32_t ticks) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] { e->eng.step(ticks); });
+}
+
+VxlSphStatus vxl_sph_engine_sync(VxlSphEngine *e) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] { e->eng.sync(); });
+}
+
+VxlSphStatus vxl_sph_engine_device_bytes(VxlSphEngine *e, uint64_t *out) {
+    if (!e || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    *out = e->eng.device_bytes();
+    return VXL_SPH_OK;
+}
+
+VxlSphStatus vxl_sph_scene_add(VxlSphEngine *e, const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid,
+                               const float *pos, const float *vel, const float *pmass, const VxlSphSpan *spans,
+                               uint32_t nspans, uint32_t *out_scene) {
+    if (!e || !cfg || !pos || !out_scene || (nspans && !spans)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        const SceneConfig c = to_scene_config(*cfg);
+        const std::vector<Span> sp = to_spans(spans, nspans);
+        *out_scene = e->eng.add_scene(c, n, n_fluid, pos, vel, pmass, sp.data(), nspans);
+    });
+}
+
+VxlSphStatus vxl_sph_scene_remove(VxlSphEngine *e, uint32_t scene) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] { e->eng.remove_scene(scene); });
+}
+
+VxlSphStatus vxl_sph_scene_upload_boundary(VxlSphEngine *e, uint32_t scene, uint32_t nb, const float *pos,
+                                           const float *vel) {
+    if (!e || (nb && (!pos || !vel))) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] { e->eng.upload_boundary(scene, nb, pos, vel); });
+}
+
+VxlSphStatus vxl_sph_scene_read_state(VxlSphEngine *e, uint32_t scene, uint32_t n, float *pos, float *vel) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] {
+        if (n != e->eng.scene(scene).params.n) throw std::length_error("n differs from the scene's particle count");
+        e->eng.read_state(scene, pos, vel);
+    });
+}
+
+VxlSphStatus vxl_sph_scene_read_fields(VxlSphEngine *e, uint32_t scene, uint32_t n, float *dens, float *press,
+                                       uint32_t *ncount, float *acc, float *xsph) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] {
+        if (n != e->eng.scene(scene).params.n) throw std::length_error("n differs from the scene's particle count");
+        e->eng.read_fields(scene, dens, press, ncount, acc, xsph);
+    });
+}
+
+VxlSphStatus vxl_sph_scene_read_grid(VxlSphEngine *e, uint32_t scene, VxlSphGridInfo *info, uint32_t start_cap,
+                                     uint32_t *start, uint32_t n, uint32_t *items, uint32_t *keys) {
+    if (!e || !info) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        const SceneSlot &sc = e->eng.scene(scene);
+        std::vector<uint32_t> st, it;
+        const SceneGrid g = e->eng.read_grid(scene, &st, &it);
+        for (int a = 0; a < 3; ++a) {
+            info->min[a] = g.gmin[a];
+            info->dims[a] = g.dims[a];
+        }
+        info->inv = g.inv;
+        info->bin = g.bin;
+        info->total = g.total;
+        if (start) {
+            if (start_cap < g.total + 1) throw std::length_error("start buffer smaller than total + 1");
+            std::memcpy(start, st.data(), 4ull * (g.total + 1));
+        }
+        if (items || keys) {
+            if (n != sc.params.n) throw std::length_error("n differs from the scene's particle count");
+            if (items) std::memcpy(items, it.data(), 4ull * n);
+            if (keys)
+                for (uint32_t c = 0; c < g.total; ++c)
+                    for (uint32_t k = st[c]; k < st[c + 1]; ++k) keys[it[k]] = c;
+        }
+    });
+}
+
+VxlSphStatus vxl_sph_scene_read_bounds(VxlSphEngine *e, uint32_t scene, float lo[3], float hi[3]) {
+    if (!e || !lo || !hi) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] { e->eng.read_bounds(scene, lo, hi); });
+}
+
+VxlSphStatus vxl_sph_scene_state_hash(VxlSphEngine *e, uint32_t scene, uint64_t *out) {
+    if (!e || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        const SceneSlot &sc = e->eng.scene(scene);
+        std::vector<float> p(3ull * sc.params.n), v(3ull * sc.params.n);
+        e->eng.read_state(scene, p.data(), v.data());
+        *out = fnv_state(p.data(), v.data(), sc.params.nf);
+    });
+}
+
+// ---- FluidStepper drop-in ----
+
+VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,
+                                    VxlSphStepper **out) {
+    if (!out || !cfg) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    *out = nullptr;
+    if (abi_version != VXL_SPH_ABI_VERSION) return fail(VXL_SPH_VERSION_MISMATCH, "ABI version mismatch");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] { *out = new VxlSphStepper(device, to_scene_config(*cfg)); });
+}
+
+VxlSphStatus vxl_sph_stepper_destroy(VxlSphStepper *s) {
+    delete s;
+    return VXL_SPH_OK;
+}
+
+VxlSphStatus vxl_sph_stepper_step(VxlSphStepper *s, float dt_tick, uint32_t n_fluid, uint32_t n, const float *pos,
+                                  const float *vel, const float *pmass, const VxlSphSpan *spans, uint32_t nspans) {
+    if (!s || !pos || !vel || !pmass || (nspans && !spans)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        if (!s->started) {
+            s->cfg.dt_tick = dt_tick;
+            s->spans = to_spans(spans, nspans);
+            s->scene = s->eng.add_scene(s->cfg, n, n_fluid, pos, vel, pmass, s->spans.data(), nspans);
+            s->n = n;
+            s->nf = n_fluid;
+            s->started = true;
+        } else {
+            // pipeline/stepper.rs:80-96: fixed particle counts; pmass length-checked only.
+            if (n_fluid != s->nf) throw std::length_error("fluid prefix length differs from the stepper's");
+            if (n != s->n) throw std::length_error("particle count differs from the stepper's");
+            if (dt_tick != s->cfg.dt_tick) throw std::invalid_argument("tick length differs from the first step's");
+            s->eng.upload_boundary(s->scene, n - n_fluid, pos + 3ull * n_fluid, vel + 3ull * n_fluid);
+        }
+        s->eng.step(1);
+        const std::vector<float> r = s->eng.read_reactions(s->scene);
+        s->reactions.resize(s->spans.size());
+        for (size_t k = 0; k < s->spans.size(); ++k) {
+            s->reactions[k].body = s->spans[k].body;
+            for (int a = 0; a < 3; ++a) {
+                s->reactions[k].force[a] = r[6 * k + a];
+                s->reactions[k].torque[a] = r[6 * k + 3 + a];
+            }
+        }
+    });
+}
+
+VxlSphStatus vxl_sph_stepper_reactions(VxlSphStepper *s, uint32_t cap, VxlSphReaction *out, uint32_t *count) {
+    if (!s || !count || (cap && !out)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    *count = (uint32_t)s->reactions.size();
+    for (uint32_t k = 0; k < cap && k < *count; ++k) out[k] = s->reactions[k];
+    return VXL_SPH_OK;
+}
+
+VxlSphStatus vxl_sph_stepper_bounds(VxlSphStepper *s, float lo[3], float hi[3]) {
+    if (!s || !lo || !hi) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    if (!s->started) return fail(VXL_SPH_NOT_READY, "no step taken yet");
+    return guard([&] { s->eng.read_bounds(s->scene, lo, hi); });
+}
+
+VxlSphStatus vxl_sph_stepper_read_state(VxlSphStepper *s, uint32_t n, float *pos, float *vel) {
+    if (!s) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    if (!s->started) return fail(VXL_SPH_NOT_READY, "no step taken yet");
+    return guard([&] {
+        if (n != s->n) throw std::length_error("n differs from the stepper's particle count");
+        s->eng.read_state(s->scene, pos, vel);
+    });
+}
+
+// ---- scene construction ----
+
+VxlSphStatus vxl_sph_box_lattice(const float half[3], float sp, float h, uint32_t layers, uint32_t cap,
+                                 float *out_pts, uint32_t *out_n, uint32_t *out_n0, float *out_volume) {
+    if (!half || !out_n) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        std::vector<float> pts;
+        uint32_t n0 = 0;
+        float vol = 0.0f;
+        const uint32_t n = box_lattice(half, sp, h, layers, pts, n0, vol);
+        *out_n = n;
+        if (out_n0) *out_n0 = n0;
+        if (out_volume) *out_volume = vol;
+        if (out_pts) std::memcpy(out_pts, pts.data(), 12ull * std::min(n, cap));
+    });
+}
+
+VxlSphStatus vxl_sph_fluid_constants(const VxlSphFluidConfig *cfg, float out[8]) {
+    if (!cfg || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        const FluidConstants c = fluid_constants(to_scene_config(*cfg));
+        out[0] = c.h; out[1] = c.h2; out[2] = c.k6; out[3] = c.ks;
+        out[4] = c.w0; out[5] = c.b_tait; out[6] = c.mass; out[7] = c.alpha_c;
+    });
+}
+
+VxlSphStatus vxl_sph_dam_break(const VxlSphDamBreak *spec, const VxlSphFluidConfig *cfg, uint32_t *n,
+                               uint32_t *n_fluid, uint32_t *nspans, float *pos, float *vel, float *pmass,
+                               VxlSphSpan *spans) {
+    if (!spec || !cfg || !n || !n_fluid || !nspans) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(spec->hdr, sizeof(VxlSphDamBreak));
+    if (st != VXL_SPH_OK) return st;
+    st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        DamBreakSpec ds;
+        for (int a = 0; a < 3; ++a) {
+            ds.tank[a] = spec->tank[a];
+            ds.block[a] = spec->block[a];
+        }
+        ds.d = spec->d;
+        ds.jitter = spec->jitter;
+        ds.seed = spec->seed;
+        ds.walls = spec->walls != 0;
+        std::vector<float> p, v, m;
+        std::vector<Span> sp;
+        uint32_t nf = 0;
+        dam_break(ds, to_scene_config(*cfg), p, v, m, sp, nf);
+        const uint32_t count = (uint32_t)(m.size());
+        const bool fill = pos != nullptr;
+        if (fill && (*n < count || *nspans < sp.size())) throw std::length_error("output buffers too small");
+        *n = count;
+        *n_fluid = nf;
+        *nspans = (uint32_t)sp.size();
+        if (!fill) return;
+        std::memcpy(pos, p.data(), 4 * p.size());
+        if (vel) std::memcpy(vel, v.data(), 4 * v.size());
+        if (pmass) std::memcpy(pmass, m.data(), 4 * m.size());
+        if (spans)
+            for (size_t k = 0; k < sp.size(); ++k) {
+                spans[k].body = sp[k].body;
+                for (int a = 0; a < 3; ++a) spans[k].origin[a] = sp[k].origin[a];
+                spans[k].start = sp[k].start;
+                spans[k].end = sp[k].end;
+            }
+    });
+}
+
+}  // extern "C"

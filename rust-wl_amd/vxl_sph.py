@@ -1,0 +1,434 @@
+"""Python host mirror of the MI355X WCSPH C ABI (include/vxl_sph.h).
+
+Mirrors the reference engine's interfaces for the fluid path:
+  * ``Backend``      — PhysGpuBackend (crates/vxl-phys-gpu/src/lib.rs:20-29)
+  * ``FluidStepper`` — FluidStepper / GpuFluidStepper
+                       (crates/vxl-phys-core/src/interop.rs:293-316,
+                        crates/vxl-phys-gpu/src/pipeline/stepper.rs:70-147)
+  * ``Engine`` / ``Scene`` — FluidSystem step + accessors over a pool of
+                       independent scenes (crates/vxl-phys-fluid/src/fluid_step.rs,
+                       fluid_access.rs)
+
+Everything calls libvxl_sph.so (hand-written gfx950 kernels). There is no CPU
+fallback: if the library is missing or no device is visible the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvxl_sph.so")
+ABI_VERSION = 1
+
+STATUS = {
+    0: "OK", 1: "NULL_POINTER", 2: "VERSION_MISMATCH", 3: "SIZE_MISMATCH", 4: "NOT_READY",
+    5: "PANIC_CAUGHT", 16: "INVALID_ARGUMENT", 17: "UNSUPPORTED", 18: "DEVICE_ERROR",
+    19: "OUT_OF_MEMORY", 20: "CAPACITY", 21: "NOT_FOUND",
+}
+
+
+class VxlSphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+        self.status = STATUS.get(code, str(code))
+
+
+class Header(C.Structure):
+    _fields_ = [("size", C.c_uint32), ("version", C.c_uint32)]
+
+
+class FluidConfig(C.Structure):
+    """VxlSphFluidConfig = FluidConfig (fluid/src/config.rs:6-69) + scene values."""
+    _fields_ = [
+        ("hdr", Header),
+        ("rest_density", C.c_float),
+        ("smoothing_radius", C.c_float),
+        ("xsph_viscosity", C.c_float),
+        ("sound_speed", C.c_float),
+        ("gamma_tait", C.c_float),
+        ("substeps", C.c_uint32),
+        ("tensile_clamp", C.c_uint32),
+        ("boundary_layers", C.c_uint32),
+        ("gravity", C.c_float * 3),
+        ("max_speed_frac", C.c_float),
+        ("artificial_viscosity", C.c_float),
+        ("particle_mass", C.c_float),
+        ("spacing", C.c_float),
+        ("max_bins", C.c_uint64),
+        ("dt_tick", C.c_float),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class Span(C.Structure):
+    _fields_ = [("body", C.c_uint32), ("origin", C.c_float * 3), ("start", C.c_uint32), ("end", C.c_uint32)]
+
+
+class Reaction(C.Structure):
+    _fields_ = [("body", C.c_uint32), ("force", C.c_float * 3), ("torque", C.c_float * 3)]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [("min", C.c_float * 3), ("inv", C.c_float), ("bin", C.c_float), ("dims", C.c_uint32 * 3),
+                ("total", C.c_uint32)]
+
+
+class DamBreakSpec(C.Structure):
+    _fields_ = [("hdr", Header), ("tank", C.c_float * 3), ("block", C.c_uint32 * 3), ("d", C.c_float),
+                ("jitter", C.c_float), ("seed", C.c_uint64), ("walls", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+_lib = None
+
+# Every symbol declared in include/vxl_sph.h.
+EXPORTS = (
+    "vxl_sph_abi_version", "vxl_sph_backend_name", "vxl_sph_backend_available", "vxl_sph_backend_submit",
+    "vxl_sph_device_count", "vxl_sph_last_error", "vxl_sph_default_config",
+    "vxl_sph_engine_create", "vxl_sph_engine_destroy", "vxl_sph_engine_set_diagnostics",
+    "vxl_sph_engine_set_graphs", "vxl_sph_engine_step", "vxl_sph_engine_sync", "vxl_sph_engine_device_bytes",
+    "vxl_sph_scene_add", "vxl_sph_scene_remove", "vxl_sph_scene_upload_boundary", "vxl_sph_scene_read_state",
+    "vxl_sph_scene_read_fields", "vxl_sph_scene_read_grid", "vxl_sph_scene_read_bounds",
+    "vxl_sph_scene_state_hash",
+    "vxl_sph_stepper_create", "vxl_sph_stepper_destroy", "vxl_sph_stepper_step", "vxl_sph_stepper_reactions",
+    "vxl_sph_stepper_bounds", "vxl_sph_stepper_read_state",
+    "vxl_sph_box_lattice", "vxl_sph_fluid_constants", "vxl_sph_dam_break",
+)
+
+
+def lib():
+    """Load libvxl_sph.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    up = C.POINTER(C.c_uint32)
+    st = C.c_int
+    L.vxl_sph_abi_version.restype = C.c_uint32
+    L.vxl_sph_backend_name.restype = C.c_char_p
+    L.vxl_sph_backend_available.restype = C.c_int32
+    L.vxl_sph_backend_submit.restype = C.c_int32
+    L.vxl_sph_backend_submit.argtypes = [C.c_char_p]
+    L.vxl_sph_last_error.restype = C.c_char_p
+    L.vxl_sph_device_count.restype = st
+    L.vxl_sph_device_count.argtypes = [C.POINTER(C.c_int32)]
+    L.vxl_sph_default_config.restype = st
+    L.vxl_sph_default_config.argtypes = [C.POINTER(FluidConfig)]
+    L.vxl_sph_engine_create.restype = st
+    L.vxl_sph_engine_create.argtypes = [C.c_uint32, C.c_int32, C.POINTER(P)]
+    for name in ("vxl_sph_engine_destroy", "vxl_sph_engine_sync"):
+        getattr(L, name).restype = st
+        getattr(L, name).argtypes = [P]
+    for name in ("vxl_sph_engine_set_diagnostics", "vxl_sph_engine_set_graphs"):
+        getattr(L, name).restype = st
+        getattr(L, name).argtypes = [P, C.c_int32]
+    L.vxl_sph_engine_step.restype = st
+    L.vxl_sph_engine_step.argtypes = [P, C.c_uint32]
+    L.vxl_sph_engine_device_bytes.restype = st
+    L.vxl_sph_engine_device_bytes.argtypes = [P, C.POINTER(C.c_uint64)]
+    L.vxl_sph_scene_add.restype = st
+    L.vxl_sph_scene_add.argtypes = [P, C.POINTER(FluidConfig), C.c_uint32, C.c_uint32, fp, fp, fp,
+                                    C.POINTER(Span), C.c_uint32, up]
+    L.vxl_sph_scene_remove.restype = st
+    L.vxl_sph_scene_remove.argtypes = [P, C.c_uint32]
+    L.vxl_sph_scene_upload_boundary.restype = st
+    L.vxl_sph_scene_upload_boundary.argtypes = [P, C.c_uint32, C.c_uint32, fp, fp]
+    L.vxl_sph_scene_read_state.restype = st
+    L.vxl_sph_scene_read_state.argtypes = [P, C.c_uint32, C.c_uint32, fp, fp]
+    L.vxl_sph_scene_read_fields.restype = st
+    L.vxl_sph_scene_read_fields.argtypes = [P, C.c_uint32, C.c_uint32, fp, fp, up, fp, fp]
+    L.vxl_sph_scene_read_grid.restype = st
+    L.vxl_sph_scene_read_grid.argtypes = [P, C.c_uint32, C.POINTER(GridInfo), C.c_uint32, up, C.c_uint32, up, up]
+    L.vxl_sph_scene_read_bounds.restype = st
+    L.vxl_sph_scene_read_bounds.argtypes = [P, C.c_uint32, fp, fp]
+    L.vxl_sph_scene_state_hash.restype = st
+    L.vxl_sph_scene_state_hash.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint64)]
+    L.vxl_sph_stepper_create.restype = st
+    L.vxl_sph_stepper_create.argtypes = [C.c_uint32, C.POINTER(FluidConfig), C.c_int32, C.POINTER(P)]
+    L.vxl_sph_stepper_destroy.restype = st
+    L.vxl_sph_stepper_destroy.argtypes = [P]
+    L.vxl_sph_stepper_step.restype = st
+    L.vxl_sph_stepper_step.argtypes = [P, C.c_float, C.c_uint32, C.c_uint32, fp, fp, fp, C.POINTER(Span),
+                                       C.c_uint32]
+    L.vxl_sph_stepper_reactions.restype = st
+    L.vxl_sph_stepper_reactions.argtypes = [P, C.c_uint32, C.POINTER(Reaction), up]
+    L.vxl_sph_stepper_bounds.restype = st
+    L.vxl_sph_stepper_bounds.argtypes = [P, fp, fp]
+    L.vxl_sph_stepper_read_state.restype = st
+    L.vxl_sph_stepper_read_state.argtypes = [P, C.c_uint32, fp, fp]
+    L.vxl_sph_box_lattice.restype = st
+    L.vxl_sph_box_lattice.argtypes = [fp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, fp, up, up, fp]
+    L.vxl_sph_fluid_constants.restype = st
+    L.vxl_sph_fluid_constants.argtypes = [C.POINTER(FluidConfig), fp]
+    L.vxl_sph_dam_break.restype = st
+    L.vxl_sph_dam_break.argtypes = [C.POINTER(DamBreakSpec), C.POINTER(FluidConfig), up, up, up, fp, fp, fp,
+                                    C.POINTER(Span)]
+    _lib = L
+    return L
+
+
+def _check(code):
+    if code != 0:
+        raise VxlSphError(code, (lib().vxl_sph_last_error() or b"").decode(errors="replace"))
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _u(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32)) if a is not None else None
+
+
+def fluid_config(**kw):
+    """FluidConfig::default() (config.rs:51-69) with overrides."""
+    c = FluidConfig()
+    _check(lib().vxl_sph_default_config(C.byref(c)))
+    for k, v in kw.items():
+        if k == "gravity":
+            c.gravity = (C.c_float * 3)(*v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def dam_break_config(d, c0=20.0, alpha=0.02, eps=0.5, dt=5e-4, layers=1, max_bins=1 << 21):
+    """Physics of the dam-break scenes: h = 2d, mass = 1000 d^3, Tait gamma 7,
+    one substep per tick of length dt."""
+    d = np.float32(d)
+    rho0 = np.float32(1000.0)
+    mass = np.float32(np.float32(np.float32(rho0 * d) * d) * d)
+    return fluid_config(smoothing_radius=float(np.float32(2.0) * d), sound_speed=c0, artificial_viscosity=alpha,
+                        xsph_viscosity=eps, substeps=1, boundary_layers=layers, particle_mass=float(mass),
+                        spacing=float(d), max_bins=max_bins, dt_tick=dt)
+
+
+def fluid_constants(cfg):
+    o = np.zeros(8, np.float32)
+    _check(lib().vxl_sph_fluid_constants(C.byref(cfg), _f(o)))
+    return dict(zip(["h", "h2", "k6", "ks", "w0", "b_tait", "mass", "alpha_c"], o))
+
+
+def box_lattice(half, s, h, layers):
+    """boundary::lattice for Shape::Box (boundary.rs:67-88)."""
+    half = np.asarray(half, np.float32)
+    n = C.c_uint32()
+    _check(lib().vxl_sph_box_lattice(_f(half), C.c_float(s), C.c_float(h), layers, 0, None, C.byref(n), None, None))
+    pts = np.zeros((max(n.value, 1), 3), np.float32)
+    n0 = C.c_uint32()
+    vol = C.c_float()
+    _check(lib().vxl_sph_box_lattice(_f(half), C.c_float(s), C.c_float(h), layers, n.value, _f(pts), C.byref(n),
+                                     C.byref(n0), C.byref(vol)))
+    return pts[:n.value], n0.value, np.float32(vol.value)
+
+
+class SceneData:
+    """All particles of a scene in index order (fluid prefix first)."""
+
+    def __init__(self, pos, vel, pmass, n_fluid, spans):
+        self.pos = pos
+        self.vel = vel
+        self.pmass = pmass
+        self.n_fluid = n_fluid
+        self.spans = spans  # list of (body, origin(3), start, end)
+
+    @property
+    def n(self):
+        return len(self.pmass)
+
+
+def dam_break(cfg, tank, block, d, jitter=0.05, seed=1, walls=True):
+    """Seeded dam-break scene built by the library (jittered lattice + Akinci tank)."""
+    spec = DamBreakSpec()
+    spec.hdr.size = C.sizeof(DamBreakSpec)
+    spec.hdr.version = ABI_VERSION
+    spec.tank = (C.c_float * 3)(*tank)
+    spec.block = (C.c_uint32 * 3)(*block)
+    spec.d = d
+    spec.jitter = jitter
+    spec.seed = seed
+    spec.walls = 1 if walls else 0
+    n, nf, ns = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    _check(lib().vxl_sph_dam_break(C.byref(spec), C.byref(cfg), C.byref(n), C.byref(nf), C.byref(ns),
+                                   None, None, None, None))
+    pos = np.zeros((n.value, 3), np.float32)
+    vel = np.zeros((n.value, 3), np.float32)
+    pm = np.zeros(n.value, np.float32)
+    sp = (Span * max(ns.value, 1))()
+    _check(lib().vxl_sph_dam_break(C.byref(spec), C.byref(cfg), C.byref(n), C.byref(nf), C.byref(ns),
+                                   _f(pos), _f(vel), _f(pm), sp))
+    spans = [(sp[k].body, tuple(sp[k].origin), sp[k].start, sp[k].end) for k in range(ns.value)]
+    return SceneData(pos, vel, pm, nf.value, spans)
+
+
+def _spans_arr(spans):
+    arr = (Span * max(len(spans), 1))()
+    for k, (body, origin, s, e) in enumerate(spans):
+        arr[k].body = body
+        arr[k].origin = (C.c_float * 3)(*origin)
+        arr[k].start = s
+        arr[k].end = e
+    return arr
+
+
+class Backend:
+    """PhysGpuBackend (gpu/src/lib.rs:20-29)."""
+
+    def name(self):
+        return lib().vxl_sph_backend_name().decode()
+
+    def available(self):
+        return bool(lib().vxl_sph_backend_available())
+
+    def submit(self, tag):
+        return bool(lib().vxl_sph_backend_submit(tag.encode()))
+
+
+class Engine:
+    """A pool of independent scenes on one device, stepped together."""
+
+    def __init__(self, device=0, diagnostics=False, graphs=True):
+        self.h = C.c_void_p()
+        _check(lib().vxl_sph_engine_create(ABI_VERSION, device, C.byref(self.h)))
+        if diagnostics:
+            _check(lib().vxl_sph_engine_set_diagnostics(self.h, 1))
+        if not graphs:
+            _check(lib().vxl_sph_engine_set_graphs(self.h, 0))
+        self.scenes = {}
+
+    def close(self):
+        if self.h:
+            lib().vxl_sph_engine_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, cfg, data, reactions=True):
+        pos = np.ascontiguousarray(data.pos, np.float32)
+        vel = np.ascontiguousarray(data.vel, np.float32)
+        pm = np.ascontiguousarray(data.pmass, np.float32)
+        spans = data.spans if reactions else []
+        sid = C.c_uint32()
+        _check(lib().vxl_sph_scene_add(self.h, C.byref(cfg), data.n, data.n_fluid, _f(pos), _f(vel), _f(pm),
+                                       _spans_arr(spans), len(spans), C.byref(sid)))
+        self.scenes[sid.value] = (data.n, data.n_fluid, len(spans))
+        return sid.value
+
+    def remove(self, sid):
+        _check(lib().vxl_sph_scene_remove(self.h, sid))
+        self.scenes.pop(sid, None)
+
+    def step(self, ticks=1):
+        _check(lib().vxl_sph_engine_step(self.h, ticks))
+
+    def sync(self):
+        _check(lib().vxl_sph_engine_sync(self.h))
+
+    def device_bytes(self):
+        v = C.c_uint64()
+        _check(lib().vxl_sph_engine_device_bytes(self.h, C.byref(v)))
+        return v.value
+
+    def upload_boundary(self, sid, pos, vel):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        _check(lib().vxl_sph_scene_upload_boundary(self.h, sid, len(pos), _f(pos), _f(vel)))
+
+    def read_state(self, sid):
+        n = self.scenes[sid][0]
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        _check(lib().vxl_sph_scene_read_state(self.h, sid, n, _f(pos), _f(vel)))
+        return pos, vel
+
+    def read_fields(self, sid, diagnostics=False):
+        n, nf, _ = self.scenes[sid]
+        dens = np.zeros(n, np.float32)
+        press = np.zeros(n, np.float32)
+        if not diagnostics:
+            _check(lib().vxl_sph_scene_read_fields(self.h, sid, n, _f(dens), _f(press), None, None, None))
+            return dict(dens=dens, press=press)
+        ncount = np.zeros(n, np.uint32)
+        acc = np.zeros((nf, 3), np.float32)
+        xs = np.zeros((nf, 3), np.float32)
+        _check(lib().vxl_sph_scene_read_fields(self.h, sid, n, _f(dens), _f(press), _u(ncount), _f(acc), _f(xs)))
+        return dict(dens=dens, press=press, ncount=ncount, acc=acc, xsph=xs)
+
+    def read_grid(self, sid):
+        n = self.scenes[sid][0]
+        info = GridInfo()
+        _check(lib().vxl_sph_scene_read_grid(self.h, sid, C.byref(info), 0, None, 0, None, None))
+        start = np.zeros(info.total + 1, np.uint32)
+        items = np.zeros(n, np.uint32)
+        keys = np.zeros(n, np.uint32)
+        _check(lib().vxl_sph_scene_read_grid(self.h, sid, C.byref(info), len(start), _u(start), n, _u(items),
+                                             _u(keys)))
+        return dict(min=np.array(info.min, np.float32), inv=np.float32(info.inv), bin=np.float32(info.bin),
+                    dims=[int(x) for x in info.dims], total=int(info.total), start=start, items=items, keys=keys)
+
+    def read_bounds(self, sid):
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        _check(lib().vxl_sph_scene_read_bounds(self.h, sid, _f(lo), _f(hi)))
+        return lo, hi
+
+    def state_hash(self, sid):
+        v = C.c_uint64()
+        _check(lib().vxl_sph_scene_state_hash(self.h, sid, C.byref(v)))
+        return v.value
+
+
+class FluidStepper:
+    """FluidStepper drop-in with GpuFluidStepper semantics (pipeline/stepper.rs:70-147)."""
+
+    def __init__(self, cfg, device=0):
+        self.h = C.c_void_p()
+        _check(lib().vxl_sph_stepper_create(ABI_VERSION, C.byref(cfg), device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().vxl_sph_stepper_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, dt_tick, n_fluid, pos, vel, pmass, spans):
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        pmass = np.ascontiguousarray(pmass, np.float32)
+        self._n = len(pmass)
+        _check(lib().vxl_sph_stepper_step(self.h, C.c_float(dt_tick), n_fluid, len(pmass), _f(pos), _f(vel),
+                                          _f(pmass), _spans_arr(spans), len(spans)))
+
+    def reactions(self):
+        cnt = C.c_uint32()
+        _check(lib().vxl_sph_stepper_reactions(self.h, 0, None, C.byref(cnt)))
+        arr = (Reaction * max(cnt.value, 1))()
+        _check(lib().vxl_sph_stepper_reactions(self.h, cnt.value, arr, C.byref(cnt)))
+        return [(arr[k].body, np.array(arr[k].force, np.float32), np.array(arr[k].torque, np.float32))
+                for k in range(cnt.value)]
+
+    def bounds(self):
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        _check(lib().vxl_sph_stepper_bounds(self.h, _f(lo), _f(hi)))
+        return lo, hi
+
+    def read_state(self):
+        pos = np.zeros((self._n, 3), np.float32)
+        vel = np.zeros((self._n, 3), np.float32)
+        _check(lib().vxl_sph_stepper_read_state(self.h, self._n, _f(pos), _f(vel)))
+        return pos, vel
