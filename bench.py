@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""WCSPH step benchmark on one MI355X.
+
+Workload: the large single dam-break scene — 215^3 = 9,938,375 fluid particles
+at d = 0.01 m (h = 0.02 m) in a 4 x 2.5 x 2.5 m open tank sampled as one layer
+of Akinci boundary particles, dt = 2.5e-4 s, strict f32. One "step" = one
+substep of every stage (grid box, keys, scan, place, canonical gather,
+density + Tait, force + AV + XSPH + integration) for the whole scene, inputs
+resident in HBM. Prints ONE JSON line.
+
+value = fluid particle-updates/s (fluid particles x steps / wall time of the
+timed region, bracketed by device synchronisation). roofline = the dominant
+kernel's algorithmic bytes per launch / its mean HIP-event duration against the
+8 TB/s HBM3E peak. cpu_baseline = the oracle (a strict-f32 C++ restatement of
+the reference CPU step) timed on this machine's host cores on a bounded sample
+of the same scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rust-wl_amd"))
+
+METRIC = "WCSPH particle-updates/s at 10M (HBM roofline %); p50/p99 step latency, 64 scenes"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of each stage (compulsory SoA traffic, neighbour reads
+# served on chip): per particle N, per fluid particle NF, per grid cell C.
+STAGE_BYTES = {
+    "keys": dict(N=16),            # R pos 12, W key 4
+    "scan": dict(C=8),             # R counts 4, W start 4
+    "place": dict(N=8),            # R key 4, W item 4
+    "canon_gather": dict(N=60),    # R item 4, R pos/vel/pmass 28, W copies 28
+    "density": dict(N=20, C=4),    # R pos 12 + pmass 4, W dens 4; R start
+    "force": dict(NF=60, C=4),     # R pos, vel, pmass, dens 32 + item 4, W pos, vel 24; R start
+}
+
+
+def stage_bytes(name, N, NF, C):
+    m = STAGE_BYTES.get(name, {})
+    return m.get("N", 0) * N + m.get("NF", 0) * NF + m.get("C", 0) * C
+
+
+def build_scene(V, block, d, tank, max_bins, seed=1):
+    cfg = V.dam_break_config(d, dt=2.5e-4, max_bins=max_bins)
+    data = V.dam_break(cfg, tank, block, d, seed=seed)
+    return cfg, data
+
+
+def cpu_baseline(cfg, data, steps, threads):
+    """Oracle (test infrastructure) on the same scene, bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    oc = O.default_config(
+        rest_density=cfg.rest_density, smoothing_radius=cfg.smoothing_radius, xsph_viscosity=cfg.xsph_viscosity,
+        sound_speed=cfg.sound_speed, gamma_tait=cfg.gamma_tait, substeps=cfg.substeps, threads=threads,
+        tensile_clamp=cfg.tensile_clamp, boundary_layers=cfg.boundary_layers, gravity=tuple(cfg.gravity),
+        max_speed_frac=cfg.max_speed_frac, artificial_viscosity=cfg.artificial_viscosity, max_bins=cfg.max_bins)
+    nf = data.n_fluid
+    s = O.System.particles(oc, data.pos[:nf], data.vel[:nf], np.float32(cfg.particle_mass), np.float32(cfg.spacing))
+    s.set_boundary_raw(data.pos[nf:], data.vel[nf:], data.pmass[nf:], data.spans)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(cfg.dt_tick)
+    dt = time.perf_counter() - t0
+    return nf * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--block", type=int, default=215, help="fluid lattice edge (215 -> 9.94M particles)")
+    ap.add_argument("--budget-log2", type=int, default=22,
+                    help="grid budget GRID_MAX_BINS = 2^k (22 keeps bin = h at 10M; the reference rule is 21)")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=5)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import vxl_sph as V
+
+    d = 0.01
+    tank = (4.0, 2.5, 2.5)
+    block = (args.block,) * 3
+    t0 = time.perf_counter()
+    cfg, data = build_scene(V, block, d, tank, 1 << args.budget_log2)
+    t_build = time.perf_counter() - t0
+    N, NF = data.n, data.n_fluid
+    print(f"# scene: {NF} fluid + {N - NF} Akinci particles, built in {t_build:.1f}s", file=sys.stderr, flush=True)
+
+    eng = V.Engine()
+    sid = eng.add(cfg, data, reactions=False)
+    eng.step(args.warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = NF * args.steps / wall
+    print(f"# timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    lat = eng.step_timed(args.steps)
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+    prof = eng.profile(args.profile_steps)
+    g = eng.read_grid(sid)
+    C = g["total"] + 1
+    per_kernel = {k: v / args.profile_steps for k, v in prof.items()}
+    scan_ms = sum(per_kernel.get(k, 0.0) for k in ("scan_reduce", "scan_tiles", "scan_apply"))
+    kern = {k: v for k, v in per_kernel.items() if not k.startswith("scan_")}
+    kern["scan"] = scan_ms
+    dom = max(kern, key=kern.get)
+    dom_bytes = stage_bytes(dom, N, NF, C)
+    dom_gbs = dom_bytes / (kern[dom] * 1e-3) / 1e9
+    stage_gbs = {k: round(stage_bytes(k, N, NF, C) / (v * 1e-3) / 1e9, 1) for k, v in kern.items() if v > 0}
+    b_step = 104 * N + 60 * NF + 16 * C
+    step_gbs = b_step / (ms_per_step * 1e-3) / 1e9
+    print(f"# kernels (ms/step): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(kern.items(), key=lambda x: -x[1])),
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        print(f"# cpu baseline: oracle, {threads} threads, {args.cpu_steps} steps", file=sys.stderr, flush=True)
+        cv, cdt = cpu_baseline(cfg, data, args.cpu_steps, threads)
+        cpu = {"value": cv, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same {NF}-fluid scene ({cdt:.1f} s), strict-f32 C++ oracle, "
+                         f"phase-parallel over {threads} threads"}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered dam-break lattice + one-layer Akinci tank)",
+        "config": {"workload": f"dam break {block[0]}^3 = {NF} fluid + {N - NF} Akinci tank particles, "
+                               f"d = {d} m, h = {2 * d} m, tank {tank[0]}x{tank[1]}x{tank[2]} m, dt = 2.5e-4 s, "
+                               f"grid budget 2^{args.budget_log2} (bin {float(g['bin']):.3g} m, dims {g['dims']})",
+                   "fluid_particles": NF, "total_particles": N, "grid_cells": C - 1},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "step_latency_ms": {"p50": p50, "p99": p99},
+        "step_roofline": {"algorithmic_bytes_per_step": b_step, "achieved_GBs": round(step_gbs, 1),
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
+        "kernel_algorithmic_GBs": stage_gbs,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

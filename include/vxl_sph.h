@@ -117,6 +117,19 @@ VxlSphStatus vxl_sph_engine_step(VxlSphEngine *e, uint32_t ticks);
 VxlSphStatus vxl_sph_engine_sync(VxlSphEngine *e);
 VxlSphStatus vxl_sph_engine_device_bytes(VxlSphEngine *e, uint64_t *out);
 
+/* Stage timings. No C-ABI counterpart in the reference; they replace its
+ * per-tick stage timer TickMs (crates/vxl-phys-gpu/src/pipeline/gpu_types.rs:48-58)
+ * with HIP events on the engine's stream. */
+typedef struct VxlSphKernelTime {
+    char name[32];
+    float ms_total;   /* summed over the profiled substeps */
+} VxlSphKernelTime;
+/* Run `substeps` substeps eagerly with a HIP event between consecutive kernels. */
+VxlSphStatus vxl_sph_engine_profile(VxlSphEngine *e, uint32_t substeps, uint32_t cap, VxlSphKernelTime *out,
+                                    uint32_t *count);
+/* Run `ticks` ticks and return the device time of each (ms_out[ticks]). */
+VxlSphStatus vxl_sph_engine_step_timed(VxlSphEngine *e, uint32_t ticks, float *ms_out);
+
 /* Add a scene: all particles (fluid prefix, then boundary) in index order,
  * FluidSystem::raw_particles layout (fluid_access.rs:147-155). vel / pmass
  * may be NULL (zero velocity / cfg mass). */
@@ -160,6 +173,22 @@ VxlSphStatus vxl_sph_stepper_read_state(VxlSphStepper *s, uint32_t n, float *pos
  * points (3 floats each), returns the count, points per layer and V_b. */
 VxlSphStatus vxl_sph_box_lattice(const float half[3], float s, float h, uint32_t layers, uint32_t cap,
                                  float *out_pts, uint32_t *out_n, uint32_t *out_n0, float *out_volume);
+/* A Box body and its pose (boundary.rs BodyPose, fluid/src/types.rs:6-15). */
+typedef struct VxlSphBoxBody {
+    uint32_t id;
+    float half[3];
+    float pos[3];
+    float rot[4];     /* quaternion x, y, z, w */
+    float linvel[3];
+    float angvel[3];
+} VxlSphBoxBody;
+/* set_boundary_particles for Box bodies (fluid_step.rs:13-53): Akinci particles
+ * of cfg->boundary_layers layers at cfg->spacing, world pose p + q*lp, velocity
+ * v + w x r, pmass = rho0 * V_b, body order x layer x face order, one span per
+ * body with indices starting at `first`. Call with pos == NULL to query n / nspans. */
+VxlSphStatus vxl_sph_box_particles(const VxlSphFluidConfig *cfg, uint32_t nbodies, const VxlSphBoxBody *bodies,
+                                   uint32_t first, uint32_t *n, uint32_t *nspans, float *pos, float *vel,
+                                   float *pmass, VxlSphSpan *spans);
 /* FluidSystem::new constants (fluid_access.rs:8-42): h, h2, k6, ks, w0, b_tait, mass, alpha_c. */
 VxlSphStatus vxl_sph_fluid_constants(const VxlSphFluidConfig *cfg, float out[8]);
 

@@ -71,6 +71,7 @@ def lib():
         L.or_set_boundary_raw.argtypes = [P, C.c_uint32, fp, fp, fp, C.c_uint32, up, fp, up, up]
         L.or_set_velocities.argtypes = [P, C.c_uint32, fp]
         L.or_set_threads.argtypes = [P, C.c_uint32]
+        L.or_set_fault.argtypes = [P, C.c_int]
         L.or_step.argtypes = [P, C.c_float]
         for name in ("or_n_fluid", "or_n_total", "or_n_spans"):
             getattr(L, name).restype = C.c_uint32
@@ -208,6 +209,9 @@ class System:
 
     def set_threads(self, t):
         lib().or_set_threads(self.p, t)
+
+    def set_fault(self, code):
+        lib().or_set_fault(self.p, code)
 
     def step(self, dt_tick):
         lib().or_step(self.p, C.c_float(dt_tick))

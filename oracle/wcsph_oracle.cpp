@@ -42,22 +42,13 @@ inline V3 rotate(const float q[4], V3 v) {
     return add(add(v, cross(qv, t)), mul(t, q[3]));
 }
 
-// Rust `powi` with a constant exponent is binary exponentiation
-// (res = x for the lowest set bit, then res*cur for each further bit).
-inline float powi_rs(float x, int n) {
-    float res = 0.0f;
-    bool have = false;
-    float cur = x;
-    while (n) {
-        if (n & 1) {
-            res = have ? res * cur : cur;
-            have = true;
-        }
-        n >>= 1;
-        if (n) cur = cur * cur;
-    }
-    return have ? res : 1.0f;
-}
+// h.powi(n) as the reference binaries evaluate it: the smoothing radius is a
+// compile-time constant there, and LLVM folds llvm.powi.f32 through the host
+// pow() in double precision, rounding once to f32. (At run time powi would
+// be binary exponentiation in f32, which differs by an ulp for h = 0.1.)
+// Pinned by the reference's recorded Akinci canary readout, y_min = -1.81726
+// (docs/PLAN-gpu.md:2704), which only the folded constants reproduce.
+inline float powi_rs(float x, int n) { return (float)std::pow((double)x, (double)n); }
 
 // `f as u32` in Rust saturates: NaN -> 0, negative -> 0, >= 2^32 -> MAX.
 inline uint32_t sat_u32(float f) {
@@ -155,17 +146,25 @@ struct Grid {
 
     // grid.rs:115-157: cells dz, dy, dx (x fastest in the loop nest), items in
     // ascending index order; j == i skipped; accept r2 <= h2.
+    int fault = 0;  // test-only planted mistakes (see or_set_fault)
     template <class F>
     inline void for_neighbors(const std::vector<V3> &pos, float h2, size_t i, F &&f) const {
         V3 pi = pos[i];
         uint32_t cx = axis(min.x, pi.x, nx), cy = axis(min.y, pi.y, ny), cz = axis(min.z, pi.z, nz);
-        for (int dz = -1; dz <= 1; ++dz) {
-            int z = (int)cz + dz;
-            if (z < 0 || z >= (int)nz) continue;
+        for (int o0 = -1; o0 <= 1; ++o0) {
+            const int dz = fault == 3 ? 0 : o0;
+            const int dxo = fault == 3 ? o0 : 0;
+            (void)dxo;
+            int z = (int)cz + (fault == 3 ? 0 : dz);
+            if (fault != 3 && (z < 0 || z >= (int)nz)) continue;
             for (int dy = -1; dy <= 1; ++dy) {
                 int y = (int)cy + dy;
                 if (y < 0 || y >= (int)ny) continue;
-                for (int dx = -1; dx <= 1; ++dx) {
+                for (int o2 = -1; o2 <= 1; ++o2) {
+                    // fault 3: dx outer / dz inner (transposed enumeration)
+                    const int dx = fault == 3 ? o0 : o2;
+                    if (fault == 3) z = (int)cz + o2;
+                    if (fault == 3 && (z < 0 || z >= (int)nz)) continue;
                     int x = (int)cx + dx;
                     if (x < 0 || x >= (int)nx) continue;
                     uint32_t idx = ((uint32_t)x * ny + (uint32_t)y) * nz + (uint32_t)z;
@@ -186,6 +185,7 @@ struct Grid {
 
 struct OrSystem {
     OrConfig cfg;
+    int fault = 0;  // planted mistake for tests of the pins (0 = none)
     float h, h2, k6, ks, w0, b_tait, mass, spacing;
     std::vector<V3> pos, vel, acc, xsph, bforce;
     std::vector<float> dens, press, pmass;
@@ -274,7 +274,7 @@ void density_pass(OrSystem *s) {
             uint32_t cnt = 0;
             g.for_neighbors(s->pos, h2, i, [&](size_t j, V3, float r2) {
                 float t = h2 - r2;
-                float w = ((k6 * t) * t) * t;
+                float w = s->fault == 1 ? k6 * ((t * t) * t) : ((k6 * t) * t) * t;
                 if (j < nf)
                     sum += w;
                 else
@@ -376,7 +376,7 @@ void force_pass(OrSystem *s) {
                 }
                 float tt = h2 - r2;
                 float w = ((k6 * tt) * tt) * tt;
-                xs = add(xs, mul(sub(s->vel[j], vi), ((mj * 2.0f) / (rho_i + rho_j)) * w));
+                if (s->fault != 2) xs = add(xs, mul(sub(s->vel[j], vi), ((mj * 2.0f) / (rho_i + rho_j)) * w));
             });
             s->acc[i] = acc;
             s->xsph[i] = xs;
@@ -693,6 +693,14 @@ extern "C" void or_set_velocities(OrSystem *s, uint32_t n, const float *vel) {
 }
 
 extern "C" void or_set_threads(OrSystem *s, uint32_t threads) { s->cfg.threads = threads; }
+
+// Test-only: plant a plausible mistake so tests can show the pins catch it.
+// 1 = poly6 product reassociated k6*((t*t)*t); 2 = XSPH term dropped;
+// 3 = neighbour cells enumerated dx outer / dz inner.
+extern "C" void or_set_fault(OrSystem *s, int fault) {
+    s->fault = fault;
+    s->grid.fault = fault;
+}
 
 // fluid_step.rs:75-89
 extern "C" void or_step(OrSystem *s, float dt_tick) {

@@ -76,6 +76,9 @@ void or_set_boundary_raw(OrSystem *s, uint32_t nb, const float *pos, const float
 
 void or_set_velocities(OrSystem *s, uint32_t n, const float *vel);
 void or_set_threads(OrSystem *s, uint32_t threads);
+/* Test-only planted mistakes (1 = poly6 reassociated, 2 = XSPH dropped,
+ * 3 = neighbour order transposed); 0 restores the reference algorithm. */
+void or_set_fault(OrSystem *s, int fault);
 
 /* FluidSystem::step (fluid_step.rs:75-89). */
 void or_step(OrSystem *s, float dt_tick);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -69,6 +70,12 @@ public:
     // Tick-averaged (F, tau) per span of the scene: 6 floats each.
     std::vector<float> read_reactions(uint32_t id);
 
+    // Per-kernel device time: runs `substeps` eager substeps with a HIP event
+    // between consecutive kernels on the engine stream. Returns (name, ms total).
+    std::vector<std::pair<std::string, float>> profile(uint32_t substeps);
+    // Runs `ticks` ticks, recording the device time of each tick (ms).
+    std::vector<float> step_timed(uint32_t ticks);
+
     void set_diagnostics(bool on);
     void set_graphs(bool on) { use_graph_ = on; invalidate_graph(); }
     hipStream_t stream() const { return stream_; }
@@ -83,7 +90,9 @@ public:
 
 private:
     void layout();            // assign slot ranges, (re)allocate, move + upload
-    void enqueue_substep();   // the launch sequence of one substep
+    // The launch sequence of one substep; `mark` (if set) is called after each
+    // kernel launch with the kernel's name.
+    void enqueue_substep(const std::function<void(const char *)> *mark = nullptr);
     void build_graph();
     void invalidate_graph();
     void check_error_state();  // read the control block; grow tables on demand

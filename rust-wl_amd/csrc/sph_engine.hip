@@ -2,6 +2,7 @@
 #include "sph_engine.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace vxl {
@@ -113,6 +114,8 @@ uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, cons
         sc.pmass.assign(pmass, pmass + n);
     else
         sc.pmass.assign(n, sc.params.mass);
+    for (float m : sc.pmass)
+        if (!(m > 0.0f) || !std::isfinite(m)) throw std::invalid_argument("particle masses must be positive and finite");
     for (uint32_t k = 0; k < nspans; ++k) {
         if (spans[k].start < nf || spans[k].end > n || spans[k].start > spans[k].end)
             throw std::invalid_argument("span outside the boundary segment");
@@ -167,13 +170,17 @@ void Engine::alloc_pool(uint32_t slots) {
         p = dalloc<uint32_t>(slots);
         pool_allocs_.push_back(p);
     };
-    F(P_.cx); F(P_.cy); F(P_.cz); F(P_.cvx); F(P_.cvy); F(P_.cvz); F(P_.cm); U(P_.cg);
-    F(P_.sx); F(P_.sy); F(P_.sz); F(P_.svx); F(P_.svy); F(P_.svz); F(P_.sm); U(P_.sg);
+    auto F4 = [&](float4 *&p) {
+        p = dalloc<float4>(slots);
+        pool_allocs_.push_back(p);
+    };
+    F4(P_.cA); F4(P_.cV); U(P_.cg);
+    F4(P_.sA); F4(P_.sV); U(P_.sg);
+    F(P_.c2); F(P_.press);
     U(P_.key); U(P_.slot); U(P_.items_m); U(P_.items_g);
-    F(P_.dens); F(P_.press); F(P_.c2);
     if (diagnostics_) {
         U(P_.ncount);
-        F(P_.ax); F(P_.ay); F(P_.az); F(P_.xx); F(P_.xy); F(P_.xz);
+        F4(P_.acc); F4(P_.xsph);
     }
 }
 
@@ -197,7 +204,7 @@ void Engine::layout() {
     // Save resident scenes' device state (cur arrays, this grid order).
     struct Saved {
         uint32_t idx, n;
-        std::vector<float> f[7];
+        std::vector<float4> a, v;
         std::vector<uint32_t> g;
     };
     std::vector<Saved> saved;
@@ -207,12 +214,10 @@ void Engine::layout() {
         Saved sv;
         sv.idx = k;
         sv.n = sc.params.n;
-        float *src[7] = {P_.cx, P_.cy, P_.cz, P_.cvx, P_.cvy, P_.cvz, P_.cm};
-        for (int a = 0; a < 7; ++a) {
-            sv.f[a].resize(sv.n);
-            hip_check(hipMemcpy(sv.f[a].data(), src[a] + sc.params.pbase, 4ull * sv.n, hipMemcpyDeviceToHost),
-                      "save scene");
-        }
+        sv.a.resize(sv.n);
+        sv.v.resize(sv.n);
+        hip_check(hipMemcpy(sv.a.data(), P_.cA + sc.params.pbase, 16ull * sv.n, hipMemcpyDeviceToHost), "save A");
+        hip_check(hipMemcpy(sv.v.data(), P_.cV + sc.params.pbase, 16ull * sv.n, hipMemcpyDeviceToHost), "save V");
         sv.g.resize(sv.n);
         hip_check(hipMemcpy(sv.g.data(), P_.cg + sc.params.pbase, 4ull * sv.n, hipMemcpyDeviceToHost), "save gid");
         saved.push_back(std::move(sv));
@@ -262,11 +267,10 @@ void Engine::layout() {
     }
     nspans_ = (uint32_t)spans.size();
     if (nspans_) {
-        auto F = [&](float *&p) {
-            p = dalloc<float>(slots_);
-            pool_allocs_.push_back(p);
-        };
-        F(P_.bfx); F(P_.bfy); F(P_.bfz); F(P_.bpx); F(P_.bpy); F(P_.bpz);
+        P_.bf = dalloc<float4>(slots_);
+        pool_allocs_.push_back(P_.bf);
+        P_.bp = dalloc<float4>(slots_);
+        pool_allocs_.push_back(P_.bp);
         d_spans_ = dalloc<SpanDev>(nspans_);
         hip_check(hipMemcpy(d_spans_, spans.data(), sizeof(SpanDev) * nspans_, hipMemcpyHostToDevice), "spans");
         d_react_ = dalloc<float>(6ull * nspans_);
@@ -284,26 +288,24 @@ void Engine::layout() {
                   "block_scene");
     // Upload state: resident scenes from the saved copies, new scenes from
     // their staging (index order; the first grid build sorts them).
-    std::vector<float> tmp;
-    float *dst[7] = {P_.cx, P_.cy, P_.cz, P_.cvx, P_.cvy, P_.cvz, P_.cm};
     for (auto &sv : saved) {
         SceneSlot &sc = scenes_[sv.idx];
-        for (int a = 0; a < 7; ++a)
-            hip_check(hipMemcpy(dst[a] + sc.params.pbase, sv.f[a].data(), 4ull * sv.n, hipMemcpyHostToDevice),
-                      "restore");
+        hip_check(hipMemcpy(P_.cA + sc.params.pbase, sv.a.data(), 16ull * sv.n, hipMemcpyHostToDevice), "restore A");
+        hip_check(hipMemcpy(P_.cV + sc.params.pbase, sv.v.data(), 16ull * sv.n, hipMemcpyHostToDevice), "restore V");
         hip_check(hipMemcpy(P_.cg + sc.params.pbase, sv.g.data(), 4ull * sv.n, hipMemcpyHostToDevice), "restore gid");
     }
     for (auto &sc : scenes_) {
         if (sc.resident) continue;
-        const uint32_t n = sc.params.n;
-        tmp.resize(n);
-        for (int a = 0; a < 3; ++a) {
-            for (uint32_t i = 0; i < n; ++i) tmp[i] = sc.pos[3 * i + a];
-            hip_check(hipMemcpy(dst[a] + sc.params.pbase, tmp.data(), 4ull * n, hipMemcpyHostToDevice), "pos");
-            for (uint32_t i = 0; i < n; ++i) tmp[i] = sc.vel[3 * i + a];
-            hip_check(hipMemcpy(dst[3 + a] + sc.params.pbase, tmp.data(), 4ull * n, hipMemcpyHostToDevice), "vel");
+        const uint32_t n = sc.params.n, nf = sc.params.nf;
+        // A = (x, y, z, m) with m = +pmass (fluid) / -pmass (boundary); V = (v, 0).
+        std::vector<float4> a(n), v(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const float m = sc.pmass[i];
+            a[i] = make_float4(sc.pos[3 * i], sc.pos[3 * i + 1], sc.pos[3 * i + 2], i < nf ? m : -m);
+            v[i] = make_float4(sc.vel[3 * i], sc.vel[3 * i + 1], sc.vel[3 * i + 2], 0.0f);
         }
-        hip_check(hipMemcpy(P_.cm + sc.params.pbase, sc.pmass.data(), 4ull * n, hipMemcpyHostToDevice), "pmass");
+        hip_check(hipMemcpy(P_.cA + sc.params.pbase, a.data(), 16ull * n, hipMemcpyHostToDevice), "A");
+        hip_check(hipMemcpy(P_.cV + sc.params.pbase, v.data(), 16ull * n, hipMemcpyHostToDevice), "V");
         std::vector<uint32_t> g(n);
         for (uint32_t i = 0; i < n; ++i) g[i] = i;
         hip_check(hipMemcpy(P_.cg + sc.params.pbase, g.data(), 4ull * n, hipMemcpyHostToDevice), "gid");
@@ -338,24 +340,93 @@ void Engine::layout() {
     dirty_ = false;
 }
 
-void Engine::enqueue_substep() {
+void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     const uint32_t nb = P_.nblocks;
     const uint32_t grid_slots = tuning_.persist_blocks ? std::min(nb, tuning_.persist_blocks) : nb;
     const uint32_t grid_scan = std::min<uint32_t>(tiles_cap_, 4096);
+    auto M = [&](const char *name) {
+        if (mark) (*mark)(name);
+    };
     hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream_, P_);
+    M("grid_box");
     hipLaunchKernelGGL(k_keys, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    M("keys");
     hipLaunchKernelGGL(k_scan_reduce, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+    M("scan_reduce");
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream_, P_, d_tile_sums_);
+    M("scan_tiles");
     hipLaunchKernelGGL(k_scan_apply, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+    M("scan_apply");
     hipLaunchKernelGGL(k_place, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    M("place");
     hipLaunchKernelGGL(k_canon_gather, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    M("canon_gather");
     hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    M("density");
     hipLaunchKernelGGL(k_force, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    M("force");
     if (nspans_) {
         hipLaunchKernelGGL(k_bforce, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        M("bforce");
         hipLaunchKernelGGL(k_reactions, dim3((nspans_ + 63) / 64), dim3(64), 0, stream_, P_);
+        M("reactions");
     }
     hip_check(hipGetLastError(), "substep launch");
+}
+
+std::vector<std::pair<std::string, float>> Engine::profile(uint32_t substeps) {
+    sync();
+    if (scenes_.empty()) return {};
+    std::vector<hipEvent_t> evs;
+    std::vector<std::string> names;
+    auto ev = [&]() {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipEventRecord(e, stream_), "hipEventRecord");
+        evs.push_back(e);
+    };
+    std::function<void(const char *)> mark = [&](const char *name) {
+        names.push_back(name);
+        ev();
+    };
+    for (uint32_t k = 0; k < substeps; ++k) {
+        ev();
+        names.push_back("");
+        enqueue_substep(&mark);
+    }
+    substeps_done_ += substeps;
+    hip_check(hipStreamSynchronize(stream_), "profile sync");
+    std::vector<std::pair<std::string, float>> out;
+    for (size_t k = 1; k < evs.size(); ++k) {
+        if (names[k].empty()) continue;
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, evs[k - 1], evs[k]), "hipEventElapsedTime");
+        auto it = std::find_if(out.begin(), out.end(), [&](const auto &p) { return p.first == names[k]; });
+        if (it == out.end())
+            out.push_back({names[k], ms});
+        else
+            it->second += ms;
+    }
+    for (auto e : evs) (void)hipEventDestroy(e);
+    check_error_state();
+    return out;
+}
+
+std::vector<float> Engine::step_timed(uint32_t ticks) {
+    if (dirty_) layout();
+    std::vector<hipEvent_t> evs(ticks + 1);
+    for (auto &e : evs) hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventRecord(evs[0], stream_), "hipEventRecord");
+    for (uint32_t k = 0; k < ticks; ++k) {
+        step(1);
+        hip_check(hipEventRecord(evs[k + 1], stream_), "hipEventRecord");
+    }
+    hip_check(hipStreamSynchronize(stream_), "step_timed sync");
+    std::vector<float> ms(ticks);
+    for (uint32_t k = 0; k < ticks; ++k) hip_check(hipEventElapsedTime(&ms[k], evs[k], evs[k + 1]), "elapsed");
+    for (auto e : evs) (void)hipEventDestroy(e);
+    check_error_state();
+    return ms;
 }
 
 void Engine::build_graph() {
@@ -451,15 +522,17 @@ void Engine::upload_boundary(uint32_t id, uint32_t nb, const float *pos, const f
     const uint32_t pb = sc.params.pbase;
     std::vector<uint32_t> gid(n);
     hip_check(hipMemcpy(gid.data(), P_.cg + pb, 4ull * n, hipMemcpyDeviceToHost), "gid");
-    float *dst[6] = {P_.cx, P_.cy, P_.cz, P_.cvx, P_.cvy, P_.cvz};
-    std::vector<float> cur(n);
-    for (int a = 0; a < 6; ++a) {
-        hip_check(hipMemcpy(cur.data(), dst[a] + pb, 4ull * n, hipMemcpyDeviceToHost), "cur");
-        const float *src = a < 3 ? pos : vel;
-        for (uint32_t m = 0; m < n; ++m)
-            if (gid[m] >= nf) cur[m] = src[3 * (gid[m] - nf) + (a % 3)];
-        hip_check(hipMemcpy(dst[a] + pb, cur.data(), 4ull * n, hipMemcpyHostToDevice), "cur");
+    std::vector<float4> a(n), v(n);
+    hip_check(hipMemcpy(a.data(), P_.cA + pb, 16ull * n, hipMemcpyDeviceToHost), "A");
+    hip_check(hipMemcpy(v.data(), P_.cV + pb, 16ull * n, hipMemcpyDeviceToHost), "V");
+    for (uint32_t m = 0; m < n; ++m) {
+        if (gid[m] < nf) continue;
+        const size_t q = 3ull * (gid[m] - nf);
+        a[m].x = pos[q]; a[m].y = pos[q + 1]; a[m].z = pos[q + 2];
+        v[m].x = vel[q]; v[m].y = vel[q + 1]; v[m].z = vel[q + 2];
     }
+    hip_check(hipMemcpy(P_.cA + pb, a.data(), 16ull * n, hipMemcpyHostToDevice), "A");
+    hip_check(hipMemcpy(P_.cV + pb, v.data(), 16ull * n, hipMemcpyHostToDevice), "V");
     // Boundary motion changes the AABB of the next grid build.
     std::vector<SceneBox> boxes(2 * scenes_.size());
     hip_check(hipMemcpy(boxes.data(), d_box_, sizeof(SceneBox) * boxes.size(), hipMemcpyDeviceToHost), "box");
@@ -482,21 +555,12 @@ void Engine::read_state(uint32_t id, float *pos, float *vel) {
     sync();
     const SceneSlot &sc = scene(id);
     const uint32_t n = sc.params.n, pb = sc.params.pbase;
-    float *o = d_out_f_;
-    hipLaunchKernelGGL(k_unsort, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, o, o + slots_, o + 2ull * slots_,
-                       o + 3ull * slots_, o + 4ull * slots_, o + 5ull * slots_);
+    float *op = d_out_f_, *ov = d_out_f_ + 3ull * slots_;
+    hipLaunchKernelGGL(k_unsort, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, op, ov);
     hip_check(hipGetLastError(), "k_unsort");
-    std::vector<float> tmp(6ull * n);
-    for (int a = 0; a < 6; ++a)
-        hip_check(hipMemcpyAsync(tmp.data() + (size_t)a * n, o + (size_t)a * slots_ + pb, 4ull * n,
-                                 hipMemcpyDeviceToHost, stream_),
-                  "read");
+    if (pos) hip_check(hipMemcpyAsync(pos, op + 3ull * pb, 12ull * n, hipMemcpyDeviceToHost, stream_), "pos");
+    if (vel) hip_check(hipMemcpyAsync(vel, ov + 3ull * pb, 12ull * n, hipMemcpyDeviceToHost, stream_), "vel");
     hip_check(hipStreamSynchronize(stream_), "sync");
-    for (uint32_t i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-            if (pos) pos[3 * i + a] = tmp[(size_t)a * n + i];
-            if (vel) vel[3 * i + a] = tmp[(size_t)(3 + a) * n + i];
-        }
 }
 
 void Engine::read_fields(uint32_t id, float *dens, float *press, uint32_t *ncount, float *acc, float *xsph) {
@@ -506,7 +570,7 @@ void Engine::read_fields(uint32_t id, float *dens, float *press, uint32_t *ncoun
     const SceneSlot &sc = scene(id);
     const uint32_t n = sc.params.n, nf = sc.params.nf, pb = sc.params.pbase;
     float *f = d_out_fields_;
-    float *od = f, *op = f + slots_, *oa = f + 2ull * slots_, *ox = f + 5ull * slots_;
+    float *od = f, *op = f + slots_, *oa = f + 2ull * slots_, *ox = f + 5ull * slots_;  // acc/xsph AoS
     hipLaunchKernelGGL(k_unsort_fields, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, od, op,
                        diagnostics_ ? d_out_u_ : nullptr, diagnostics_ ? oa : nullptr, diagnostics_ ? ox : nullptr);
     hip_check(hipGetLastError(), "k_unsort_fields");
@@ -528,14 +592,26 @@ SceneGrid Engine::read_grid(uint32_t id, std::vector<uint32_t> *start, std::vect
     SceneGrid g{};
     hip_check(hipMemcpy(&g, d_grid_ + k, sizeof(SceneGrid), hipMemcpyDeviceToHost), "grid");
     const uint32_t pb = sc.params.pbase, n = sc.params.n;
-    if (start) {
-        start->resize((size_t)g.total + 1);
-        hip_check(hipMemcpy(start->data(), P_.start + g.cbase, 4ull * (g.total + 1), hipMemcpyDeviceToHost), "start");
-        for (auto &v : *start) v -= pb;
-    }
-    if (items) {
-        items->resize(n);
-        hip_check(hipMemcpy(items->data(), P_.sg + pb, 4ull * n, hipMemcpyDeviceToHost), "items");
+    if (start || items) {
+        // Device tables are x fastest; rebuild the reference layout
+        // (cell = (x*ny + y)*nz + z, fluid grid.rs:80-89).
+        std::vector<uint32_t> st((size_t)g.total + 1), it(n);
+        hip_check(hipMemcpy(st.data(), P_.start + g.cbase, 4ull * (g.total + 1), hipMemcpyDeviceToHost), "start");
+        hip_check(hipMemcpy(it.data(), P_.sg + pb, 4ull * n, hipMemcpyDeviceToHost), "items");
+        const uint32_t nx = g.dims[0], ny = g.dims[1], nz = g.dims[2];
+        std::vector<uint32_t> rs((size_t)g.total + 1), ri(n);
+        uint32_t w = 0;
+        for (uint32_t x = 0; x < nx; ++x)
+            for (uint32_t y = 0; y < ny; ++y)
+                for (uint32_t z = 0; z < nz; ++z) {
+                    const uint32_t c = (x * ny + y) * nz + z;
+                    const uint32_t ci = (z * ny + y) * nx + x;
+                    rs[c] = w;
+                    for (uint32_t k = st[ci]; k < st[ci + 1]; ++k) ri[w++] = it[k - pb];
+                }
+        rs[g.total] = w;
+        if (start) *start = std::move(rs);
+        if (items) *items = std::move(ri);
     }
     return g;
 }

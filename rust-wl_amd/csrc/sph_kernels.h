@@ -8,21 +8,28 @@
 namespace vxl {
 
 // Device pointers of the particle pool and cell tables (see sph_layout.h).
+//
+// Particle records are packed for one 16-byte load per neighbour candidate:
+//   A = (x, y, z, m)   m = +pmass for fluid, -pmass for boundary particles
+//   V = (vx, vy, vz, rho)
+// Cell tables are stored with x fastest (internal cell = (z*ny + y)*nx + x),
+// so the three cells dx = -1, 0, +1 of one (dz, dy) row of the reference's
+// dz -> dy -> dx enumeration are one contiguous slot range.
 struct Pool {
     // state in the previous substep's grid order
-    float *cx, *cy, *cz, *cvx, *cvy, *cvz, *cm;
+    float4 *cA, *cV;
     uint32_t *cg;
     // state gathered into this substep's grid order
-    float *sx, *sy, *sz, *svx, *svy, *svz, *sm;
+    float4 *sA, *sV;
     uint32_t *sg;
+    float *c2;     // p / (rho * rho)
+    float *press;  // p
     // sort
     uint32_t *key, *slot, *items_m, *items_g;
     uint32_t *counts, *start;  // counts[ccap], start[ccap + 1]
-    // per-particle fields (grid order)
-    float *dens, *press, *c2;
     // optional diagnostics (grid order; null = off)
     uint32_t *ncount;
-    float *ax, *ay, *az, *xx, *xy, *xz;
+    float4 *acc, *xsph;
     // scene tables
     const SceneParams *params;
     SceneGrid *grid;
@@ -34,7 +41,7 @@ struct Pool {
     uint32_t ccap;     // cell table capacity
     uint32_t nspans;   // reaction spans (0 = no reaction pass)
     // reactions (index order, pool slots): boundary force and position
-    float *bfx, *bfy, *bfz, *bpx, *bpy, *bpz;
+    float4 *bf, *bp;
     const SpanDev *spans;
     float *react;      // [nspans][6]: F, tau (tick accumulator)
 };
@@ -51,9 +58,8 @@ __global__ void k_density(Pool P);
 __global__ void k_force(Pool P);
 __global__ void k_bforce(Pool P);
 __global__ void k_reactions(Pool P);
-__global__ void k_unsort(Pool P, float *ox, float *oy, float *oz, float *ovx, float *ovy, float *ovz);
-__global__ void k_unsort_fields(Pool P, float *odens, float *opress, uint32_t *oncount, float *oacc,
-                                float *oxs);
+__global__ void k_unsort(Pool P, float *opos, float *ovel);
+__global__ void k_unsort_fields(Pool P, float *odens, float *opress, uint32_t *oncount, float *oacc, float *oxs);
 
 }  // namespace vxl
 #endif

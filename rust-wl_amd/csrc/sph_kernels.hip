@@ -2,13 +2,16 @@
 // of the pool; a single scene is a pool of one).
 //
 // Substep = grid_box -> keys -> scan (3 passes) -> place -> canon_gather ->
-// density(+Tait) -> force(+AV, XSPH, integrate, next AABB).
+// density(+Tait) -> force(+AV, XSPH, integrate, next AABB) [-> bforce ->
+// reactions].
 //
 // Arithmetic contract: strict IEEE f32 (built with -ffp-contract=off,
 // correctly rounded / and sqrt, denormals kept), every expression in the
 // operation order of the reference CPU step, neighbours enumerated cell by
 // cell (dz, dy, dx; within a cell ascending original index), so every sum is
-// the reference's sum bit for bit.
+// the reference's sum bit for bit. Cells that cannot hold a neighbour within
+// h are skipped (a conservative test with a 2 % margin), which removes only
+// candidates the r^2 <= h^2 test would reject and keeps the order of the rest.
 #include <hip/hip_runtime.h>
 
 #include "sph_kernels.h"
@@ -26,11 +29,58 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// fluid grid.rs:91-101: u = min(max(floor((v - o) * inv), 0) as u32, n - 1).
-__device__ __forceinline__ uint32_t axis_bin(float o, float v, float inv, uint32_t n) {
-    float f = fmaxf(floorf((v - o) * inv), 0.0f);
-    uint32_t lim = n - 1u;
-    return f >= (float)lim ? lim : (uint32_t)f;
+// fluid grid.rs:91-101: u = min(max(floor((v - o) * inv), 0) as u32, n - 1);
+// also returns the position inside the cell in cell units, clamped to [0, 1].
+__device__ __forceinline__ uint32_t axis_bin(float o, float v, float inv, uint32_t n, float *frac) {
+    const float t = (v - o) * inv;
+    const float f = fmaxf(floorf(t), 0.0f);
+    const uint32_t lim = n - 1u;
+    const uint32_t u = f >= (float)lim ? lim : (uint32_t)f;
+    *frac = fminf(fmaxf(t - (float)u, 0.0f), 1.0f);
+    return u;
+}
+
+// Neighbour geometry of one particle: its cell and where it sits inside it.
+struct Cursor {
+    int cx, cy, cz;
+    float fx, fy, fz;
+    float cut;  // (h / bin)^2 with margin, in cell units
+};
+
+__device__ __forceinline__ Cursor make_cursor(const SceneGrid &g, float h, float px, float py, float pz) {
+    Cursor c;
+    c.cx = (int)axis_bin(g.gmin[0], px, g.inv, g.dims[0], &c.fx);
+    c.cy = (int)axis_bin(g.gmin[1], py, g.inv, g.dims[1], &c.fy);
+    c.cz = (int)axis_bin(g.gmin[2], pz, g.inv, g.dims[2], &c.fz);
+    const float hc = h * g.inv;
+    c.cut = hc * hc * 1.02f + 1e-6f;
+    return c;
+}
+
+// Visit the 3x3 rows of the 27-cell neighbourhood in the reference order
+// (dz outer, dy inner); each row is handed over as one slot range covering
+// the cells dx = -1, 0, +1 that survive the conservative distance test.
+template <class F>
+__device__ __forceinline__ void for_rows(const SceneGrid &g, const uint32_t *__restrict__ start, const Cursor &c,
+                                         F &&visit) {
+    const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
+    const float lx = c.fx * c.fx, rx = (1.0f - c.fx) * (1.0f - c.fx);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = c.cz + dz;
+        if (z < 0 || z >= nz) continue;
+        const float ez = dz < 0 ? c.fz : (dz > 0 ? 1.0f - c.fz : 0.0f);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = c.cy + dy;
+            if (y < 0 || y >= ny) continue;
+            const float ey = dy < 0 ? c.fy : (dy > 0 ? 1.0f - c.fy : 0.0f);
+            const float rd = ez * ez + ey * ey;
+            if (rd > c.cut) continue;
+            const int xl = (c.cx > 0 && rd + lx <= c.cut) ? c.cx - 1 : c.cx;
+            const int xr = (c.cx + 1 < nx && rd + rx <= c.cut) ? c.cx + 1 : c.cx;
+            const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+            visit(start[row + (uint32_t)xl], start[row + (uint32_t)xr + 1u]);
+        }
+    }
 }
 
 // Block-wide AABB of the positions handed in by live lanes, merged into one
@@ -75,9 +125,9 @@ __global__ __launch_bounds__(kBlock) void k_box(Pool P) {
         const SceneParams &sp = P.params[s];
         const uint32_t m = b * kBlock + threadIdx.x;
         const bool live = m < sp.pbase + sp.n;
-        float x = 0, y = 0, z = 0;
-        if (live) { x = P.cx[m]; y = P.cy[m]; z = P.cz[m]; }
-        block_box(live, x, y, z, &P.box[par * P.nscenes + s]);
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (live) a = P.cA[m];
+        block_box(live, a.x, a.y, a.z, &P.box[par * P.nscenes + s]);
     }
 }
 
@@ -113,10 +163,10 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P) {
             uint32_t dims[3] = {1, 1, 1};
             for (int it = 0; it < 256; ++it) {
                 for (int a = 0; a < 3; ++a) {
-                    float f = floorf(ext[a] / bin);
+                    const float f = floorf(ext[a] / bin);
                     // `as usize` saturates (NaN, negative -> 0), then +1, clamp [1, 16384].
-                    uint32_t d = (f >= (float)kGridDimMax) ? kGridDimMax
-                                 : (f > 0.0f ? (uint32_t)f + 1u : 1u);
+                    const uint32_t d = (f >= (float)kGridDimMax) ? kGridDimMax
+                                       : (f > 0.0f ? (uint32_t)f + 1u : 1u);
                     dims[a] = min(max(d, 1u), kGridDimMax);
                 }
                 if ((uint64_t)dims[0] * dims[1] * dims[2] <= sp.max_bins) break;
@@ -140,15 +190,12 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P) {
         scan[threadIdx.x] = need;
         __syncthreads();
         for (uint32_t o = 1; o < 1024; o <<= 1) {
-            uint32_t v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0u;
+            const uint32_t v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0u;
             __syncthreads();
             scan[threadIdx.x] += v;
             __syncthreads();
         }
-        if (s < P.nscenes) {
-            const uint32_t base = carry + scan[threadIdx.x] - need;
-            P.grid[s].cbase = base;
-        }
+        if (s < P.nscenes) P.grid[s].cbase = carry + scan[threadIdx.x] - need;
         __syncthreads();
         if (threadIdx.x == 0) carry += scan[1023];
         __syncthreads();
@@ -172,24 +219,41 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P) {
     }
 }
 
-// Cell key per cur slot (fluid grid.rs:80-89, z fastest) and its arrival
-// rank inside the cell (atomic, order-free: canonicalised later).
+// Cell of every cur slot (fluid grid.rs:80-89; stored x fastest) and its
+// arrival rank inside the cell. Lanes of a wave that hold the same cell in a
+// run (the cur order is the previous grid order) share one atomic.
 __global__ __launch_bounds__(kBlock) void k_keys(Pool P) {
     if (P.ctl->err) return;
+    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
         const uint32_t m = b * kBlock + threadIdx.x;
-        if (m >= sp.pbase + sp.n) continue;
-        const SceneGrid &g = P.grid[s];
-        const uint32_t ux = axis_bin(g.gmin[0], P.cx[m], g.inv, g.dims[0]);
-        const uint32_t uy = axis_bin(g.gmin[1], P.cy[m], g.inv, g.dims[1]);
-        const uint32_t uz = axis_bin(g.gmin[2], P.cz[m], g.inv, g.dims[2]);
-        uint32_t local = (ux * g.dims[1] + uy) * g.dims[2] + uz;
-        local = min(local, g.total - 1u);
-        const uint32_t c = g.cbase + local;
-        P.key[m] = c;
-        P.slot[m] = atomicAdd(&P.counts[c], 1u);
+        const bool live = m < sp.pbase + sp.n;
+        uint32_t c = 0xffffffffu;
+        if (live) {
+            const SceneGrid &g = P.grid[s];
+            const float4 a = P.cA[m];
+            float f;
+            const uint32_t ux = axis_bin(g.gmin[0], a.x, g.inv, g.dims[0], &f);
+            const uint32_t uy = axis_bin(g.gmin[1], a.y, g.inv, g.dims[1], &f);
+            const uint32_t uz = axis_bin(g.gmin[2], a.z, g.inv, g.dims[2], &f);
+            c = g.cbase + (uz * g.dims[1] + uy) * g.dims[0] + ux;
+        }
+        const uint32_t prev = (uint32_t)__shfl_up((int)c, 1, 64);
+        const bool head = lane == 0u || prev != c;
+        const uint64_t heads = __ballot(head);
+        const uint64_t upto = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);
+        const int hl = 63 - __clzll((long long)(heads & upto));
+        const uint64_t after = heads & ~upto;
+        const int nl = after ? __ffsll((long long)after) - 1 : 64;
+        uint32_t base = 0;
+        if (head && live) base = atomicAdd(&P.counts[c], (uint32_t)(nl - (int)lane));
+        base = (uint32_t)__shfl((int)base, hl, 64);
+        if (live) {
+            P.key[m] = c;
+            P.slot[m] = base + (lane - (uint32_t)hl);
+        }
     }
 }
 
@@ -235,7 +299,7 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(Pool P, uint32_t *tile_sums
         sh[threadIdx.x] = v;
         __syncthreads();
         for (uint32_t o = 1; o < 1024; o <<= 1) {
-            uint32_t a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+            const uint32_t a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
             __syncthreads();
             sh[threadIdx.x] += a;
             __syncthreads();
@@ -266,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(Pool P, const uint32_t *t
         sh[threadIdx.x] = local;
         __syncthreads();
         for (uint32_t o = 1; o < kBlock; o <<= 1) {
-            uint32_t a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
+            const uint32_t a = threadIdx.x >= o ? sh[threadIdx.x - o] : 0u;
             __syncthreads();
             sh[threadIdx.x] += a;
             __syncthreads();
@@ -315,13 +379,8 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
         uint32_t rank = 0;
         for (uint32_t k = s0; k < e0; ++k) rank += P.items_g[k] < g ? 1u : 0u;
         const uint32_t d = s0 + rank;
-        P.sx[d] = P.cx[m];
-        P.sy[d] = P.cy[m];
-        P.sz[d] = P.cz[m];
-        P.svx[d] = P.cvx[m];
-        P.svy[d] = P.cvy[m];
-        P.svz[d] = P.cvz[m];
-        P.sm[d] = P.cm[m];
+        P.sA[d] = P.cA[m];
+        P.sV[d] = P.cV[m];
         P.sg[d] = g;
     }
 }
@@ -336,53 +395,41 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
         const uint32_t i = b * kBlock + threadIdx.x;
         if (i >= sp.pbase + sp.n) continue;
         const SceneGrid &g = P.grid[s];
-        const float px = P.sx[i], py = P.sy[i], pz = P.sz[i];
-        const bool fluid_i = P.sg[i] < sp.nf;
-        const int cx = (int)axis_bin(g.gmin[0], px, g.inv, g.dims[0]);
-        const int cy = (int)axis_bin(g.gmin[1], py, g.inv, g.dims[1]);
-        const int cz = (int)axis_bin(g.gmin[2], pz, g.inv, g.dims[2]);
-        const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
+        const float4 ai = P.sA[i];
+        const bool fluid_i = ai.w > 0.0f;
+        const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
         const float h2 = sp.h2, k6 = sp.k6;
         float sum = fluid_i ? sp.w0 : 0.0f;
         float sum_b = 0.0f;
         uint32_t cnt = 0;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int z = cz + dz;
-            if (z < 0 || z >= nz) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int y = cy + dy;
-                if (y < 0 || y >= ny) continue;
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int x = cx + dx;
-                    if (x < 0 || x >= nx) continue;
-                    const uint32_t c = g.cbase + ((uint32_t)x * (uint32_t)ny + (uint32_t)y) * (uint32_t)nz + (uint32_t)z;
-                    const uint32_t j0 = P.start[c], j1 = P.start[c + 1];
-                    for (uint32_t j = j0; j < j1; ++j) {
-                        if (j == i) continue;
-                        const float ddx = px - P.sx[j], ddy = py - P.sy[j], ddz = pz - P.sz[j];
-                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        if (!(r2 <= h2)) continue;
-                        ++cnt;
-                        const bool fluid_j = P.sg[j] < sp.nf;
-                        const float t = h2 - r2;
-                        if (fluid_i) {
-                            const float w = ((k6 * t) * t) * t;
-                            if (fluid_j) sum += w;
-                            else sum_b += P.sm[j] * w;
-                        } else if (fluid_j) {
-                            sum += P.sm[j] * (((k6 * t) * t) * t);
-                        }
-                    }
+        const float4 *__restrict__ sA = P.sA;
+        for_rows(g, P.start, cur, [&](uint32_t j0, uint32_t j1) {
+            for (uint32_t j = j0; j < j1; ++j) {
+                if (j == i) continue;
+                const float4 aj = sA[j];
+                const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (!(r2 <= h2)) continue;
+                ++cnt;
+                const float t = h2 - r2;
+                if (fluid_i) {
+                    const float w = ((k6 * t) * t) * t;
+                    if (aj.w > 0.0f)
+                        sum += w;
+                    else
+                        sum_b += (-aj.w) * w;
+                } else if (aj.w > 0.0f) {
+                    sum += aj.w * (((k6 * t) * t) * t);
                 }
             }
-        }
-        const float rho = fluid_i ? sp.mass * sum + sum_b : P.sm[i] * sp.w0 + sum;
+        });
+        const float rho = fluid_i ? sp.mass * sum + sum_b : (-ai.w) * sp.w0 + sum;
         const float q = rho / sp.rho0;
         const float q2 = q * q;
         const float q4 = q2 * q2;
         float p = sp.b_tait * ((q * q2) * q4 - 1.0f);
         if (sp.clamp_neg && p < 0.0f) p = 0.0f;
-        P.dens[i] = rho;
+        P.sV[i].w = rho;
         P.press[i] = p;
         P.c2[i] = p / (rho * rho);
         if (P.ncount) P.ncount[i] = cnt;
@@ -403,93 +450,80 @@ __global__ __launch_bounds__(kBlock) void k_force(Pool P) {
         const bool live = i < sp.pbase + sp.n;
         float ox = 0, oy = 0, oz = 0;
         if (live) {
-            const uint32_t gi = P.sg[i];
-            const float px = P.sx[i], py = P.sy[i], pz = P.sz[i];
-            const float vix = P.svx[i], viy = P.svy[i], viz = P.svz[i];
-            if (gi < sp.nf) {
+            const float4 ai = P.sA[i];
+            const float4 vi = P.sV[i];
+            float4 vo = make_float4(vi.x, vi.y, vi.z, 0.0f);
+            ox = ai.x; oy = ai.y; oz = ai.z;
+            if (ai.w > 0.0f) {
                 const SceneGrid &g = P.grid[s];
-                const int cx = (int)axis_bin(g.gmin[0], px, g.inv, g.dims[0]);
-                const int cy = (int)axis_bin(g.gmin[1], py, g.inv, g.dims[1]);
-                const int cz = (int)axis_bin(g.gmin[2], pz, g.inv, g.dims[2]);
-                const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
+                const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
                 const float h = sp.h, h2 = sp.h2, k6 = sp.k6, ks = sp.ks, alpha_c = sp.alpha_c;
-                const float rho_i = P.dens[i];
+                const float rho_i = vi.w;
                 const float ci2 = P.c2[i];
                 float ax = sp.gx, ay = sp.gy, az = sp.gz;
                 float xx = 0.0f, xy = 0.0f, xz = 0.0f;
-                for (int dz = -1; dz <= 1; ++dz) {
-                    const int z = cz + dz;
-                    if (z < 0 || z >= nz) continue;
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int y = cy + dy;
-                        if (y < 0 || y >= ny) continue;
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            const int x = cx + dx;
-                            if (x < 0 || x >= nx) continue;
-                            const uint32_t c = g.cbase + ((uint32_t)x * (uint32_t)ny + (uint32_t)y) * (uint32_t)nz + (uint32_t)z;
-                            const uint32_t j0 = P.start[c], j1 = P.start[c + 1];
-                            for (uint32_t j = j0; j < j1; ++j) {
-                                if (j == i) continue;
-                                const float ddx = px - P.sx[j], ddy = py - P.sy[j], ddz = pz - P.sz[j];
-                                const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                                if (!(r2 <= h2)) continue;
-                                const float rho_j = P.dens[j];
-                                const float cj2 = P.c2[j];
-                                const float r = sqrtf(r2);
-                                const float t = h - r;
-                                const float mj = P.sm[j];
-                                const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
-                                const float denom = fmaxf(r, 1e-9f);
-                                const float qp = coef / denom;
-                                ax = ax + ddx * qp;
-                                ay = ay + ddy * qp;
-                                az = az + ddz * qp;
-                                const float vjx = P.svx[j], vjy = P.svy[j], vjz = P.svz[j];
-                                const float wx = vix - vjx, wy = viy - vjy, wz = viz - vjz;
-                                const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
-                                if (vdn > 0.0f) {
-                                    const float mu = (vdn * h) / (r2 + 0.01f * h2);
-                                    const float cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
-                                    const float qv = cv / denom;
-                                    ax = ax + ddx * qv;
-                                    ay = ay + ddy * qv;
-                                    az = az + ddz * qv;
-                                }
-                                const float tt = h2 - r2;
-                                const float w = ((k6 * tt) * tt) * tt;
-                                const float fx = ((mj * 2.0f) / (rho_i + rho_j)) * w;
-                                xx = xx + (vjx - vix) * fx;
-                                xy = xy + (vjy - viy) * fx;
-                                xz = xz + (vjz - viz) * fx;
-                            }
+                const float4 *__restrict__ sA = P.sA;
+                const float4 *__restrict__ sV = P.sV;
+                const float *__restrict__ c2 = P.c2;
+                for_rows(g, P.start, cur, [&](uint32_t j0, uint32_t j1) {
+                    for (uint32_t j = j0; j < j1; ++j) {
+                        if (j == i) continue;
+                        const float4 aj = sA[j];
+                        const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        if (!(r2 <= h2)) continue;
+                        const float4 vj = sV[j];
+                        const float rho_j = vj.w;
+                        const float cj2 = c2[j];
+                        const float r = sqrtf(r2);
+                        const float t = h - r;
+                        const float mj = fabsf(aj.w);
+                        const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
+                        const float denom = fmaxf(r, 1e-9f);
+                        const float qp = coef / denom;
+                        ax = ax + ddx * qp;
+                        ay = ay + ddy * qp;
+                        az = az + ddz * qp;
+                        const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
+                        const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
+                        if (vdn > 0.0f) {
+                            const float mu = (vdn * h) / (r2 + 0.01f * h2);
+                            const float cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+                            const float qv = cv / denom;
+                            ax = ax + ddx * qv;
+                            ay = ay + ddy * qv;
+                            az = az + ddz * qv;
                         }
+                        const float tt = h2 - r2;
+                        const float w = ((k6 * tt) * tt) * tt;
+                        const float fx = ((mj * 2.0f) / (rho_i + rho_j)) * w;
+                        xx = xx + (vj.x - vi.x) * fx;
+                        xy = xy + (vj.y - vi.y) * fx;
+                        xz = xz + (vj.z - vi.z) * fx;
                     }
-                }
-                if (P.ax) {
-                    P.ax[i] = ax; P.ay[i] = ay; P.az[i] = az;
-                    P.xx[i] = xx; P.xy[i] = xy; P.xz[i] = xz;
+                });
+                if (P.acc) {
+                    P.acc[i] = make_float4(ax, ay, az, 0.0f);
+                    P.xsph[i] = make_float4(xx, xy, xz, 0.0f);
                 }
                 // fluid_step.rs:110-123
                 const float dt = sp.dt, eps = sp.eps;
-                float vx = (vix + ax * dt) + xx * eps;
-                float vy = (viy + ay * dt) + xy * eps;
-                float vz = (viz + az * dt) + xz * eps;
+                float vx = (vi.x + ax * dt) + xx * eps;
+                float vy = (vi.y + ay * dt) + xy * eps;
+                float vz = (vi.z + az * dt) + xz * eps;
                 const float s2 = vx * vx + vy * vy + vz * vz;
                 if (s2 > sp.vmax2) {
                     const float k = sp.vmax / sqrtf(s2);
                     vx = vx * k; vy = vy * k; vz = vz * k;
                 }
-                ox = px + vx * dt;
-                oy = py + vy * dt;
-                oz = pz + vz * dt;
-                P.cvx[i] = vx; P.cvy[i] = vy; P.cvz[i] = vz;
-            } else {
-                ox = px; oy = py; oz = pz;
-                P.cvx[i] = vix; P.cvy[i] = viy; P.cvz[i] = viz;
+                ox = ai.x + vx * dt;
+                oy = ai.y + vy * dt;
+                oz = ai.z + vz * dt;
+                vo = make_float4(vx, vy, vz, 0.0f);
             }
-            P.cx[i] = ox; P.cy[i] = oy; P.cz[i] = oz;
-            P.cm[i] = P.sm[i];
-            P.cg[i] = gi;
+            P.cA[i] = make_float4(ox, oy, oz, ai.w);
+            P.cV[i] = vo;
+            P.cg[i] = P.sg[i];
         }
         block_box(live, ox, oy, oz, &P.box[par * P.nscenes + s]);
     }
@@ -509,69 +543,55 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
         if (!sp.react) continue;
         const uint32_t j = b * kBlock + threadIdx.x;
         if (j >= sp.pbase + sp.n) continue;
+        const float4 aj = P.sA[j];
+        if (aj.w > 0.0f) continue;
         const uint32_t gj = P.sg[j];
-        if (gj < sp.nf) continue;
         const SceneGrid &g = P.grid[s];
-        const float px = P.sx[j], py = P.sy[j], pz = P.sz[j];
-        const float vjx = P.svx[j], vjy = P.svy[j], vjz = P.svz[j];
-        const float rho_j = P.dens[j], cj2 = P.c2[j], mj = P.sm[j];
-        const int cx = (int)axis_bin(g.gmin[0], px, g.inv, g.dims[0]);
-        const int cy = (int)axis_bin(g.gmin[1], py, g.inv, g.dims[1]);
-        const int cz = (int)axis_bin(g.gmin[2], pz, g.inv, g.dims[2]);
-        const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
+        const float4 vj = P.sV[j];
+        const float rho_j = vj.w, cj2 = P.c2[j], mj = -aj.w;
+        const Cursor cur = make_cursor(g, sp.h, aj.x, aj.y, aj.z);
         const float h = sp.h, h2 = sp.h2, ks = sp.ks, alpha_c = sp.alpha_c;
         uint32_t kid[kReactMaxNeighbours];
         float kx[kReactMaxNeighbours], ky[kReactMaxNeighbours], kz[kReactMaxNeighbours];
         uint32_t cnt = 0;
         bool over = false;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int z = cz + dz;
-            if (z < 0 || z >= nz) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int y = cy + dy;
-                if (y < 0 || y >= ny) continue;
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int x = cx + dx;
-                    if (x < 0 || x >= nx) continue;
-                    const uint32_t c = g.cbase + ((uint32_t)x * (uint32_t)ny + (uint32_t)y) * (uint32_t)nz + (uint32_t)z;
-                    const uint32_t i0 = P.start[c], i1 = P.start[c + 1];
-                    for (uint32_t i = i0; i < i1; ++i) {
-                        const uint32_t gi = P.sg[i];
-                        if (gi >= sp.nf) continue;  // fluid neighbours only (cells list fluid first)
-                        // d = p_i - p_j as seen from the fluid particle i.
-                        const float ddx = P.sx[i] - px, ddy = P.sy[i] - py, ddz = P.sz[i] - pz;
-                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        if (!(r2 <= h2)) continue;
-                        const float rho_i = P.dens[i], ci2 = P.c2[i], mi = P.sm[i];
-                        const float r = sqrtf(r2);
-                        const float t = h - r;
-                        const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
-                        const float denom = fmaxf(r, 1e-9f);
-                        const float wx = P.svx[i] - vjx, wy = P.svy[i] - vjy, wz = P.svz[i] - vjz;
-                        const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
-                        float cv = 0.0f;
-                        if (vdn > 0.0f) {
-                            const float mu = (vdn * h) / (r2 + 0.01f * h2);
-                            cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
-                        }
-                        const float q = (mi * (coef + cv)) / denom;
-                        if (cnt == kReactMaxNeighbours) {
-                            over = true;
-                            continue;
-                        }
-                        // insertion by original index
-                        uint32_t k = cnt++;
-                        while (k > 0 && kid[k - 1] > gi) {
-                            kid[k] = kid[k - 1];
-                            kx[k] = kx[k - 1]; ky[k] = ky[k - 1]; kz[k] = kz[k - 1];
-                            --k;
-                        }
-                        kid[k] = gi;
-                        kx[k] = ddx * q; ky[k] = ddy * q; kz[k] = ddz * q;
-                    }
+        for_rows(g, P.start, cur, [&](uint32_t i0, uint32_t i1) {
+            for (uint32_t i = i0; i < i1; ++i) {
+                const float4 ai = P.sA[i];
+                if (!(ai.w > 0.0f)) continue;  // fluid neighbours only
+                // d = p_i - p_j as seen from the fluid particle i.
+                const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (!(r2 <= h2)) continue;
+                const float4 vi = P.sV[i];
+                const float rho_i = vi.w, ci2 = P.c2[i], mi = ai.w;
+                const float r = sqrtf(r2);
+                const float t = h - r;
+                const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
+                const float denom = fmaxf(r, 1e-9f);
+                const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
+                const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
+                float cv = 0.0f;
+                if (vdn > 0.0f) {
+                    const float mu = (vdn * h) / (r2 + 0.01f * h2);
+                    cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
                 }
+                const float q = (mi * (coef + cv)) / denom;
+                if (cnt == kReactMaxNeighbours) {
+                    over = true;
+                    continue;
+                }
+                const uint32_t gi = P.sg[i];
+                uint32_t k = cnt++;  // insertion by original index
+                while (k > 0 && kid[k - 1] > gi) {
+                    kid[k] = kid[k - 1];
+                    kx[k] = kx[k - 1]; ky[k] = ky[k - 1]; kz[k] = kz[k - 1];
+                    --k;
+                }
+                kid[k] = gi;
+                kx[k] = ddx * q; ky[k] = ddy * q; kz[k] = ddz * q;
             }
-        }
+        });
         if (over) atomicOr(&P.ctl->overflow, 1u);
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         for (uint32_t k = 0; k < cnt; ++k) {
@@ -580,8 +600,8 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
             fz = fz - kz[k];
         }
         const uint32_t d = sp.pbase + gj;
-        P.bfx[d] = fx; P.bfy[d] = fy; P.bfz[d] = fz;
-        P.bpx[d] = px; P.bpy[d] = py; P.bpz[d] = pz;
+        P.bf[d] = make_float4(fx, fy, fz, 0.0f);
+        P.bp[d] = make_float4(aj.x, aj.y, aj.z, 0.0f);
     }
 }
 
@@ -595,12 +615,13 @@ __global__ __launch_bounds__(64) void k_reactions(Pool P) {
     const SpanDev sd = P.spans[k];
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
     for (uint32_t q = sd.start; q < sd.end; ++q) {
-        const float fx = P.bfx[q], fy = P.bfy[q], fz = P.bfz[q];
-        Fx = Fx + fx; Fy = Fy + fy; Fz = Fz + fz;
-        const float rx = P.bpx[q] - sd.origin[0], ry = P.bpy[q] - sd.origin[1], rz = P.bpz[q] - sd.origin[2];
-        Tx = Tx + (ry * fz - rz * fy);
-        Ty = Ty + (rz * fx - rx * fz);
-        Tz = Tz + (rx * fy - ry * fx);
+        const float4 f = P.bf[q];
+        const float4 p = P.bp[q];
+        Fx = Fx + f.x; Fy = Fy + f.y; Fz = Fz + f.z;
+        const float rx = p.x - sd.origin[0], ry = p.y - sd.origin[1], rz = p.z - sd.origin[2];
+        Tx = Tx + (ry * f.z - rz * f.y);
+        Ty = Ty + (rz * f.x - rx * f.z);
+        Tz = Tz + (rx * f.y - ry * f.x);
     }
     const uint32_t sub = sd.substeps;
     const uint32_t phase = (P.ctl->step - 1u) % sub;  // substep index inside the tick
@@ -617,23 +638,23 @@ __global__ __launch_bounds__(64) void k_reactions(Pool P) {
     }
 }
 
-// Scatter cur state to index order (pool slot pbase + gid).
-__global__ __launch_bounds__(kBlock) void k_unsort(Pool P, float *ox, float *oy, float *oz,
-                                                   float *ovx, float *ovy, float *ovz) {
+// Scatter cur state to index order (AoS, pool slot pbase + gid).
+__global__ __launch_bounds__(kBlock) void k_unsort(Pool P, float *opos, float *ovel) {
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
         const uint32_t m = b * kBlock + threadIdx.x;
         if (m >= sp.pbase + sp.n) continue;
-        const uint32_t d = sp.pbase + P.cg[m];
-        ox[d] = P.cx[m]; oy[d] = P.cy[m]; oz[d] = P.cz[m];
-        ovx[d] = P.cvx[m]; ovy[d] = P.cvy[m]; ovz[d] = P.cvz[m];
+        const size_t d = 3ull * (sp.pbase + P.cg[m]);
+        const float4 a = P.cA[m], v = P.cV[m];
+        opos[d] = a.x; opos[d + 1] = a.y; opos[d + 2] = a.z;
+        ovel[d] = v.x; ovel[d + 1] = v.y; ovel[d + 2] = v.z;
     }
 }
 
 // Scatter sorted per-particle fields to index order (pool slot pbase + gid).
-__global__ __launch_bounds__(kBlock) void k_unsort_fields(Pool P, float *odens, float *opress,
-                                                          uint32_t *oncount, float *oacc, float *oxs) {
+__global__ __launch_bounds__(kBlock) void k_unsort_fields(Pool P, float *odens, float *opress, uint32_t *oncount,
+                                                          float *oacc, float *oxs) {
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
@@ -641,12 +662,13 @@ __global__ __launch_bounds__(kBlock) void k_unsort_fields(Pool P, float *odens, 
         if (i >= sp.pbase + sp.n) continue;
         const uint32_t gi = P.sg[i];
         const uint32_t d = sp.pbase + gi;
-        odens[d] = P.dens[i];
+        odens[d] = P.sV[i].w;
         opress[d] = P.press[i];
         if (oncount && P.ncount) oncount[d] = P.ncount[i];
-        if (oacc && P.ax && gi < sp.nf) {
-            oacc[3 * d] = P.ax[i]; oacc[3 * d + 1] = P.ay[i]; oacc[3 * d + 2] = P.az[i];
-            oxs[3 * d] = P.xx[i]; oxs[3 * d + 1] = P.xy[i]; oxs[3 * d + 2] = P.xz[i];
+        if (oacc && P.acc && gi < sp.nf) {
+            const float4 a = P.acc[i], x = P.xsph[i];
+            oacc[3ull * d] = a.x; oacc[3ull * d + 1] = a.y; oacc[3ull * d + 2] = a.z;
+            oxs[3ull * d] = x.x; oxs[3ull * d + 1] = x.y; oxs[3ull * d + 2] = x.z;
         }
     }
 }

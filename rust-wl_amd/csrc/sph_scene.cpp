@@ -11,21 +11,10 @@ namespace {
 
 constexpr float kPi = 3.14159274101257324f;  // std::f32::consts::PI
 
-// `powi` with a constant exponent: binary exponentiation (lowest set bit
-// first), the expansion the reference's h.powi(9) / h.powi(6) compile to.
-float powi(float x, int n) {
-    float res = 1.0f, cur = x;
-    bool have = false;
-    while (n) {
-        if (n & 1) {
-            res = have ? res * cur : cur;
-            have = true;
-        }
-        n >>= 1;
-        if (n) cur = cur * cur;
-    }
-    return res;
-}
+// h.powi(n) as the reference binaries evaluate it: with the smoothing
+// radius a compile-time constant, LLVM folds llvm.powi.f32 through pow() in
+// double precision and rounds once to f32.
+float powi(float x, int n) { return (float)std::pow((double)x, (double)n); }
 
 float poly6_k6(float h) { return 315.0f / ((64.0f * kPi) * powi(h, 9)); }
 

@@ -213,6 +213,28 @@ VxlSphStatus vxl_sph_engine_device_bytes(VxlSphEngine *e, uint64_t *out) {
     return VXL_SPH_OK;
 }
 
+VxlSphStatus vxl_sph_engine_profile(VxlSphEngine *e, uint32_t substeps, uint32_t cap, VxlSphKernelTime *out,
+                                    uint32_t *count) {
+    if (!e || !count || (cap && !out)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        const auto t = e->eng.profile(substeps);
+        *count = (uint32_t)t.size();
+        for (uint32_t k = 0; k < cap && k < t.size(); ++k) {
+            std::memset(out[k].name, 0, sizeof(out[k].name));
+            std::strncpy(out[k].name, t[k].first.c_str(), sizeof(out[k].name) - 1);
+            out[k].ms_total = t[k].second;
+        }
+    });
+}
+
+VxlSphStatus vxl_sph_engine_step_timed(VxlSphEngine *e, uint32_t ticks, float *ms_out) {
+    if (!e || (ticks && !ms_out)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        const auto t = e->eng.step_timed(ticks);
+        std::memcpy(ms_out, t.data(), 4ull * t.size());
+    });
+}
+
 VxlSphStatus vxl_sph_scene_add(VxlSphEngine *e, const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid,
                                const float *pos, const float *vel, const float *pmass, const VxlSphSpan *spans,
                                uint32_t nspans, uint32_t *out_scene) {
@@ -381,6 +403,49 @@ VxlSphStatus vxl_sph_box_lattice(const float half[3], float sp, float h, uint32_
         if (out_n0) *out_n0 = n0;
         if (out_volume) *out_volume = vol;
         if (out_pts) std::memcpy(out_pts, pts.data(), 12ull * std::min(n, cap));
+    });
+}
+
+VxlSphStatus vxl_sph_box_particles(const VxlSphFluidConfig *cfg, uint32_t nbodies, const VxlSphBoxBody *bodies,
+                                   uint32_t first, uint32_t *n, uint32_t *nspans, float *pos, float *vel,
+                                   float *pmass, VxlSphSpan *spans) {
+    if (!cfg || !n || !nspans || (nbodies && !bodies)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        std::vector<BoxBody> bs(nbodies);
+        for (uint32_t k = 0; k < nbodies; ++k) {
+            bs[k].id = bodies[k].id;
+            for (int a = 0; a < 3; ++a) {
+                bs[k].half[a] = bodies[k].half[a];
+                bs[k].pos[a] = bodies[k].pos[a];
+                bs[k].linvel[a] = bodies[k].linvel[a];
+                bs[k].angvel[a] = bodies[k].angvel[a];
+            }
+            for (int a = 0; a < 4; ++a) bs[k].rot[a] = bodies[k].rot[a];
+        }
+        std::vector<float> p, v, m;
+        std::vector<Span> sp;
+        p.resize(3ull * first);  // spans index the whole particle array
+        v.resize(3ull * first);
+        m.resize(first);
+        append_boundary_boxes(to_scene_config(*cfg), bs, p, v, m, sp);
+        const uint32_t count = (uint32_t)m.size() - first;
+        const bool fill = pos != nullptr;
+        if (fill && (*n < count || *nspans < sp.size())) throw std::length_error("output buffers too small");
+        *n = count;
+        *nspans = (uint32_t)sp.size();
+        if (!fill) return;
+        std::memcpy(pos, p.data() + 3ull * first, 12ull * count);
+        if (vel) std::memcpy(vel, v.data() + 3ull * first, 12ull * count);
+        if (pmass) std::memcpy(pmass, m.data() + first, 4ull * count);
+        if (spans)
+            for (size_t k = 0; k < sp.size(); ++k) {
+                spans[k].body = sp[k].body;
+                for (int a = 0; a < 3; ++a) spans[k].origin[a] = sp[k].origin[a];
+                spans[k].start = sp[k].start;
+                spans[k].end = sp[k].end;
+            }
     });
 }
 

@@ -62,6 +62,10 @@ class FluidConfig(C.Structure):
     ]
 
 
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("ms_total", C.c_float)]
+
+
 class Span(C.Structure):
     _fields_ = [("body", C.c_uint32), ("origin", C.c_float * 3), ("start", C.c_uint32), ("end", C.c_uint32)]
 
@@ -73,6 +77,11 @@ class Reaction(C.Structure):
 class GridInfo(C.Structure):
     _fields_ = [("min", C.c_float * 3), ("inv", C.c_float), ("bin", C.c_float), ("dims", C.c_uint32 * 3),
                 ("total", C.c_uint32)]
+
+
+class BoxBody(C.Structure):
+    _fields_ = [("id", C.c_uint32), ("half", C.c_float * 3), ("pos", C.c_float * 3), ("rot", C.c_float * 4),
+                ("linvel", C.c_float * 3), ("angvel", C.c_float * 3)]
 
 
 class DamBreakSpec(C.Structure):
@@ -88,12 +97,13 @@ EXPORTS = (
     "vxl_sph_device_count", "vxl_sph_last_error", "vxl_sph_default_config",
     "vxl_sph_engine_create", "vxl_sph_engine_destroy", "vxl_sph_engine_set_diagnostics",
     "vxl_sph_engine_set_graphs", "vxl_sph_engine_step", "vxl_sph_engine_sync", "vxl_sph_engine_device_bytes",
+    "vxl_sph_engine_profile", "vxl_sph_engine_step_timed",
     "vxl_sph_scene_add", "vxl_sph_scene_remove", "vxl_sph_scene_upload_boundary", "vxl_sph_scene_read_state",
     "vxl_sph_scene_read_fields", "vxl_sph_scene_read_grid", "vxl_sph_scene_read_bounds",
     "vxl_sph_scene_state_hash",
     "vxl_sph_stepper_create", "vxl_sph_stepper_destroy", "vxl_sph_stepper_step", "vxl_sph_stepper_reactions",
     "vxl_sph_stepper_bounds", "vxl_sph_stepper_read_state",
-    "vxl_sph_box_lattice", "vxl_sph_fluid_constants", "vxl_sph_dam_break",
+    "vxl_sph_box_lattice", "vxl_sph_box_particles", "vxl_sph_fluid_constants", "vxl_sph_dam_break",
 )
 
 
@@ -131,6 +141,10 @@ def lib():
     L.vxl_sph_engine_step.argtypes = [P, C.c_uint32]
     L.vxl_sph_engine_device_bytes.restype = st
     L.vxl_sph_engine_device_bytes.argtypes = [P, C.POINTER(C.c_uint64)]
+    L.vxl_sph_engine_profile.restype = st
+    L.vxl_sph_engine_profile.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(KernelTime), up]
+    L.vxl_sph_engine_step_timed.restype = st
+    L.vxl_sph_engine_step_timed.argtypes = [P, C.c_uint32, fp]
     L.vxl_sph_scene_add.restype = st
     L.vxl_sph_scene_add.argtypes = [P, C.POINTER(FluidConfig), C.c_uint32, C.c_uint32, fp, fp, fp,
                                     C.POINTER(Span), C.c_uint32, up]
@@ -163,6 +177,9 @@ def lib():
     L.vxl_sph_stepper_read_state.argtypes = [P, C.c_uint32, fp, fp]
     L.vxl_sph_box_lattice.restype = st
     L.vxl_sph_box_lattice.argtypes = [fp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, fp, up, up, fp]
+    L.vxl_sph_box_particles.restype = st
+    L.vxl_sph_box_particles.argtypes = [C.POINTER(FluidConfig), C.c_uint32, C.POINTER(BoxBody), C.c_uint32, up, up,
+                                        fp, fp, fp, C.POINTER(Span)]
     L.vxl_sph_fluid_constants.restype = st
     L.vxl_sph_fluid_constants.argtypes = [C.POINTER(FluidConfig), fp]
     L.vxl_sph_dam_break.restype = st
@@ -225,6 +242,30 @@ def box_lattice(half, s, h, layers):
     _check(lib().vxl_sph_box_lattice(_f(half), C.c_float(s), C.c_float(h), layers, n.value, _f(pts), C.byref(n),
                                      C.byref(n0), C.byref(vol)))
     return pts[:n.value], n0.value, np.float32(vol.value)
+
+
+def box_particles(cfg, bodies, first):
+    """set_boundary_particles for Box bodies (fluid_step.rs:13-53): returns
+    (pos, vel, pmass, spans) with span indices starting at `first`."""
+    arr = (BoxBody * max(len(bodies), 1))()
+    for k, b in enumerate(bodies):
+        arr[k].id = b["id"]
+        arr[k].half = (C.c_float * 3)(*b["half"])
+        arr[k].pos = (C.c_float * 3)(*b["pos"])
+        arr[k].rot = (C.c_float * 4)(*b.get("rot", (0.0, 0.0, 0.0, 1.0)))
+        arr[k].linvel = (C.c_float * 3)(*b.get("linvel", (0.0, 0.0, 0.0)))
+        arr[k].angvel = (C.c_float * 3)(*b.get("angvel", (0.0, 0.0, 0.0)))
+    n, ns = C.c_uint32(), C.c_uint32()
+    _check(lib().vxl_sph_box_particles(C.byref(cfg), len(bodies), arr, first, C.byref(n), C.byref(ns),
+                                       None, None, None, None))
+    pos = np.zeros((n.value, 3), np.float32)
+    vel = np.zeros((n.value, 3), np.float32)
+    pm = np.zeros(n.value, np.float32)
+    sp = (Span * max(ns.value, 1))()
+    _check(lib().vxl_sph_box_particles(C.byref(cfg), len(bodies), arr, first, C.byref(n), C.byref(ns),
+                                       _f(pos), _f(vel), _f(pm), sp))
+    spans = [(sp[k].body, tuple(sp[k].origin), sp[k].start, sp[k].end) for k in range(ns.value)]
+    return pos, vel, pm, spans
 
 
 class SceneData:
@@ -337,6 +378,19 @@ class Engine:
         v = C.c_uint64()
         _check(lib().vxl_sph_engine_device_bytes(self.h, C.byref(v)))
         return v.value
+
+    def profile(self, substeps):
+        """{kernel name: total ms} over `substeps` eager substeps (HIP events)."""
+        arr = (KernelTime * 32)()
+        cnt = C.c_uint32()
+        _check(lib().vxl_sph_engine_profile(self.h, substeps, 32, arr, C.byref(cnt)))
+        return {arr[k].name.decode(): float(arr[k].ms_total) for k in range(min(cnt.value, 32))}
+
+    def step_timed(self, ticks):
+        """Device ms of each of `ticks` ticks."""
+        ms = np.zeros(max(ticks, 1), np.float32)
+        _check(lib().vxl_sph_engine_step_timed(self.h, ticks, _f(ms)))
+        return ms[:ticks]
 
     def upload_boundary(self, sid, pos, vel):
         pos = np.ascontiguousarray(pos, np.float32)

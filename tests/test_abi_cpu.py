@@ -1,0 +1,125 @@
+"""The C-ABI library on the CPU: it loads, exports every symbol the header
+declares, rejects misuse with the reference's error codes, and its host-side
+scene construction (constants, Akinci lattice, dam-break builder) matches the
+oracle. No device calls."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+import vxl_sph as V
+from scenes import bits, oracle_config, tank_bodies
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "vxl_sph.h")).read()
+    return sorted(set(re.findall(r"\b(vxl_sph_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = V.lib()
+    syms = header_symbols()
+    assert len(syms) >= 30
+    for name in syms:
+        assert hasattr(L, name), name
+    assert set(syms) == set(V.EXPORTS)
+
+
+def test_abi_version_and_backend_name():
+    assert V.lib().vxl_sph_abi_version() == V.ABI_VERSION == 1
+    assert V.Backend().name() == "hip-gfx950"
+    assert V.Backend().submit("cloth") is False
+
+
+def test_error_codes():
+    L = V.lib()
+    h = C.c_void_p()
+    assert L.vxl_sph_engine_create(V.ABI_VERSION + 1, 0, C.byref(h)) == 2  # VersionMismatch
+    assert L.vxl_sph_engine_create(V.ABI_VERSION, 0, None) == 1            # NullPointer
+    cfg = V.fluid_config()
+    cfg.hdr.size = 8
+    out = np.zeros(8, np.float32)
+    assert L.vxl_sph_fluid_constants(C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_float))) == 3  # SizeMismatch
+    cfg = V.fluid_config()
+    cfg.hdr.version = 9
+    assert L.vxl_sph_fluid_constants(C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_float))) == 2
+    assert L.vxl_sph_engine_step(None, 1) == 1
+    assert L.vxl_sph_stepper_step(None, C.c_float(0.01), 0, 0, None, None, None, None, 0) == 1
+
+
+def test_engine_without_device_reports_device_error():
+    n = C.c_int32()
+    assert V.lib().vxl_sph_device_count(C.byref(n)) == 0
+    if n.value > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(V.VxlSphError) as e:
+        V.Engine()
+    assert e.value.status == "DEVICE_ERROR"
+
+
+def test_constants_match_oracle():
+    for cfg in (V.fluid_config(), V.dam_break_config(0.02), V.dam_break_config(0.01, c0=27.0, alpha=0.04)):
+        c = V.fluid_constants(cfg)
+        if cfg.particle_mass > 0:
+            s = O.System.particles(oracle_config(cfg), np.zeros((1, 3), np.float32), None,
+                                   np.float32(cfg.particle_mass), np.float32(cfg.spacing))
+        else:
+            s = O.System.lattice(oracle_config(cfg), (0, 0, 0), (1, 1, 1), cfg.spacing)
+        oc = s.constants()
+        for k in oc:
+            assert bits([c[k]])[0] == bits([oc[k]])[0], k
+
+
+@pytest.mark.parametrize("half,s,h,layers", [((0.06,) * 3, 0.05, 0.1, 2), ((1.04, 0.02, 0.54), 0.02, 0.04, 1),
+                                             ((0.3, 0.05, 0.3), 0.05, 0.1, 2), ((0.12, 0.2, 0.05), 0.02, 0.04, 3)])
+def test_box_lattice_matches_oracle(half, s, h, layers):
+    p1, n01, v1 = V.box_lattice(half, s, h, layers)
+    p2, n02, v2 = O.lattice_box(half, s, h, layers)
+    assert n01 == n02 and len(p1) == len(p2)
+    np.testing.assert_array_equal(bits(p1), bits(p2))
+    assert bits([v1])[0] == bits([v2])[0]
+
+
+def test_dam_break_tank_matches_oracle_lattice():
+    """The library's tank particles equal the oracle's set_boundary_particles
+    of the restated tank bodies, bit for bit (positions, masses, spans)."""
+    d = 0.02
+    tank = (0.6, 0.4, 0.5)
+    cfg = V.dam_break_config(d)
+    data = V.dam_break(cfg, tank, (10, 8, 6), d, seed=4)
+    nf = data.n_fluid
+    assert nf == 480
+    orc = O.System.particles(oracle_config(cfg), data.pos[:nf], data.vel[:nf], np.float32(cfg.particle_mass),
+                             np.float32(d))
+    nb = orc.set_boundary_boxes(tank_bodies(tank, d))
+    assert nb == data.n - nf
+    np.testing.assert_array_equal(bits(orc.pos()), bits(data.pos))
+    np.testing.assert_array_equal(bits(orc.pmass()), bits(data.pmass))
+    assert [s[0] for s in data.spans] == [0, 1, 2, 3, 4]
+
+
+def test_dam_break_block_lattice_and_jitter():
+    d = np.float32(0.02)
+    cfg = V.dam_break_config(0.02)
+    a = V.dam_break(cfg, (1.0, 1.0, 1.0), (4, 3, 2), 0.02, seed=9, walls=False)
+    b = V.dam_break(cfg, (1.0, 1.0, 1.0), (4, 3, 2), 0.02, seed=9, walls=False)
+    c = V.dam_break(cfg, (1.0, 1.0, 1.0), (4, 3, 2), 0.02, seed=10, walls=False)
+    assert np.array_equal(bits(a.pos), bits(b.pos)) and not np.array_equal(bits(a.pos), bits(c.pos))
+    i, j, k = np.meshgrid(np.arange(4), np.arange(3), np.arange(2), indexing="ij")
+    base = (np.stack([i, j, k], -1).reshape(-1, 3).astype(np.float32) + np.float32(0.5)) * d
+    jit = a.pos - base
+    assert np.all(np.abs(jit) <= np.float32(0.05) * d * np.float32(1.0001))
+    assert np.abs(jit).max() > np.float32(0.01) * d
+    assert np.all(a.pmass == np.float32(cfg.particle_mass))
+
+
+def test_gamma_other_than_seven_is_unsupported_not_silent():
+    cfg = V.fluid_config(gamma_tait=5.0)
+    L = V.lib()
+    out = np.zeros(8, np.float32)
+    assert L.vxl_sph_fluid_constants(C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_float))) == 0

@@ -223,3 +223,38 @@ def test_config1_dam_break_200_steps():
     print("drift after 200 steps: max|dpos| %.3e m, max|dvel| %.3e m/s" % (
         np.abs(pos - orc.pos()).max(), np.abs(vel - orc.vel()).max()))
     assert eng.state_hash(sid) == orc.state_hash()
+
+
+def test_reference_akinci_canary_on_gpu():
+    """The reference's Akinci canary scene (packet_grid_canary.rs:27-57:
+    8^3 lattice + floor Box, FluidConfig::default, 1 + 60 ticks x 4 substeps)
+    through the FluidStepper drop-in: the fluid's lowest y must round to the
+    reference's recorded CPU readout -1.81726 (docs/PLAN-gpu.md:2704), and the
+    final state must equal the oracle's bit for bit."""
+    N, SP = 8, np.float32(0.05)
+    cfg = V.fluid_config()
+    h = np.float32(cfg.smoothing_radius)
+    orc = O.System.lattice(O.default_config(), (-N * SP * 0.5, 0.5, -N * SP * 0.5), (N, N, N), SP)
+    fpos = orc.pos()
+    nf = len(fpos)
+    half = np.float32(N) * SP * np.float32(0.5) + np.float32(4.0) * h
+    body = dict(id=0, half=(half, np.float32(2.0) * SP, half), pos=(0.0, np.float32(-2.0) * h, 0.0))
+    bpos, bvel, bpm, spans = V.box_particles(cfg, [body], nf)
+    orc.set_boundary_boxes([body])
+    pos = np.concatenate([fpos, bpos])
+    vel = np.concatenate([np.zeros_like(fpos), bvel])
+    pm = np.concatenate([np.full(nf, orc.mass, np.float32), bpm])
+    np.testing.assert_array_equal(bits(pos), bits(orc.pos()))
+    np.testing.assert_array_equal(bits(pm), bits(orc.pmass()))
+    st = V.FluidStepper(cfg)
+    for _ in range(61):
+        st.step(1.0 / 60.0, nf, pos, vel, pm, spans)
+        orc.step(1.0 / 60.0)
+    gpos, gvel = st.read_state()
+    assert round(float(gpos[:nf, 1].min()), 5) == -1.81726
+    np.testing.assert_array_equal(bits(gpos), bits(orc.pos()))
+    np.testing.assert_array_equal(bits(gvel), bits(orc.vel()))
+    body_ids, ft = orc.reactions()
+    got = st.reactions()
+    gft = np.array([np.concatenate([g[1], g[2]]) for g in got], np.float32)
+    np.testing.assert_array_equal(bits(gft), bits(ft))
