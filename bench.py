@@ -83,7 +83,23 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--force-depth", type=int, default=None, help="override the force kernel's LDS list depth")
+    ap.add_argument("--sweep", action="store_true", help="print per-kernel times for every tuning variant")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device copy of BYTES (scale for the HBM byte counters) and exit")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every box here has one MI355X (asked for --gpus {args.gpus}).",
@@ -103,6 +119,19 @@ def main():
 
     eng = V.Engine()
     sid = eng.add(cfg, data, reactions=False)
+    if args.force_depth is not None:
+        eng.set_tuning(force_depth=args.force_depth)
+    else:
+        tuned = eng.autotune(2)
+        print(f"# autotuned: {tuned}", file=sys.stderr, flush=True)
+    if args.sweep:
+        eng.step(2)
+        for fd in (0, 16, 32, 64):
+            eng.set_tuning(force_depth=fd)
+            pr = eng.profile(3)
+            print(f"# sweep force_depth={fd}: force {pr['force'] / 3:.3f} ms, density {pr['density'] / 3:.3f} ms",
+                  file=sys.stderr, flush=True)
+        eng.autotune(2)
     eng.step(args.warmup)
     eng.sync()
     t0 = time.perf_counter()
@@ -166,6 +195,7 @@ def main():
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
         "kernel_algorithmic_GBs": stage_gbs,
+        "tuning": eng.get_tuning(),
     }
     print(json.dumps(out), flush=True)
 

@@ -110,6 +110,22 @@ VxlSphStatus vxl_sph_engine_create(uint32_t abi_version, int32_t device, VxlSphE
 VxlSphStatus vxl_sph_engine_destroy(VxlSphEngine *e);
 /* Enable per-particle diagnostics (neighbour counts, acceleration, XSPH). */
 VxlSphStatus vxl_sph_engine_set_diagnostics(VxlSphEngine *e, int32_t on);
+/* Launch geometry of the substep kernels (chosen by the autotuner). No
+ * counterpart in the reference (its workgroup size is fixed at 64,
+ * crates/vxl-phys-gpu/src/density.wgsl:52). 0 keeps a field's default. */
+typedef struct VxlSphTuning {
+    VxlSphHeader hdr;
+    uint32_t persist_blocks; /* grid size cap of the per-slot kernels (0 = one block per 256 slots) */
+    uint32_t force_depth;    /* LDS neighbour-list depth of the force kernel: 0 (direct), 16, 32, 64 */
+    uint32_t density_depth;  /* LDS neighbour-list depth of the density kernel: 0 (direct), 16, 32 */
+    uint32_t reserved;
+} VxlSphTuning;
+VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t);
+VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out);
+/* Autotuner: times every variant for `substeps` real substeps each (results
+ * are identical for all variants) and keeps the fastest; cached per
+ * (arch, log2 particle count) for the process. `out` may be NULL. */
+VxlSphStatus vxl_sph_engine_autotune(VxlSphEngine *e, uint32_t substeps, VxlSphTuning *out);
 /* Replay each substep as one hipGraph (default on). */
 VxlSphStatus vxl_sph_engine_set_graphs(VxlSphEngine *e, int32_t on);
 /* Advance every scene by `ticks` ticks (enqueue only; see vxl_sph_engine_sync). */

@@ -84,8 +84,16 @@ public:
     // Launch geometry knobs (autotuned). 0 = default.
     struct Tuning {
         uint32_t persist_blocks = 0;  // grid-stride grid size cap for slot kernels
+        uint32_t force_depth = 32;    // LDS neighbour-list depth of the force kernel (0, 16, 32, 64)
+        uint32_t density_depth = 0;   // LDS neighbour-list depth of the density kernel (0, 16, 32)
     };
     void set_tuning(const Tuning &t) { tuning_ = t; invalidate_graph(); }
+    // Time every launch variant on the current pool for `substeps` substeps
+    // each (they are real substeps: all variants give identical results) and
+    // keep the fastest. Results are cached per (device arch, log2 particle
+    // count) for the life of the process.
+    Tuning autotune(uint32_t substeps);
+    const Tuning &tuning() const { return tuning_; }
     uint64_t device_bytes() const { return bytes_; }
 
 private:

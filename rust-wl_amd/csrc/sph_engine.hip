@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace vxl {
 
@@ -363,7 +365,12 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     M("canon_gather");
     hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("density");
-    hipLaunchKernelGGL(k_force, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    switch (tuning_.force_depth) {
+        case 0: hipLaunchKernelGGL(k_force_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
+        case 16: hipLaunchKernelGGL(k_force_t<16>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
+        case 64: hipLaunchKernelGGL(k_force_t<64>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
+        default: hipLaunchKernelGGL(k_force_t<32>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
+    }
     M("force");
     if (nspans_) {
         hipLaunchKernelGGL(k_bforce, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
@@ -410,6 +417,52 @@ std::vector<std::pair<std::string, float>> Engine::profile(uint32_t substeps) {
     for (auto e : evs) (void)hipEventDestroy(e);
     check_error_state();
     return out;
+}
+
+Engine::Tuning Engine::autotune(uint32_t substeps) {
+    if (dirty_) layout();
+    if (scenes_.empty() || P_.nblocks == 0) return tuning_;
+    static std::mutex mu;
+    static std::map<std::pair<std::string, uint32_t>, Tuning> cache;
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
+    uint32_t bucket = 0;
+    for (uint32_t v = used_slots_; v > 1; v >>= 1) ++bucket;
+    const auto key = std::make_pair(std::string(prop.gcnArchName), bucket);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) {
+            set_tuning(it->second);
+            return tuning_;
+        }
+    }
+    const uint32_t cus = (uint32_t)prop.multiProcessorCount;
+    std::vector<Tuning> space;
+    for (uint32_t fd : {0u, 16u, 32u, 64u})
+        for (uint32_t pb : {0u, 8u * cus}) {
+            Tuning t;
+            t.force_depth = fd;
+            t.persist_blocks = pb;
+            space.push_back(t);
+        }
+    const Tuning saved = tuning_;
+    Tuning best = saved;
+    float best_ms = 1e30f;
+    const uint32_t n = std::max<uint32_t>(substeps, 1);
+    for (const Tuning &t : space) {
+        set_tuning(t);
+        float ms = 0.0f;
+        for (const auto &kv : profile(n)) ms += kv.second;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    set_tuning(best);
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = best;
+    return best;
 }
 
 std::vector<float> Engine::step_timed(uint32_t ticks) {

@@ -55,7 +55,8 @@ __global__ void k_scan_apply(Pool P, const uint32_t *tile_sums);
 __global__ void k_place(Pool P);
 __global__ void k_canon_gather(Pool P);
 __global__ void k_density(Pool P);
-__global__ void k_force(Pool P);
+template <uint32_t DEPTH>
+__global__ void k_force_t(Pool P);  // DEPTH = LDS neighbour-list depth (0 = direct)
 __global__ void k_bforce(Pool P);
 __global__ void k_reactions(Pool P);
 __global__ void k_unsort(Pool P, float *opos, float *ovel);

@@ -40,6 +40,17 @@ __device__ __forceinline__ uint32_t axis_bin(float o, float v, float inv, uint32
     return u;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2. Neighbour kernels remap the hardware block id so that XCD x works on one
+// contiguous run of pool blocks: spatial neighbours then share an L2 instead
+// of being fetched into all eight. (A bijection for any grid size.)
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t nb) {
+    const uint32_t q = nb / kXcds, r = nb % kXcds;
+    const uint32_t x = hw % kXcds, k = hw / kXcds;
+    return x * q + min(x, r) + k;
+}
+
 // Neighbour geometry of one particle: its cell and where it sits inside it.
 struct Cursor {
     int cx, cy, cz;
@@ -389,7 +400,8 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
 // gamma = 7) + the per-particle pressure term p / (rho * rho) used by the force.
 __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
     if (P.ctl->err) return;
-    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
+    for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
+        const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
         const uint32_t i = b * kBlock + threadIdx.x;
@@ -403,23 +415,30 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
         float sum_b = 0.0f;
         uint32_t cnt = 0;
         const float4 *__restrict__ sA = P.sA;
+        // Candidates are tested without branches: a rejected candidate leaves
+        // the sums untouched through a select, so the accumulation order and
+        // bits are those of the reference loop.
+        const float4 *__restrict__ sAc = sA;
         for_rows(g, P.start, cur, [&](uint32_t j0, uint32_t j1) {
-            for (uint32_t j = j0; j < j1; ++j) {
-                if (j == i) continue;
-                const float4 aj = sA[j];
-                const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
-                const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                if (!(r2 <= h2)) continue;
-                ++cnt;
-                const float t = h2 - r2;
-                if (fluid_i) {
+            constexpr uint32_t U = 4;
+            for (uint32_t jb = j0; jb < j1; jb += U) {
+                float4 aj[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) aj[u] = sAc[min(jb + u, j1 - 1u)];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const uint32_t j = jb + u;
+                    const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
+                    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    const bool acc = (r2 <= h2) && (j != i) && (j < j1);
+                    const float t = h2 - r2;
                     const float w = ((k6 * t) * t) * t;
-                    if (aj.w > 0.0f)
-                        sum += w;
-                    else
-                        sum_b += (-aj.w) * w;
-                } else if (aj.w > 0.0f) {
-                    sum += aj.w * (((k6 * t) * t) * t);
+                    const bool fj = aj[u].w > 0.0f;
+                    const float mj = fabsf(aj[u].w);
+                    const float add_sum = fluid_i ? w : mj * w;
+                    sum = (acc && fj) ? sum + add_sum : sum;
+                    sum_b = (acc && !fj && fluid_i) ? sum_b + mj * w : sum_b;
+                    cnt += acc ? 1u : 0u;
                 }
             }
         });
@@ -436,35 +455,106 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
     }
 }
 
+// Row-window iterator over the reference enumeration order (for_rows as a
+// resumable cursor): advances `r` to the next surviving (dz, dy) row and sets
+// [j, jend); returns false after the last row.
+__device__ __forceinline__ bool next_row(const SceneGrid &g, const uint32_t *__restrict__ start, const Cursor &c,
+                                         int &r, uint32_t &j, uint32_t &jend) {
+    const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
+    while (++r < 9) {
+        const int dz = r / 3 - 1, dy = r % 3 - 1;
+        const int z = c.cz + dz, y = c.cy + dy;
+        if (z < 0 || z >= nz || y < 0 || y >= ny) continue;
+        const float ez = dz < 0 ? c.fz : (dz > 0 ? 1.0f - c.fz : 0.0f);
+        const float ey = dy < 0 ? c.fy : (dy > 0 ? 1.0f - c.fy : 0.0f);
+        const float rd = ez * ez + ey * ey;
+        if (rd > c.cut) continue;
+        const int xl = (c.cx > 0 && rd + c.fx * c.fx <= c.cut) ? c.cx - 1 : c.cx;
+        const int xr = (c.cx + 1 < nx && rd + (1.0f - c.fx) * (1.0f - c.fx) <= c.cut) ? c.cx + 1 : c.cx;
+        const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+        j = start[row + (uint32_t)xl];
+        jend = start[row + (uint32_t)xr + 1u];
+        if (j < jend) return true;
+    }
+    return false;
+}
+
+// One neighbour pair of the force sum, fluid_force.rs:165-217 operation order.
+struct ForceAcc {
+    float ax, ay, az, xx, xy, xz;
+};
+__device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const float4 vi, const float ci2,
+                                           const float4 aj, const float4 vj, const float cj2, const SceneParams &sp) {
+    const float h = sp.h, h2 = sp.h2, k6 = sp.k6, ks = sp.ks, alpha_c = sp.alpha_c;
+    const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    const float rho_i = vi.w, rho_j = vj.w;
+    const float r = sqrtf(r2);
+    const float t = h - r;
+    const float mj = fabsf(aj.w);
+    const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
+    const float denom = fmaxf(r, 1e-9f);
+    const float qp = coef / denom;
+    f.ax = f.ax + ddx * qp;
+    f.ay = f.ay + ddy * qp;
+    f.az = f.az + ddz * qp;
+    const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
+    const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
+    if (vdn > 0.0f) {
+        const float mu = (vdn * h) / (r2 + 0.01f * h2);
+        const float cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+        const float qv = cv / denom;
+        f.ax = f.ax + ddx * qv;
+        f.ay = f.ay + ddy * qv;
+        f.az = f.az + ddz * qv;
+    }
+    const float tt = h2 - r2;
+    const float w = ((k6 * tt) * tt) * tt;
+    const float fx = ((mj * 2.0f) / (rho_i + rho_j)) * w;
+    f.xx = f.xx + (vj.x - vi.x) * fx;
+    f.xy = f.xy + (vj.y - vi.y) * fx;
+    f.xz = f.xz + (vj.z - vi.z) * fx;
+}
+
 // Force (fluid_force.rs:165-217: gravity + spiky pressure + Monaghan AV, XSPH)
 // fused with the semi-implicit Euler + CFL clamp of fluid_step.rs:110-123.
 // Writes the next state into the cur arrays (this substep's grid order) and
 // accumulates the next substep's AABB. Boundary slots are carried over.
-__global__ __launch_bounds__(kBlock) void k_force(Pool P) {
+//
+// DEPTH = 0: every lane evaluates the pair terms inside its candidate loop.
+// DEPTH > 0: in-wave compaction. Each lane scans its candidates and appends
+// the accepted neighbour slots (in enumeration order) to its own LDS list of
+// DEPTH entries; the wave then evaluates the pair terms over the lists, so the
+// expensive branch runs once per accepted pair instead of once per candidate
+// iteration of the slowest lane. Summation order is unchanged.
+template <uint32_t DEPTH>
+__global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
     if (P.ctl->err) return;
     const uint32_t par = P.ctl->step & 1u;
-    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
+    __shared__ uint32_t lst[DEPTH > 0 ? DEPTH * kBlock : 1];
+    for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
+        const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
         const uint32_t i = b * kBlock + threadIdx.x;
         const bool live = i < sp.pbase + sp.n;
-        float ox = 0, oy = 0, oz = 0;
+        float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = ai;
         if (live) {
-            const float4 ai = P.sA[i];
-            const float4 vi = P.sV[i];
-            float4 vo = make_float4(vi.x, vi.y, vi.z, 0.0f);
-            ox = ai.x; oy = ai.y; oz = ai.z;
-            if (ai.w > 0.0f) {
-                const SceneGrid &g = P.grid[s];
+            ai = P.sA[i];
+            vi = P.sV[i];
+        }
+        const bool fluid = live && ai.w > 0.0f;
+        const SceneGrid &g = P.grid[s];
+        const float4 *__restrict__ sA = P.sA;
+        const float4 *__restrict__ sV = P.sV;
+        const float *__restrict__ c2 = P.c2;
+        ForceAcc f{sp.gx, sp.gy, sp.gz, 0.0f, 0.0f, 0.0f};
+        float ci2 = 0.0f;
+        if (fluid) ci2 = c2[i];
+        if constexpr (DEPTH == 0) {
+            if (fluid) {
                 const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
-                const float h = sp.h, h2 = sp.h2, k6 = sp.k6, ks = sp.ks, alpha_c = sp.alpha_c;
-                const float rho_i = vi.w;
-                const float ci2 = P.c2[i];
-                float ax = sp.gx, ay = sp.gy, az = sp.gz;
-                float xx = 0.0f, xy = 0.0f, xz = 0.0f;
-                const float4 *__restrict__ sA = P.sA;
-                const float4 *__restrict__ sV = P.sV;
-                const float *__restrict__ c2 = P.c2;
+                const float h2 = sp.h2;
                 for_rows(g, P.start, cur, [&](uint32_t j0, uint32_t j1) {
                     for (uint32_t j = j0; j < j1; ++j) {
                         if (j == i) continue;
@@ -472,45 +562,60 @@ __global__ __launch_bounds__(kBlock) void k_force(Pool P) {
                         const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
                         const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                         if (!(r2 <= h2)) continue;
-                        const float4 vj = sV[j];
-                        const float rho_j = vj.w;
-                        const float cj2 = c2[j];
-                        const float r = sqrtf(r2);
-                        const float t = h - r;
-                        const float mj = fabsf(aj.w);
-                        const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
-                        const float denom = fmaxf(r, 1e-9f);
-                        const float qp = coef / denom;
-                        ax = ax + ddx * qp;
-                        ay = ay + ddy * qp;
-                        az = az + ddz * qp;
-                        const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
-                        const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
-                        if (vdn > 0.0f) {
-                            const float mu = (vdn * h) / (r2 + 0.01f * h2);
-                            const float cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
-                            const float qv = cv / denom;
-                            ax = ax + ddx * qv;
-                            ay = ay + ddy * qv;
-                            az = az + ddz * qv;
-                        }
-                        const float tt = h2 - r2;
-                        const float w = ((k6 * tt) * tt) * tt;
-                        const float fx = ((mj * 2.0f) / (rho_i + rho_j)) * w;
-                        xx = xx + (vj.x - vi.x) * fx;
-                        xy = xy + (vj.y - vi.y) * fx;
-                        xz = xz + (vj.z - vi.z) * fx;
+                        force_pair(f, ai, vi, ci2, aj, sV[j], c2[j], sp);
                     }
                 });
+            }
+        } else {
+            Cursor cur{};
+            if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+            const float h2 = sp.h2;
+            int r = -1;
+            uint32_t j = 0, jend = 0;
+            bool done = !fluid;
+            for (;;) {
+                uint32_t cnt = 0;
+                // Scan candidates 4 at a time (loads first), appending the
+                // accepted slots in order; stop once the list could overflow.
+                while (!done && cnt + 4u <= DEPTH) {
+                    if (j >= jend) {
+                        if (!next_row(g, P.start, cur, r, j, jend)) done = true;
+                        continue;
+                    }
+                    float4 aj[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) aj[u] = sA[min(j + u, jend - 1u)];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        const uint32_t jj = j + u;
+                        const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
+                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        const bool acc = (r2 <= h2) && (jj != i) && (jj < jend);
+                        if (acc) lst[cnt * kBlock + threadIdx.x] = jj;
+                        cnt += acc ? 1u : 0u;
+                    }
+                    j = min(j + 4u, jend);
+                }
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const uint32_t jj = lst[k * kBlock + threadIdx.x];
+                    force_pair(f, ai, vi, ci2, sA[jj], sV[jj], c2[jj], sp);
+                }
+                if (!__any(!done)) break;
+            }
+        }
+        float ox = ai.x, oy = ai.y, oz = ai.z;
+        if (live) {
+            float4 vo = make_float4(vi.x, vi.y, vi.z, 0.0f);
+            if (fluid) {
                 if (P.acc) {
-                    P.acc[i] = make_float4(ax, ay, az, 0.0f);
-                    P.xsph[i] = make_float4(xx, xy, xz, 0.0f);
+                    P.acc[i] = make_float4(f.ax, f.ay, f.az, 0.0f);
+                    P.xsph[i] = make_float4(f.xx, f.xy, f.xz, 0.0f);
                 }
                 // fluid_step.rs:110-123
                 const float dt = sp.dt, eps = sp.eps;
-                float vx = (vi.x + ax * dt) + xx * eps;
-                float vy = (vi.y + ay * dt) + xy * eps;
-                float vz = (vi.z + az * dt) + xz * eps;
+                float vx = (vi.x + f.ax * dt) + f.xx * eps;
+                float vy = (vi.y + f.ay * dt) + f.xy * eps;
+                float vz = (vi.z + f.az * dt) + f.xz * eps;
                 const float s2 = vx * vx + vy * vy + vz * vz;
                 if (s2 > sp.vmax2) {
                     const float k = sp.vmax / sqrtf(s2);
@@ -529,6 +634,11 @@ __global__ __launch_bounds__(kBlock) void k_force(Pool P) {
     }
 }
 
+template __global__ void k_force_t<0>(Pool P);
+template __global__ void k_force_t<16>(Pool P);
+template __global__ void k_force_t<32>(Pool P);
+template __global__ void k_force_t<64>(Pool P);
+
 // Reaction on every boundary particle of a scene with reactions:
 // bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending
 // index, X = d * ((m_i * (coef + cv)) / denom) with d = p_i - p_j (the
@@ -537,7 +647,8 @@ __global__ __launch_bounds__(kBlock) void k_force(Pool P) {
 // and subtracted in index order, so the sum is the reference's bit for bit.
 __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
     if (P.ctl->err) return;
-    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
+    for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
+        const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
         if (!sp.react) continue;

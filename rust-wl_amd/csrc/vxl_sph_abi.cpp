@@ -197,6 +197,41 @@ VxlSphStatus vxl_sph_engine_set_graphs(VxlSphEngine *e, int32_t on) {
     return guard([&] { e->eng.set_graphs(on != 0); });
 }
 
+VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t) {
+    if (!e || !t) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(t->hdr, sizeof(VxlSphTuning));
+    if (st != VXL_SPH_OK) return st;
+    const auto ok_depth = [](uint32_t d, bool allow64) { return d == 0 || d == 16 || d == 32 || (allow64 && d == 64); };
+    if (!ok_depth(t->force_depth, true) || !ok_depth(t->density_depth, false))
+        return fail(VXL_SPH_INVALID_ARGUMENT, "unsupported neighbour-list depth");
+    return guard([&] {
+        Engine::Tuning tu;
+        tu.persist_blocks = t->persist_blocks;
+        tu.force_depth = t->force_depth;
+        tu.density_depth = t->density_depth;
+        e->eng.set_tuning(tu);
+    });
+}
+
+VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out) {
+    if (!e || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    const Engine::Tuning &tu = e->eng.tuning();
+    std::memset(out, 0, sizeof(*out));
+    out->hdr.size = sizeof(VxlSphTuning);
+    out->hdr.version = VXL_SPH_ABI_VERSION;
+    out->persist_blocks = tu.persist_blocks;
+    out->force_depth = tu.force_depth;
+    out->density_depth = tu.density_depth;
+    return VXL_SPH_OK;
+}
+
+VxlSphStatus vxl_sph_engine_autotune(VxlSphEngine *e, uint32_t substeps, VxlSphTuning *out) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    VxlSphStatus st = guard([&] { e->eng.autotune(substeps); });
+    if (st == VXL_SPH_OK && out) return vxl_sph_engine_get_tuning(e, out);
+    return st;
+}
+
 VxlSphStatus vxl_sph_engine_step(VxlSphEngine *e, uint32_t ticks) {
     if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
     return guard([&] { e->eng.step(ticks); });

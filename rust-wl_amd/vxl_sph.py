@@ -62,6 +62,11 @@ class FluidConfig(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("hdr", Header), ("persist_blocks", C.c_uint32), ("force_depth", C.c_uint32),
+                ("density_depth", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("ms_total", C.c_float)]
 
@@ -97,7 +102,8 @@ EXPORTS = (
     "vxl_sph_device_count", "vxl_sph_last_error", "vxl_sph_default_config",
     "vxl_sph_engine_create", "vxl_sph_engine_destroy", "vxl_sph_engine_set_diagnostics",
     "vxl_sph_engine_set_graphs", "vxl_sph_engine_step", "vxl_sph_engine_sync", "vxl_sph_engine_device_bytes",
-    "vxl_sph_engine_profile", "vxl_sph_engine_step_timed",
+    "vxl_sph_engine_profile", "vxl_sph_engine_step_timed", "vxl_sph_engine_set_tuning",
+    "vxl_sph_engine_get_tuning", "vxl_sph_engine_autotune",
     "vxl_sph_scene_add", "vxl_sph_scene_remove", "vxl_sph_scene_upload_boundary", "vxl_sph_scene_read_state",
     "vxl_sph_scene_read_fields", "vxl_sph_scene_read_grid", "vxl_sph_scene_read_bounds",
     "vxl_sph_scene_state_hash",
@@ -141,6 +147,12 @@ def lib():
     L.vxl_sph_engine_step.argtypes = [P, C.c_uint32]
     L.vxl_sph_engine_device_bytes.restype = st
     L.vxl_sph_engine_device_bytes.argtypes = [P, C.POINTER(C.c_uint64)]
+    L.vxl_sph_engine_set_tuning.restype = st
+    L.vxl_sph_engine_set_tuning.argtypes = [P, C.POINTER(Tuning)]
+    L.vxl_sph_engine_get_tuning.restype = st
+    L.vxl_sph_engine_get_tuning.argtypes = [P, C.POINTER(Tuning)]
+    L.vxl_sph_engine_autotune.restype = st
+    L.vxl_sph_engine_autotune.argtypes = [P, C.c_uint32, C.POINTER(Tuning)]
     L.vxl_sph_engine_profile.restype = st
     L.vxl_sph_engine_profile.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(KernelTime), up]
     L.vxl_sph_engine_step_timed.restype = st
@@ -378,6 +390,27 @@ class Engine:
         v = C.c_uint64()
         _check(lib().vxl_sph_engine_device_bytes(self.h, C.byref(v)))
         return v.value
+
+    def set_tuning(self, persist_blocks=0, force_depth=32, density_depth=0):
+        t = Tuning()
+        t.hdr.size = C.sizeof(Tuning)
+        t.hdr.version = ABI_VERSION
+        t.persist_blocks = persist_blocks
+        t.force_depth = force_depth
+        t.density_depth = density_depth
+        _check(lib().vxl_sph_engine_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, substeps=2):
+        """Pick the fastest launch variant on the current pool (advances it by
+        substeps x variants substeps; results are identical for every variant)."""
+        t = Tuning()
+        _check(lib().vxl_sph_engine_autotune(self.h, substeps, C.byref(t)))
+        return dict(persist_blocks=t.persist_blocks, force_depth=t.force_depth, density_depth=t.density_depth)
+
+    def get_tuning(self):
+        t = Tuning()
+        _check(lib().vxl_sph_engine_get_tuning(self.h, C.byref(t)))
+        return dict(persist_blocks=t.persist_blocks, force_depth=t.force_depth, density_depth=t.density_depth)
 
     def profile(self, substeps):
         """{kernel name: total ms} over `substeps` eager substeps (HIP events)."""

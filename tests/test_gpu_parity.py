@@ -258,3 +258,21 @@ def test_reference_akinci_canary_on_gpu():
     got = st.reactions()
     gft = np.array([np.concatenate([g[1], g[2]]) for g in got], np.float32)
     np.testing.assert_array_equal(bits(gft), bits(ft))
+
+
+@pytest.mark.parametrize("depth", [0, 16, 32, 64])
+def test_force_kernel_variants_bit_exact(depth):
+    """Every autotuner choice of the force kernel (direct or LDS-compacted
+    neighbour lists of depth 16/32/64) gives the oracle's state bit for bit."""
+    cfg, data, tank, d = small_scene(seed=13, block=(16, 12, 10), tank=(0.6, 0.5, 0.4))
+    orc = oracle_dam_break(cfg, data, tank, d)
+    eng = V.Engine(diagnostics=True)
+    sid = eng.add(cfg, data, reactions=False)
+    eng.set_tuning(force_depth=depth)
+    for _ in range(6):
+        eng.step(1)
+        orc.step(cfg.dt_tick)
+    f = eng.read_fields(sid, diagnostics=True)
+    np.testing.assert_array_equal(bits(f["acc"]), bits(orc.acc()))
+    np.testing.assert_array_equal(bits(f["xsph"]), bits(orc.xsph()))
+    assert eng.state_hash(sid) == orc.state_hash()
