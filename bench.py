@@ -72,6 +72,69 @@ def cpu_baseline(cfg, data, steps, threads):
     return nf * steps / dt, dt
 
 
+def bench_batched(V, args):
+    """64 uneven dam-break scenes in one pool (one launch per stage, one graph
+    per step) against the same scenes stepped one at a time."""
+    reqs = V.sample_batch(4, args.batch_scenes)
+    built = [V.request_scene(r) for r in reqs]
+    nf_total = sum(d.n_fluid for _, d in built)
+    eng = V.Engine()
+    ids = [eng.add(c, d, reactions=False) for c, d in built]
+    tuned = eng.autotune(2)
+    eng.step(3)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.batch_steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    lat = eng.step_timed(args.batch_steps)
+    mix_hash = eng.state_hash(ids[0])
+    eng.close()
+    batched_ups = nf_total * args.batch_steps / wall
+    # One scene at a time through the same engine code (plain single-scene path).
+    seq_ms = 0.0
+    one = V.Engine()
+    for c, d in built:
+        sid = one.add(c, d, reactions=False)
+        one.step(2)
+        one.sync()
+        t0 = time.perf_counter()
+        one.step(args.batch_steps // 3)
+        one.sync()
+        seq_ms += 1e3 * (time.perf_counter() - t0) / (args.batch_steps // 3)
+        one.remove(sid)
+    one.close()
+    seq_ups = nf_total / (seq_ms * 1e-3)
+    print(f"# batched: {len(built)} scenes, {nf_total} fluid, {1e3 * wall / args.batch_steps:.3f} ms/step "
+          f"vs one-at-a-time {seq_ms:.3f} ms -> {batched_ups / seq_ups:.2f}x", file=sys.stderr, flush=True)
+    return {"scenes": len(built), "fluid_particles": nf_total,
+            "total_particles": sum(d.n for _, d in built),
+            "value": batched_ups, "unit": "particle-updates/s",
+            "ms_per_step": 1e3 * wall / args.batch_steps,
+            "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "one_at_a_time": {"value": seq_ups, "ms_per_step_all_scenes": seq_ms},
+            "speedup_vs_one_at_a_time": batched_ups / seq_ups, "tuning": tuned,
+            "workload": "seeded set: fluid counts log-uniform [8192, 524288], tank x,z ~ U[1,4] m, y = 1 m, "
+                        "fill ~ U[0.1,0.5], c0 ~ U[15,30], alpha ~ U[0.01,0.05], d = 0.02 m, dt = 5e-4 s"}
+
+
+def bench_served(V, args):
+    """Continuous batching over a Poisson request stream (scaled-down form of
+    the 2000-request stream so the default run stays short)."""
+    reqs = V.sample_requests(9, args.serve_requests, rate=20.0, nmin=4096, nmax=524288, smin=50, smax=200)
+    stats, results = V.serve(reqs, budget_bytes=200 << 30, time_scale=1.0, builders=8)
+    print(f"# served: {stats['completed']}/{stats['requests']} requests, {stats['updates_per_s']:.3e} upd/s, "
+          f"step p50 {stats['step_ms_p50']:.3f} ms p99 {stats['step_ms_p99']:.3f} ms, "
+          f"peak resident {stats['peak_resident']}", file=sys.stderr, flush=True)
+    keep = ("requests", "completed", "pool_steps", "fluid_updates", "wall_s", "updates_per_s", "step_ms_p50",
+            "step_ms_p99", "loop_ms_p50", "loop_ms_p99", "latency_s_p50", "latency_s_p99", "wait_s_p50",
+            "wait_s_p99", "peak_resident", "peak_bytes", "layouts")
+    out = {k: stats[k] for k in keep}
+    out["workload"] = (f"{args.serve_requests} Poisson requests at 20/s, fluid log-uniform [4096, 524288], "
+                       "steps U[50, 200], no rigid boxes, HBM budget 200 GB, real-time arrivals")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +148,11 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--force-depth", type=int, default=None, help="override the force kernel's LDS list depth")
     ap.add_argument("--sweep", action="store_true", help="print per-kernel times for every tuning variant")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device copy of BYTES (scale for the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -146,6 +214,7 @@ def main():
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
     prof = eng.profile(args.profile_steps)
+    tuning = eng.get_tuning()
     g = eng.read_grid(sid)
     C = g["total"] + 1
     per_kernel = {k: v / args.profile_steps for k, v in prof.items()}
@@ -160,6 +229,10 @@ def main():
     step_gbs = b_step / (ms_per_step * 1e-3) / 1e9
     print(f"# kernels (ms/step): " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(kern.items(), key=lambda x: -x[1])),
           file=sys.stderr, flush=True)
+
+    eng.close()
+    batched = None if args.no_batch else bench_batched(V, args)
+    served = None if args.no_serve else bench_served(V, args)
 
     cpu = None
     if not args.no_cpu:
@@ -195,7 +268,9 @@ def main():
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
         "kernel_algorithmic_GBs": stage_gbs,
-        "tuning": eng.get_tuning(),
+        "tuning": tuning,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

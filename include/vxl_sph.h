@@ -225,6 +225,70 @@ VxlSphStatus vxl_sph_dam_break(const VxlSphDamBreak *spec, const VxlSphFluidConf
                                uint32_t *n_fluid, uint32_t *nspans, float *pos, float *vel, float *pmass,
                                VxlSphSpan *spans);
 
+/* ---- batched and served forms ---- */
+/* A sampled dam-break request: scene geometry and physics, requested steps,
+ * arrival time (served form) and dynamic boxes (served form; not supported by
+ * vxl_sph_serve yet: such requests are rejected with VXL_SPH_UNSUPPORTED). */
+typedef struct VxlSphRequest {
+    VxlSphDamBreak dam;
+    float sound_speed;
+    float artificial_viscosity;
+    uint32_t steps;
+    uint32_t nboxes;
+    double arrival_s;
+    double fill;              /* realised fluid volume / tank volume */
+    VxlSphBoxBody boxes[4];
+    float box_density[4];
+} VxlSphRequest;
+
+/* Batched form: `count` scenes of a seeded set (fluid count log-uniform in
+ * [nmin, nmax]; tank x, z ~ U[1, 4] m, y = 1 m; fill ~ U[0.1, 0.5] sets the block
+ * height; c0 ~ U[15, 30] m/s; alpha ~ U[0.01, 0.05]; d = 0.02 m). */
+VxlSphStatus vxl_sph_sample_batch(uint64_t seed, uint32_t count, uint32_t nmin, uint32_t nmax, VxlSphRequest *out);
+/* Served form: a seeded Poisson stream of `rate` requests/s, fluid count
+ * log-uniform in [nmin, nmax], steps ~ U[smin, smax], 0..max_boxes boxes. */
+VxlSphStatus vxl_sph_sample_requests(uint64_t seed, uint32_t count, double rate, uint32_t nmin, uint32_t nmax,
+                                     uint32_t smin, uint32_t smax, uint32_t max_boxes, VxlSphRequest *out);
+/* The fluid configuration a request's scene is stepped with
+ * (h = 2d, mass = rho0 d^3, gamma 7, eps 0.5, one substep of dt). */
+VxlSphStatus vxl_sph_request_config(const VxlSphRequest *req, float dt, uint64_t max_bins, VxlSphFluidConfig *out);
+
+typedef struct VxlSphServeConfig {
+    VxlSphHeader hdr;
+    uint64_t budget_bytes;   /* device bytes the resident scenes may use */
+    double time_scale;       /* arrival schedule speed-up (1 = real time) */
+    double max_wall_s;
+    uint32_t max_resident;   /* 0 = budget only */
+    uint32_t builders;       /* host threads building arrived scenes */
+    uint64_t max_bins;
+    float dt;
+    int32_t device;
+} VxlSphServeConfig;
+
+typedef struct VxlSphServeStats {
+    uint32_t requests, completed;
+    uint64_t pool_steps, fluid_updates;
+    double wall_s, updates_per_s;
+    double step_ms_p50, step_ms_p99, step_ms_mean;
+    double loop_ms_p50, loop_ms_p99;
+    double latency_s_p50, latency_s_p99;
+    double wait_s_p50, wait_s_p99;
+    uint32_t peak_resident, reserved;
+    uint64_t peak_bytes, layouts;
+} VxlSphServeStats;
+
+typedef struct VxlSphServeResult {
+    uint32_t index, n, n_fluid, steps;
+    double arrival_s, admit_s, finish_s;
+    uint64_t state_hash;      /* same FNV-1a as vxl_sph_scene_state_hash */
+} VxlSphServeResult;
+
+/* Continuous batching: requests are built after they arrive, admitted in
+ * arrival order as the HBM budget allows, co-stepped in one launch per stage
+ * with every other resident request, and retired at step boundaries. */
+VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSphRequest *reqs,
+                           VxlSphServeStats *stats, VxlSphServeResult *results);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "sph_kernels.h"
@@ -31,6 +32,7 @@ struct SceneSlot {
     std::vector<Span> spans;
     // host staging for a scene that is not on the device yet
     bool resident = false;
+    uint32_t dev_base = 0;  // slot the scene's state starts at on the device
     std::vector<float> pos, vel, pmass;  // AoS 3n / 3n / n
     uint32_t cells_hint = 0;             // grid cells of the initial box
     // reactions of the last tick (tick average), per span: F[3], tau[3]
@@ -95,6 +97,12 @@ public:
     Tuning autotune(uint32_t substeps);
     const Tuning &tuning() const { return tuning_; }
     uint64_t device_bytes() const { return bytes_; }
+    uint64_t layouts() const { return layouts_; }
+    uint32_t capacity_slots() const { return slots_; }
+    uint32_t used_slots() const { return used_slots_; }
+    // Device bytes a scene of n particles (cells_hint grid cells) adds to the
+    // pool: per-slot state and sort arrays plus its share of the cell tables.
+    static uint64_t footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins);
 
 private:
     void layout();            // assign slot ranges, (re)allocate, move + upload
@@ -137,10 +145,17 @@ private:
     float *d_react_ = nullptr;
     uint32_t nspans_ = 0;
     std::vector<uint32_t> span_first_;  // first global span of each scene (by scene position)
-    // index-order staging for readbacks
-    float *d_out_f_ = nullptr;  // 6 * slots
-    uint32_t *d_out_u_ = nullptr;
-    float *d_out_fields_ = nullptr;  // dens, press, acc(3), xsph(3) = 8 * slots
+    // spare state arrays: resident scenes move into them at a layout change
+    float4 *spare_cA_ = nullptr, *spare_cV_ = nullptr;
+    uint32_t *spare_cg_ = nullptr;
+    bool want_reaction_buffers_ = false;
+    bool diag_alloc_ = false;
+    // index-order staging for readbacks (grown on demand)
+    void *staging_ = nullptr;
+    size_t staging_bytes_ = 0;
+    void *staging(size_t bytes);
+    uint64_t layouts_ = 0;
+    std::unordered_map<void *, size_t> alloc_size_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
     uint64_t substeps_done_ = 0;

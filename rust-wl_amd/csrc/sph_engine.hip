@@ -28,6 +28,13 @@ uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
 
 }  // namespace
 
+uint64_t Engine::footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins) {
+    // cur + spare (36 B each), grid-ordered copy 36, c2/press 8, sort 16 B per
+    // slot; counts + start 8 B per reserved cell.
+    const uint64_t slot_bytes = 36 + 36 + 36 + 8 + 16;
+    return slot_bytes * round_up(n, kBlock) + 8ull * cell_reserve(cells_hint, max_bins);
+}
+
 Engine::Engine(int device) : device_(device) {
     hip_check(hipSetDevice(device_), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -56,11 +63,18 @@ T *Engine::dalloc(size_t n) {
     hipError_t e = hipMalloc(&p, n * sizeof(T));
     if (e != hipSuccess) throw DeviceError(e, std::string("hipMalloc: ") + hipGetErrorString(e));
     bytes_ += n * sizeof(T);
+    alloc_size_[p] = n * sizeof(T);
     return (T *)p;
 }
 
 void Engine::dfree(void *p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    auto it = alloc_size_.find(p);
+    if (it != alloc_size_.end()) {
+        bytes_ -= it->second;
+        alloc_size_.erase(it);
+    }
+    (void)hipFree(p);
 }
 
 void Engine::free_all() {
@@ -81,13 +95,14 @@ void Engine::free_all() {
     d_box_ = nullptr;
     d_block_scene_ = nullptr;
     d_tile_sums_ = nullptr;
-    d_out_f_ = nullptr;
-    d_out_u_ = nullptr;
-    d_out_fields_ = nullptr;
+    dfree(staging_);
+    staging_ = nullptr;
+    staging_bytes_ = 0;
+    spare_cA_ = spare_cV_ = nullptr;
+    spare_cg_ = nullptr;
     std::memset(&P_, 0, sizeof(P_));
     slots_ = 0;
     ccap_ = 0;
-    bytes_ = sizeof(Control);
 }
 
 void Engine::invalidate_graph() {
@@ -177,12 +192,21 @@ void Engine::alloc_pool(uint32_t slots) {
         pool_allocs_.push_back(p);
     };
     F4(P_.cA); F4(P_.cV); U(P_.cg);
+    F4(spare_cA_); F4(spare_cV_); U(spare_cg_);
     F4(P_.sA); F4(P_.sV); U(P_.sg);
     F(P_.c2); F(P_.press);
     U(P_.key); U(P_.slot); U(P_.items_m); U(P_.items_g);
     if (diagnostics_) {
         U(P_.ncount);
         F4(P_.acc); F4(P_.xsph);
+    } else {
+        P_.ncount = nullptr;
+        P_.acc = P_.xsph = nullptr;
+    }
+    if (want_reaction_buffers_) {
+        F4(P_.bf); F4(P_.bp);
+    } else {
+        P_.bf = P_.bp = nullptr;
     }
 }
 
@@ -198,36 +222,33 @@ void Engine::alloc_cells(uint32_t ccap) {
     P_.ccap = ccap;
 }
 
-// Assign slot ranges, (re)allocate, move resident scenes, upload new ones.
+void *Engine::staging(size_t bytes) {
+    if (bytes > staging_bytes_) {
+        dfree(staging_);
+        staging_ = nullptr;
+        staging_bytes_ = 0;
+        staging_ = dalloc<char>(bytes);
+        staging_bytes_ = bytes;
+    }
+    return staging_;
+}
+
+// Assign slot ranges; grow the pool geometrically when it is too small; move
+// resident scenes device-to-device (into the spare state arrays, which then
+// become current); upload new scenes; rebuild the small scene tables.
 void Engine::layout() {
     hip_check(hipSetDevice(device_), "hipSetDevice");
     hip_check(hipStreamSynchronize(stream_), "sync before layout");
     invalidate_graph();
-    // Save resident scenes' device state (cur arrays, this grid order).
-    struct Saved {
-        uint32_t idx, n;
-        std::vector<float4> a, v;
-        std::vector<uint32_t> g;
-    };
-    std::vector<Saved> saved;
-    for (uint32_t k = 0; k < scenes_.size(); ++k) {
-        SceneSlot &sc = scenes_[k];
-        if (!sc.resident) continue;
-        Saved sv;
-        sv.idx = k;
-        sv.n = sc.params.n;
-        sv.a.resize(sv.n);
-        sv.v.resize(sv.n);
-        hip_check(hipMemcpy(sv.a.data(), P_.cA + sc.params.pbase, 16ull * sv.n, hipMemcpyDeviceToHost), "save A");
-        hip_check(hipMemcpy(sv.v.data(), P_.cV + sc.params.pbase, 16ull * sv.n, hipMemcpyDeviceToHost), "save V");
-        sv.g.resize(sv.n);
-        hip_check(hipMemcpy(sv.g.data(), P_.cg + sc.params.pbase, 4ull * sv.n, hipMemcpyDeviceToHost), "save gid");
-        saved.push_back(std::move(sv));
-    }
-    // Slot and cell reservations.
+    const std::vector<uint32_t> old_base = [&] {
+        std::vector<uint32_t> v;
+        for (auto &sc : scenes_) v.push_back(sc.dev_base);
+        return v;
+    }();
     uint32_t used = 0;
     uint64_t cells = 0;
     std::vector<uint32_t> block_scene;
+    bool reactions = false;
     for (uint32_t k = 0; k < scenes_.size(); ++k) {
         SceneParams &p = scenes_[k].params;
         p.pbase = used;
@@ -235,67 +256,55 @@ void Engine::layout() {
         used += p.alloc;
         for (uint32_t b = 0; b < p.alloc / kBlock; ++b) block_scene.push_back(k);
         cells += cell_reserve(scenes_[k].cells_hint, p.max_bins);
+        reactions = reactions || scenes_[k].want_reactions;
     }
     used_slots_ = used;
-    const uint32_t want_cells = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(cells, ccap_), 0xfffffff0ull);
-    free_all();
-    alloc_pool(std::max<uint32_t>(used, kBlock));
-    alloc_cells(std::max<uint32_t>(want_cells, 4096));
-    d_params_ = dalloc<SceneParams>(std::max<size_t>(scenes_.size(), 1));
-    d_grid_ = dalloc<SceneGrid>(std::max<size_t>(scenes_.size(), 1));
-    d_box_ = dalloc<SceneBox>(2 * std::max<size_t>(scenes_.size(), 1));
-    d_block_scene_ = dalloc<uint32_t>(std::max<size_t>(block_scene.size(), 1));
-    d_out_f_ = dalloc<float>(6ull * slots_);
-    pool_allocs_.push_back(d_out_f_);
-    d_out_fields_ = dalloc<float>(8ull * slots_);
-    pool_allocs_.push_back(d_out_fields_);
-    d_out_u_ = dalloc<uint32_t>(slots_);
-    pool_allocs_.push_back(d_out_u_);
-    // Reaction spans (scenes with reactions only).
-    std::vector<SpanDev> spans;
-    span_first_.assign(scenes_.size(), 0);
+    const uint32_t need = std::max<uint32_t>(used, kBlock);
+    const bool grow = need > slots_ || (reactions && !want_reaction_buffers_) || diag_alloc_ != diagnostics_;
+    float4 *srcA = P_.cA, *srcV = P_.cV;
+    uint32_t *srcG = P_.cg;
+    std::vector<void *> old_allocs;
+    if (grow) {
+        // New pool; the old one stays alive until the resident state is copied.
+        old_allocs.swap(pool_allocs_);
+        const uint32_t ccap_keep = ccap_;
+        const uint32_t cap = std::max<uint32_t>(need, slots_ + slots_ / 2);
+        want_reaction_buffers_ = want_reaction_buffers_ || reactions;
+        diag_alloc_ = diagnostics_;
+        alloc_pool(round_up(cap, kBlock));
+        ccap_ = 0;
+        dfree(d_tile_sums_);
+        d_tile_sums_ = nullptr;
+        alloc_cells((uint32_t)std::min<uint64_t>(std::max<uint64_t>({cells, (uint64_t)ccap_keep, 4096ull}),
+                                                 0xfffffff0ull));
+    } else if (cells > ccap_) {
+        pool_allocs_.erase(std::remove_if(pool_allocs_.begin(), pool_allocs_.end(),
+                                          [&](void *p) { return p == P_.counts || p == P_.start; }),
+                           pool_allocs_.end());
+        dfree(P_.counts);
+        dfree(P_.start);
+        dfree(d_tile_sums_);
+        alloc_cells((uint32_t)std::min<uint64_t>(cells, 0xfffffff0ull));
+    }
+    // Move resident scenes: old position (src arrays) -> new position (dst).
+    float4 *dstA = grow ? P_.cA : spare_cA_, *dstV = grow ? P_.cV : spare_cV_;
+    uint32_t *dstG = grow ? P_.cg : spare_cg_;
     for (uint32_t k = 0; k < scenes_.size(); ++k) {
-        span_first_[k] = (uint32_t)spans.size();
-        if (!scenes_[k].want_reactions) continue;
-        for (const Span &s : scenes_[k].spans) {
-            SpanDev d{};
-            d.scene = k;
-            d.start = scenes_[k].params.pbase + s.start;
-            d.end = scenes_[k].params.pbase + s.end;
-            d.substeps = scenes_[k].substeps;
-            for (int a = 0; a < 3; ++a) d.origin[a] = s.origin[a];
-            spans.push_back(d);
-        }
+        SceneSlot &sc = scenes_[k];
+        if (!sc.resident) continue;
+        const size_t n = sc.params.n, from = old_base[k], to = sc.params.pbase;
+        hip_check(hipMemcpyAsync(dstA + to, srcA + from, 16 * n, hipMemcpyDeviceToDevice, stream_), "move A");
+        hip_check(hipMemcpyAsync(dstV + to, srcV + from, 16 * n, hipMemcpyDeviceToDevice, stream_), "move V");
+        hip_check(hipMemcpyAsync(dstG + to, srcG + from, 4 * n, hipMemcpyDeviceToDevice, stream_), "move gid");
     }
-    nspans_ = (uint32_t)spans.size();
-    if (nspans_) {
-        P_.bf = dalloc<float4>(slots_);
-        pool_allocs_.push_back(P_.bf);
-        P_.bp = dalloc<float4>(slots_);
-        pool_allocs_.push_back(P_.bp);
-        d_spans_ = dalloc<SpanDev>(nspans_);
-        hip_check(hipMemcpy(d_spans_, spans.data(), sizeof(SpanDev) * nspans_, hipMemcpyHostToDevice), "spans");
-        d_react_ = dalloc<float>(6ull * nspans_);
-        hip_check(hipMemset(d_react_, 0, 24ull * nspans_), "react");
+    hip_check(hipStreamSynchronize(stream_), "move sync");
+    if (!grow) {
+        std::swap(P_.cA, spare_cA_);
+        std::swap(P_.cV, spare_cV_);
+        std::swap(P_.cg, spare_cg_);
     }
-    P_.nspans = nspans_;
-    P_.spans = d_spans_;
-    P_.react = d_react_;
-    std::vector<SceneParams> hp;
-    for (auto &sc : scenes_) hp.push_back(sc.params);
-    if (!hp.empty())
-        hip_check(hipMemcpy(d_params_, hp.data(), sizeof(SceneParams) * hp.size(), hipMemcpyHostToDevice), "params");
-    if (!block_scene.empty())
-        hip_check(hipMemcpy(d_block_scene_, block_scene.data(), 4 * block_scene.size(), hipMemcpyHostToDevice),
-                  "block_scene");
-    // Upload state: resident scenes from the saved copies, new scenes from
-    // their staging (index order; the first grid build sorts them).
-    for (auto &sv : saved) {
-        SceneSlot &sc = scenes_[sv.idx];
-        hip_check(hipMemcpy(P_.cA + sc.params.pbase, sv.a.data(), 16ull * sv.n, hipMemcpyHostToDevice), "restore A");
-        hip_check(hipMemcpy(P_.cV + sc.params.pbase, sv.v.data(), 16ull * sv.n, hipMemcpyHostToDevice), "restore V");
-        hip_check(hipMemcpy(P_.cg + sc.params.pbase, sv.g.data(), 4ull * sv.n, hipMemcpyHostToDevice), "restore gid");
-    }
+    for (void *p : old_allocs) dfree(p);
+    // New scenes: index order; the first grid build sorts them.
     for (auto &sc : scenes_) {
         if (sc.resident) continue;
         const uint32_t n = sc.params.n, nf = sc.params.nf;
@@ -315,6 +324,52 @@ void Engine::layout() {
         sc.pos.clear(); sc.pos.shrink_to_fit();
         sc.vel.clear(); sc.vel.shrink_to_fit();
     }
+    for (auto &sc : scenes_) sc.dev_base = sc.params.pbase;
+    // Scene tables.
+    dfree(d_params_);
+    dfree(d_grid_);
+    dfree(d_box_);
+    dfree(d_block_scene_);
+    dfree(d_spans_);
+    dfree(d_react_);
+    d_spans_ = nullptr;
+    d_react_ = nullptr;
+    d_params_ = dalloc<SceneParams>(std::max<size_t>(scenes_.size(), 1));
+    d_grid_ = dalloc<SceneGrid>(std::max<size_t>(scenes_.size(), 1));
+    d_box_ = dalloc<SceneBox>(2 * std::max<size_t>(scenes_.size(), 1));
+    d_block_scene_ = dalloc<uint32_t>(std::max<size_t>(block_scene.size(), 1));
+    std::vector<SpanDev> spans;
+    span_first_.assign(scenes_.size(), 0);
+    for (uint32_t k = 0; k < scenes_.size(); ++k) {
+        span_first_[k] = (uint32_t)spans.size();
+        if (!scenes_[k].want_reactions) continue;
+        for (const Span &s : scenes_[k].spans) {
+            SpanDev d{};
+            d.scene = k;
+            d.start = scenes_[k].params.pbase + s.start;
+            d.end = scenes_[k].params.pbase + s.end;
+            d.substeps = scenes_[k].substeps;
+            for (int a = 0; a < 3; ++a) d.origin[a] = s.origin[a];
+            spans.push_back(d);
+        }
+    }
+    nspans_ = (uint32_t)spans.size();
+    if (nspans_) {
+        d_spans_ = dalloc<SpanDev>(nspans_);
+        hip_check(hipMemcpy(d_spans_, spans.data(), sizeof(SpanDev) * nspans_, hipMemcpyHostToDevice), "spans");
+        d_react_ = dalloc<float>(6ull * nspans_);
+        hip_check(hipMemset(d_react_, 0, 24ull * nspans_), "react");
+    }
+    P_.nspans = nspans_;
+    P_.spans = d_spans_;
+    P_.react = d_react_;
+    std::vector<SceneParams> hp;
+    for (auto &sc : scenes_) hp.push_back(sc.params);
+    if (!hp.empty())
+        hip_check(hipMemcpy(d_params_, hp.data(), sizeof(SceneParams) * hp.size(), hipMemcpyHostToDevice), "params");
+    if (!block_scene.empty())
+        hip_check(hipMemcpy(d_block_scene_, block_scene.data(), 4 * block_scene.size(), hipMemcpyHostToDevice),
+                  "block_scene");
     P_.params = d_params_;
     P_.grid = d_grid_;
     P_.box = d_box_;
@@ -322,7 +377,7 @@ void Engine::layout() {
     P_.block_scene = d_block_scene_;
     P_.nblocks = used / kBlock;
     P_.nscenes = (uint32_t)scenes_.size();
-    // Control block + AABB of the uploaded state into box[step & 1].
+    // Control block + AABB of the current state into box[step & 1].
     std::memset(h_ctl_, 0, sizeof(Control));
     hip_check(hipMemcpy(d_ctl_, h_ctl_, sizeof(Control), hipMemcpyHostToDevice), "ctl");
     ctl_step_base_ = 0;
@@ -339,6 +394,7 @@ void Engine::layout() {
         hip_check(hipGetLastError(), "k_box");
     }
     hip_check(hipStreamSynchronize(stream_), "layout sync");
+    ++layouts_;
     dirty_ = false;
 }
 
@@ -608,7 +664,7 @@ void Engine::read_state(uint32_t id, float *pos, float *vel) {
     sync();
     const SceneSlot &sc = scene(id);
     const uint32_t n = sc.params.n, pb = sc.params.pbase;
-    float *op = d_out_f_, *ov = d_out_f_ + 3ull * slots_;
+    float *op = (float *)staging(24ull * slots_), *ov = op + 3ull * slots_;
     hipLaunchKernelGGL(k_unsort, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, op, ov);
     hip_check(hipGetLastError(), "k_unsort");
     if (pos) hip_check(hipMemcpyAsync(pos, op + 3ull * pb, 12ull * n, hipMemcpyDeviceToHost, stream_), "pos");
@@ -622,14 +678,15 @@ void Engine::read_fields(uint32_t id, float *dens, float *press, uint32_t *ncoun
         throw std::logic_error("per-particle diagnostics are off (enable before stepping)");
     const SceneSlot &sc = scene(id);
     const uint32_t n = sc.params.n, nf = sc.params.nf, pb = sc.params.pbase;
-    float *f = d_out_fields_;
+    float *f = (float *)staging(36ull * slots_);
     float *od = f, *op = f + slots_, *oa = f + 2ull * slots_, *ox = f + 5ull * slots_;  // acc/xsph AoS
+    uint32_t *ou = (uint32_t *)(f + 8ull * slots_);
     hipLaunchKernelGGL(k_unsort_fields, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, od, op,
-                       diagnostics_ ? d_out_u_ : nullptr, diagnostics_ ? oa : nullptr, diagnostics_ ? ox : nullptr);
+                       diagnostics_ ? ou : nullptr, diagnostics_ ? oa : nullptr, diagnostics_ ? ox : nullptr);
     hip_check(hipGetLastError(), "k_unsort_fields");
     if (dens) hip_check(hipMemcpyAsync(dens, od + pb, 4ull * n, hipMemcpyDeviceToHost, stream_), "dens");
     if (press) hip_check(hipMemcpyAsync(press, op + pb, 4ull * n, hipMemcpyDeviceToHost, stream_), "press");
-    if (ncount) hip_check(hipMemcpyAsync(ncount, d_out_u_ + pb, 4ull * n, hipMemcpyDeviceToHost, stream_), "ncount");
+    if (ncount) hip_check(hipMemcpyAsync(ncount, ou + pb, 4ull * n, hipMemcpyDeviceToHost, stream_), "ncount");
     if (acc) hip_check(hipMemcpyAsync(acc, oa + 3ull * pb, 12ull * nf, hipMemcpyDeviceToHost, stream_), "acc");
     if (xsph) hip_check(hipMemcpyAsync(xsph, ox + 3ull * pb, 12ull * nf, hipMemcpyDeviceToHost, stream_), "xsph");
     hip_check(hipStreamSynchronize(stream_), "sync");

@@ -314,4 +314,75 @@ void dam_break(const DamBreakSpec &spec, const SceneConfig &cfg, std::vector<flo
     if (spec.walls) append_boundary_boxes(cfg, tank_bodies(spec.tank, d), pos, vel, pmass, spans);
 }
 
+
+namespace {
+
+// Block dimensions for ~n particles of spacing d with height `height` in a
+// tank footprint lx x lz (aspect kept, clamped to the footprint).
+void block_dims(double n, double lx, double lz, double height, double d, uint32_t out[3]) {
+    const uint32_t ny = (uint32_t)std::max(1.0, std::round(height / d));
+    const double nxz = std::max(1.0, n / ny);
+    const uint32_t cx = (uint32_t)std::max(1.0, std::floor(lx / d)), cz = (uint32_t)std::max(1.0, std::floor(lz / d));
+    uint32_t nx = (uint32_t)std::max(1.0, std::round(std::sqrt(nxz * lx / lz)));
+    nx = std::min(nx, cx);
+    uint32_t nz = (uint32_t)std::max(1.0, std::round(nxz / nx));
+    nz = std::min(nz, cz);
+    out[0] = nx;
+    out[1] = ny;
+    out[2] = nz;
+}
+
+double uniform(SplitMix64 &r, double a, double b) { return a + (b - a) * ((double)(r.next() >> 11) * 0x1.0p-53); }
+
+}  // namespace
+
+SceneSample sample_batch_scene(uint64_t seed, uint32_t index, uint32_t nmin, uint32_t nmax) {
+    SplitMix64 r(seed * 0x9e3779b97f4a7c15ull + (uint64_t)index * 0xd1b54a32d192ed03ull + 1);
+    SceneSample s;
+    const double n = std::exp(uniform(r, std::log((double)nmin), std::log((double)nmax)));
+    const float lx = (float)uniform(r, 1.0, 4.0), lz = (float)uniform(r, 1.0, 4.0), ly = 1.0f;
+    const double fill = uniform(r, 0.1, 0.5);
+    s.sound_speed = (float)uniform(r, 15.0, 30.0);
+    s.alpha = (float)uniform(r, 0.01, 0.05);
+    s.dam.tank[0] = lx;
+    s.dam.tank[1] = ly;
+    s.dam.tank[2] = lz;
+    s.dam.d = 0.02f;
+    s.dam.jitter = 0.05f;
+    s.dam.seed = r.next();
+    s.dam.walls = true;
+    block_dims(n, lx, lz, fill * ly, s.dam.d, s.dam.block);
+    s.fill = (double)s.dam.block[0] * s.dam.block[1] * s.dam.block[2] * 0.02 * 0.02 * 0.02 / ((double)lx * ly * lz);
+    return s;
+}
+
+std::vector<SceneSample> sample_requests(uint64_t seed, uint32_t count, double rate, uint32_t nmin, uint32_t nmax,
+                                         uint32_t smin, uint32_t smax, uint32_t max_boxes) {
+    std::vector<SceneSample> out;
+    SplitMix64 clock(seed ^ 0x5851f42d4c957f2dull);
+    double t = 0.0;
+    for (uint32_t k = 0; k < count; ++k) {
+        t += -std::log(1.0 - uniform(clock, 0.0, 1.0)) / rate;
+        SceneSample s = sample_batch_scene(seed ^ 0xa0761d6478bd642full, k, nmin, nmax);
+        SplitMix64 r(seed * 0xe7037ed1a0b428dbull + (uint64_t)k * 0x8ebc6af09c88c6e3ull + 7);
+        s.arrival_s = t;
+        s.steps = (uint32_t)std::min<double>(smax, std::floor(uniform(r, smin, (double)smax + 1.0)));
+        s.nboxes = max_boxes ? (uint32_t)std::min<double>(max_boxes, std::floor(uniform(r, 0.0, max_boxes + 1.0))) : 0u;
+        // Boxes dropped above the fluid surface inside the tank.
+        const float top = (float)s.dam.block[1] * s.dam.d;
+        for (uint32_t b = 0; b < s.nboxes; ++b) {
+            BoxBody &bx = s.boxes[b];
+            const float side = (float)uniform(r, 0.1, 0.3);
+            bx.id = 100u + b;
+            bx.half[0] = bx.half[1] = bx.half[2] = 0.5f * side;
+            bx.pos[0] = (float)uniform(r, 0.3, s.dam.tank[0] - 0.3);
+            bx.pos[1] = top + 0.2f + side * (float)b;
+            bx.pos[2] = (float)uniform(r, 0.3, s.dam.tank[2] - 0.3);
+            s.box_density[b] = (float)uniform(r, 500.0, 2000.0);
+        }
+        out.push_back(s);
+    }
+    return out;
+}
+
 }  // namespace vxl

@@ -91,6 +91,34 @@ void dam_break(const DamBreakSpec &spec, const SceneConfig &cfg, std::vector<flo
                uint32_t &nf);
 std::vector<BoxBody> tank_bodies(const float tank[3], float d);
 
+// A sampled dam-break scene: geometry, physics and (served form) its request.
+struct SceneSample {
+    DamBreakSpec dam;
+    float sound_speed = 20.0f;
+    float alpha = 0.02f;
+    uint32_t steps = 1000;
+    double arrival_s = 0.0;
+    uint32_t nboxes = 0;
+    BoxBody boxes[4];
+    float box_density[4] = {0, 0, 0, 0};
+    double fill = 0.0;  // realised fluid volume / tank volume
+};
+
+// Batched form: scene `index` of a seeded set. Fluid count log-uniform in
+// [nmin, nmax], tank x, z uniform in [1, 4] m with y = 1 m, fill fraction
+// uniform in [0.1, 0.5] (block height = fill * 1 m, footprint from the count
+// with the tank's aspect, clamped to the tank), c0 uniform in [15, 30] m/s,
+// alpha uniform in [0.01, 0.05], d = 0.02 m.
+SceneSample sample_batch_scene(uint64_t seed, uint32_t index, uint32_t nmin = 8192, uint32_t nmax = 524288);
+
+// Served form: request `index` of a seeded Poisson stream of `rate` requests/s.
+// Fluid count log-uniform in [nmin, nmax], steps uniform in [smin, smax],
+// 0..max_boxes dynamic boxes of side 0.1-0.3 m and density 500-2000 kg/m^3
+// dropped above the fluid. Arrival times are cumulative exponential gaps.
+std::vector<SceneSample> sample_requests(uint64_t seed, uint32_t count, double rate, uint32_t nmin = 4096,
+                                         uint32_t nmax = 2097152, uint32_t smin = 100, uint32_t smax = 2000,
+                                         uint32_t max_boxes = 4);
+
 // splitmix64 stream used by every seeded scene.
 struct SplitMix64 {
     uint64_t s;

@@ -1,0 +1,62 @@
+// Continuous-batching server for dam-break step requests on one device.
+//
+// Requests arrive over time (a recorded arrival schedule, optionally time
+// compressed). Their scenes are built by host worker threads once they have
+// arrived, admitted in arrival order as soon as the pool's HBM budget allows,
+// and co-stepped with every other resident scene in one launch per stage.
+// Finished scenes are retired (state hash recorded) at step boundaries;
+// admissions and retirements between two steps share one pool relayout.
+#ifndef VXL_SPH_SERVE_H
+#define VXL_SPH_SERVE_H
+#include <cstdint>
+#include <vector>
+
+#include "sph_engine.h"
+
+namespace vxl {
+
+struct ServeConfig {
+    uint64_t budget_bytes = 200ull << 30;  // device bytes the resident scenes may use
+    double time_scale = 1.0;               // arrival schedule speed-up (1 = real time)
+    double max_wall_s = 3600.0;            // stop admitting/stepping after this
+    uint32_t max_resident = 0;             // 0 = limited by the budget only
+    uint32_t builders = 4;                 // host threads building arrived scenes
+    uint64_t max_bins = 1ull << 21;        // grid budget of every scene
+    float dt = 5e-4f;
+    float d = 0.02f;
+};
+
+struct ServeResult {
+    uint32_t index;
+    uint32_t n, nf, steps;
+    double arrival_s;   // schedule time (s)
+    double admit_s;     // wall time since start, scaled to schedule time
+    double finish_s;
+    uint64_t state_hash;
+};
+
+struct ServeStats {
+    uint32_t requests = 0, completed = 0;
+    uint64_t pool_steps = 0;
+    uint64_t fluid_updates = 0;
+    double wall_s = 0.0;
+    double updates_per_s = 0.0;
+    double step_ms_p50 = 0.0, step_ms_p99 = 0.0, step_ms_mean = 0.0;   // device time of a pool step
+    double loop_ms_p50 = 0.0, loop_ms_p99 = 0.0;                        // wall time per step incl. admission
+    double latency_s_p50 = 0.0, latency_s_p99 = 0.0;                    // arrival -> finish (schedule s)
+    double wait_s_p50 = 0.0, wait_s_p99 = 0.0;                          // arrival -> admission
+    uint32_t peak_resident = 0;
+    uint64_t peak_bytes = 0;
+    uint64_t layouts = 0;
+};
+
+// Builds the scene of one request (dam break + tank; boxes are not
+// supported by the served path yet and are rejected).
+void build_request(const SceneSample &req, const ServeConfig &cfg, SceneConfig &sc, std::vector<float> &pos,
+                   std::vector<float> &vel, std::vector<float> &pmass, std::vector<Span> &spans, uint32_t &nf);
+
+ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeConfig &cfg,
+                 std::vector<ServeResult> &results);
+
+}  // namespace vxl
+#endif

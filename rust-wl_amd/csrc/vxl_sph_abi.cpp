@@ -10,6 +10,7 @@
 #include <string>
 
 #include "sph_engine.h"
+#include "sph_serve.h"
 
 using namespace vxl;
 
@@ -533,6 +534,184 @@ VxlSphStatus vxl_sph_dam_break(const VxlSphDamBreak *spec, const VxlSphFluidConf
                 for (int a = 0; a < 3; ++a) spans[k].origin[a] = sp[k].origin[a];
                 spans[k].start = sp[k].start;
                 spans[k].end = sp[k].end;
+            }
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+void to_request(const SceneSample &s, VxlSphRequest &r) {
+    std::memset(&r, 0, sizeof(r));
+    r.dam.hdr.size = sizeof(VxlSphDamBreak);
+    r.dam.hdr.version = VXL_SPH_ABI_VERSION;
+    for (int a = 0; a < 3; ++a) {
+        r.dam.tank[a] = s.dam.tank[a];
+        r.dam.block[a] = s.dam.block[a];
+    }
+    r.dam.d = s.dam.d;
+    r.dam.jitter = s.dam.jitter;
+    r.dam.seed = s.dam.seed;
+    r.dam.walls = s.dam.walls ? 1u : 0u;
+    r.sound_speed = s.sound_speed;
+    r.artificial_viscosity = s.alpha;
+    r.steps = s.steps;
+    r.nboxes = s.nboxes;
+    r.arrival_s = s.arrival_s;
+    r.fill = s.fill;
+    for (uint32_t b = 0; b < 4; ++b) {
+        r.boxes[b].id = s.boxes[b].id;
+        for (int a = 0; a < 3; ++a) {
+            r.boxes[b].half[a] = s.boxes[b].half[a];
+            r.boxes[b].pos[a] = s.boxes[b].pos[a];
+            r.boxes[b].linvel[a] = s.boxes[b].linvel[a];
+            r.boxes[b].angvel[a] = s.boxes[b].angvel[a];
+        }
+        for (int a = 0; a < 4; ++a) r.boxes[b].rot[a] = s.boxes[b].rot[a];
+        r.box_density[b] = s.box_density[b];
+    }
+}
+
+SceneSample from_request(const VxlSphRequest &r) {
+    SceneSample s;
+    for (int a = 0; a < 3; ++a) {
+        s.dam.tank[a] = r.dam.tank[a];
+        s.dam.block[a] = r.dam.block[a];
+    }
+    s.dam.d = r.dam.d;
+    s.dam.jitter = r.dam.jitter;
+    s.dam.seed = r.dam.seed;
+    s.dam.walls = r.dam.walls != 0;
+    s.sound_speed = r.sound_speed;
+    s.alpha = r.artificial_viscosity;
+    s.steps = r.steps;
+    s.nboxes = r.nboxes;
+    s.arrival_s = r.arrival_s;
+    s.fill = r.fill;
+    for (uint32_t b = 0; b < 4; ++b) {
+        s.boxes[b].id = r.boxes[b].id;
+        for (int a = 0; a < 3; ++a) {
+            s.boxes[b].half[a] = r.boxes[b].half[a];
+            s.boxes[b].pos[a] = r.boxes[b].pos[a];
+            s.boxes[b].linvel[a] = r.boxes[b].linvel[a];
+            s.boxes[b].angvel[a] = r.boxes[b].angvel[a];
+        }
+        for (int a = 0; a < 4; ++a) s.boxes[b].rot[a] = r.boxes[b].rot[a];
+        s.box_density[b] = r.box_density[b];
+    }
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+VxlSphStatus vxl_sph_sample_batch(uint64_t seed, uint32_t count, uint32_t nmin, uint32_t nmax, VxlSphRequest *out) {
+    if (!out && count) return fail(VXL_SPH_NULL_POINTER, "out is null");
+    if (nmin == 0 || nmax < nmin) return fail(VXL_SPH_INVALID_ARGUMENT, "bad particle-count range");
+    return guard([&] {
+        for (uint32_t k = 0; k < count; ++k) to_request(sample_batch_scene(seed, k, nmin, nmax), out[k]);
+    });
+}
+
+VxlSphStatus vxl_sph_sample_requests(uint64_t seed, uint32_t count, double rate, uint32_t nmin, uint32_t nmax,
+                                     uint32_t smin, uint32_t smax, uint32_t max_boxes, VxlSphRequest *out) {
+    if (!out && count) return fail(VXL_SPH_NULL_POINTER, "out is null");
+    if (!(rate > 0.0) || nmin == 0 || nmax < nmin || smin == 0 || smax < smin || max_boxes > 4)
+        return fail(VXL_SPH_INVALID_ARGUMENT, "bad request stream parameters");
+    return guard([&] {
+        const auto v = sample_requests(seed, count, rate, nmin, nmax, smin, smax, max_boxes);
+        for (uint32_t k = 0; k < count; ++k) to_request(v[k], out[k]);
+    });
+}
+
+VxlSphStatus vxl_sph_request_config(const VxlSphRequest *req, float dt, uint64_t max_bins, VxlSphFluidConfig *out) {
+    if (!req || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        ServeConfig sc;
+        sc.dt = dt;
+        sc.max_bins = max_bins ? max_bins : (1ull << 21);
+        SceneSample s = from_request(*req);
+        s.nboxes = 0;
+        SceneConfig c;
+        std::vector<float> p, v, m;
+        std::vector<Span> sp;
+        uint32_t nf = 0;
+        // Only the configuration is needed: build a one-particle block.
+        s.dam.block[0] = s.dam.block[1] = s.dam.block[2] = 1;
+        s.dam.walls = false;
+        build_request(s, sc, c, p, v, m, sp, nf);
+        vxl_sph_default_config(out);
+        out->rest_density = c.rest_density;
+        out->smoothing_radius = c.smoothing_radius;
+        out->xsph_viscosity = c.xsph_viscosity;
+        out->sound_speed = c.sound_speed;
+        out->gamma_tait = c.gamma_tait;
+        out->substeps = c.substeps;
+        out->tensile_clamp = c.tensile_clamp;
+        out->boundary_layers = c.boundary_layers;
+        for (int a = 0; a < 3; ++a) out->gravity[a] = c.gravity[a];
+        out->max_speed_frac = c.max_speed_frac;
+        out->artificial_viscosity = c.artificial_viscosity;
+        out->particle_mass = c.particle_mass;
+        out->spacing = c.spacing;
+        out->max_bins = c.max_bins;
+        out->dt_tick = c.dt_tick;
+    });
+}
+
+VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSphRequest *reqs,
+                           VxlSphServeStats *stats, VxlSphServeResult *results) {
+    if (!cfg || !stats || (n && !reqs)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphServeConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        ServeConfig sc;
+        sc.budget_bytes = cfg->budget_bytes;
+        sc.time_scale = cfg->time_scale > 0 ? cfg->time_scale : 1.0;
+        sc.max_wall_s = cfg->max_wall_s > 0 ? cfg->max_wall_s : 3600.0;
+        sc.max_resident = cfg->max_resident;
+        sc.builders = cfg->builders ? cfg->builders : 4;
+        sc.max_bins = cfg->max_bins ? cfg->max_bins : (1ull << 21);
+        sc.dt = cfg->dt > 0 ? cfg->dt : 5e-4f;
+        std::vector<SceneSample> v(n);
+        for (uint32_t k = 0; k < n; ++k) {
+            v[k] = from_request(reqs[k]);
+            if (v[k].nboxes) throw std::domain_error("requests with dynamic rigid boxes are not supported yet");
+        }
+        Engine eng(cfg->device);
+        std::vector<ServeResult> res;
+        const ServeStats s = serve(eng, v, sc, res);
+        std::memset(stats, 0, sizeof(*stats));
+        stats->requests = s.requests;
+        stats->completed = s.completed;
+        stats->pool_steps = s.pool_steps;
+        stats->fluid_updates = s.fluid_updates;
+        stats->wall_s = s.wall_s;
+        stats->updates_per_s = s.updates_per_s;
+        stats->step_ms_p50 = s.step_ms_p50;
+        stats->step_ms_p99 = s.step_ms_p99;
+        stats->step_ms_mean = s.step_ms_mean;
+        stats->loop_ms_p50 = s.loop_ms_p50;
+        stats->loop_ms_p99 = s.loop_ms_p99;
+        stats->latency_s_p50 = s.latency_s_p50;
+        stats->latency_s_p99 = s.latency_s_p99;
+        stats->wait_s_p50 = s.wait_s_p50;
+        stats->wait_s_p99 = s.wait_s_p99;
+        stats->peak_resident = s.peak_resident;
+        stats->peak_bytes = s.peak_bytes;
+        stats->layouts = s.layouts;
+        if (results)
+            for (uint32_t k = 0; k < n; ++k) {
+                results[k].index = res[k].index;
+                results[k].n = res[k].n;
+                results[k].n_fluid = res[k].nf;
+                results[k].steps = res[k].steps;
+                results[k].arrival_s = res[k].arrival_s;
+                results[k].admit_s = res[k].admit_s;
+                results[k].finish_s = res[k].finish_s;
+                results[k].state_hash = res[k].state_hash;
             }
     });
 }

@@ -94,6 +94,34 @@ class DamBreakSpec(C.Structure):
                 ("jitter", C.c_float), ("seed", C.c_uint64), ("walls", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class Request(C.Structure):
+    _fields_ = [("dam", DamBreakSpec), ("sound_speed", C.c_float), ("artificial_viscosity", C.c_float),
+                ("steps", C.c_uint32), ("nboxes", C.c_uint32), ("arrival_s", C.c_double), ("fill", C.c_double),
+                ("boxes", BoxBody * 4), ("box_density", C.c_float * 4)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hdr", Header), ("budget_bytes", C.c_uint64), ("time_scale", C.c_double),
+                ("max_wall_s", C.c_double), ("max_resident", C.c_uint32), ("builders", C.c_uint32),
+                ("max_bins", C.c_uint64), ("dt", C.c_float), ("device", C.c_int32)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("requests", C.c_uint32), ("completed", C.c_uint32), ("pool_steps", C.c_uint64),
+                ("fluid_updates", C.c_uint64), ("wall_s", C.c_double), ("updates_per_s", C.c_double),
+                ("step_ms_p50", C.c_double), ("step_ms_p99", C.c_double), ("step_ms_mean", C.c_double),
+                ("loop_ms_p50", C.c_double), ("loop_ms_p99", C.c_double), ("latency_s_p50", C.c_double),
+                ("latency_s_p99", C.c_double), ("wait_s_p50", C.c_double), ("wait_s_p99", C.c_double),
+                ("peak_resident", C.c_uint32), ("reserved", C.c_uint32), ("peak_bytes", C.c_uint64),
+                ("layouts", C.c_uint64)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("index", C.c_uint32), ("n", C.c_uint32), ("n_fluid", C.c_uint32), ("steps", C.c_uint32),
+                ("arrival_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
+                ("state_hash", C.c_uint64)]
+
+
 _lib = None
 
 # Every symbol declared in include/vxl_sph.h.
@@ -110,6 +138,7 @@ EXPORTS = (
     "vxl_sph_stepper_create", "vxl_sph_stepper_destroy", "vxl_sph_stepper_step", "vxl_sph_stepper_reactions",
     "vxl_sph_stepper_bounds", "vxl_sph_stepper_read_state",
     "vxl_sph_box_lattice", "vxl_sph_box_particles", "vxl_sph_fluid_constants", "vxl_sph_dam_break",
+    "vxl_sph_sample_batch", "vxl_sph_sample_requests", "vxl_sph_request_config", "vxl_sph_serve",
 )
 
 
@@ -192,6 +221,16 @@ def lib():
     L.vxl_sph_box_particles.restype = st
     L.vxl_sph_box_particles.argtypes = [C.POINTER(FluidConfig), C.c_uint32, C.POINTER(BoxBody), C.c_uint32, up, up,
                                         fp, fp, fp, C.POINTER(Span)]
+    L.vxl_sph_sample_batch.restype = st
+    L.vxl_sph_sample_batch.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Request)]
+    L.vxl_sph_sample_requests.restype = st
+    L.vxl_sph_sample_requests.argtypes = [C.c_uint64, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_uint32, C.POINTER(Request)]
+    L.vxl_sph_request_config.restype = st
+    L.vxl_sph_request_config.argtypes = [C.POINTER(Request), C.c_float, C.c_uint64, C.POINTER(FluidConfig)]
+    L.vxl_sph_serve.restype = st
+    L.vxl_sph_serve.argtypes = [C.POINTER(ServeConfig), C.c_uint32, C.POINTER(Request), C.POINTER(ServeStats),
+                                C.POINTER(ServeResult)]
     L.vxl_sph_fluid_constants.restype = st
     L.vxl_sph_fluid_constants.argtypes = [C.POINTER(FluidConfig), fp]
     L.vxl_sph_dam_break.restype = st
@@ -519,3 +558,57 @@ class FluidStepper:
         vel = np.zeros((self._n, 3), np.float32)
         _check(lib().vxl_sph_stepper_read_state(self.h, self._n, _f(pos), _f(vel)))
         return pos, vel
+
+
+def sample_batch(seed, count, nmin=8192, nmax=524288):
+    """Batched-form scene set (count seeded dam breaks of uneven size)."""
+    arr = (Request * max(count, 1))()
+    _check(lib().vxl_sph_sample_batch(seed, count, nmin, nmax, arr))
+    return [arr[k] for k in range(count)]
+
+
+def sample_requests(seed, count, rate=20.0, nmin=4096, nmax=2097152, smin=100, smax=2000, max_boxes=0):
+    """Served-form Poisson request stream."""
+    arr = (Request * max(count, 1))()
+    _check(lib().vxl_sph_sample_requests(seed, count, rate, nmin, nmax, smin, smax, max_boxes, arr))
+    return [arr[k] for k in range(count)]
+
+
+def request_config(req, dt=5e-4, max_bins=1 << 21):
+    cfg = FluidConfig()
+    _check(lib().vxl_sph_request_config(C.byref(req), C.c_float(dt), max_bins, C.byref(cfg)))
+    return cfg
+
+
+def request_scene(req, dt=5e-4, max_bins=1 << 21):
+    """(config, SceneData) of one request, built exactly as the server builds it."""
+    cfg = request_config(req, dt, max_bins)
+    dam = req.dam
+    data = dam_break(cfg, tuple(dam.tank), tuple(dam.block), dam.d, jitter=dam.jitter, seed=dam.seed,
+                     walls=bool(dam.walls))
+    return cfg, data
+
+
+def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_resident=0, builders=4,
+          max_bins=1 << 21, dt=5e-4, device=0):
+    """Continuous-batching run over a request list; returns (stats dict, results list)."""
+    cfg = ServeConfig()
+    cfg.hdr.size = C.sizeof(ServeConfig)
+    cfg.hdr.version = ABI_VERSION
+    cfg.budget_bytes = budget_bytes
+    cfg.time_scale = time_scale
+    cfg.max_wall_s = max_wall_s
+    cfg.max_resident = max_resident
+    cfg.builders = builders
+    cfg.max_bins = max_bins
+    cfg.dt = dt
+    cfg.device = device
+    n = len(reqs)
+    arr = (Request * max(n, 1))(*reqs)
+    st = ServeStats()
+    res = (ServeResult * max(n, 1))()
+    _check(lib().vxl_sph_serve(C.byref(cfg), n, arr, C.byref(st), res))
+    stats = {f: getattr(st, f) for f, _ in ServeStats._fields_ if f != "reserved"}
+    results = [dict(index=r.index, n=r.n, n_fluid=r.n_fluid, steps=r.steps, arrival_s=r.arrival_s,
+                    admit_s=r.admit_s, finish_s=r.finish_s, state_hash=r.state_hash) for r in res[:n]]
+    return stats, results

@@ -276,3 +276,42 @@ def test_force_kernel_variants_bit_exact(depth):
     np.testing.assert_array_equal(bits(f["acc"]), bits(orc.acc()))
     np.testing.assert_array_equal(bits(f["xsph"]), bits(orc.xsph()))
     assert eng.state_hash(sid) == orc.state_hash()
+
+
+def test_batch_equal_alone_and_oracle():
+    reqs = V.sample_batch(11, 5, nmin=2000, nmax=30000)
+    built = [V.request_scene(r) + (tuple(r.dam.tank), r.dam.d) for r in reqs]
+    eng = V.Engine()
+    ids = [eng.add(cfg, data, reactions=False) for cfg, data, _, _ in built]
+    eng.step(4)
+    for sid, (cfg, data, tank, d) in zip(ids, built):
+        orc = oracle_dam_break(cfg, data, tank, d, threads=8)
+        for _ in range(4):
+            orc.step(cfg.dt_tick)
+        assert eng.state_hash(sid) == orc.state_hash()
+
+
+def test_serve_results_equal_alone():
+    """Served form: requests arriving over time, admitted as they arrive and
+    co-stepped with whatever else is resident; every result is bit-identical
+    to the same request stepped alone."""
+    reqs = V.sample_requests(5, 8, rate=200.0, nmin=1500, nmax=12000, smin=3, smax=12)
+    stats, results = V.serve(reqs, budget_bytes=1 << 30, time_scale=1.0, builders=2)
+    assert stats["completed"] == len(reqs)
+    assert stats["peak_resident"] >= 2  # requests overlapped on the device
+    for r, res in zip(reqs, results):
+        cfg, data = V.request_scene(r)
+        eng = V.Engine()
+        sid = eng.add(cfg, data, reactions=False)
+        eng.step(r.steps)
+        assert eng.state_hash(sid) == res["state_hash"]
+        eng.close()
+
+
+def test_serve_residency_limit_and_fifo_admission():
+    reqs = V.sample_requests(6, 6, rate=1000.0, nmin=1500, nmax=4000, smin=4, smax=6)
+    stats, results = V.serve(reqs, max_resident=2, builders=2)
+    assert stats["completed"] == len(reqs)
+    assert stats["peak_resident"] == 2
+    admits = [r["admit_s"] for r in sorted(results, key=lambda r: r["arrival_s"])]
+    assert admits == sorted(admits)  # admitted in arrival order
