@@ -46,6 +46,18 @@ def stage_bytes(name, N, NF, C):
     return m.get("N", 0) * N + m.get("NF", 0) * NF + m.get("C", 0) * C
 
 
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), for the default
+    workload only; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "wcsph_10M_traffic.json")
+    if kernel != "force" or args.block != 215 or args.budget_log2 != 22 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return round(t["traffic_bytes"])
+
+
 def build_scene(V, block, d, tank, max_bins, seed=1):
     cfg = V.dam_break_config(d, dt=2.5e-4, max_bins=max_bins)
     data = V.dam_break(cfg, tank, block, d, seed=seed)
@@ -121,7 +133,7 @@ def bench_batched(V, args):
 def bench_served(V, args):
     """Continuous batching over a Poisson request stream (scaled-down form of
     the 2000-request stream so the default run stays short)."""
-    reqs = V.sample_requests(9, args.serve_requests, rate=20.0, nmin=4096, nmax=524288, smin=50, smax=200)
+    reqs = V.sample_requests(9, args.serve_requests, rate=20.0, nmin=4096, nmax=2097152, smin=100, smax=2000)
     stats, results = V.serve(reqs, budget_bytes=200 << 30, time_scale=1.0, builders=8)
     print(f"# served: {stats['completed']}/{stats['requests']} requests, {stats['updates_per_s']:.3e} upd/s, "
           f"step p50 {stats['step_ms_p50']:.3f} ms p99 {stats['step_ms_p99']:.3f} ms, "
@@ -130,8 +142,9 @@ def bench_served(V, args):
             "step_ms_p99", "loop_ms_p50", "loop_ms_p99", "latency_s_p50", "latency_s_p99", "wait_s_p50",
             "wait_s_p99", "peak_resident", "peak_bytes", "layouts")
     out = {k: stats[k] for k in keep}
-    out["workload"] = (f"{args.serve_requests} Poisson requests at 20/s, fluid log-uniform [4096, 524288], "
-                       "steps U[50, 200], no rigid boxes, HBM budget 200 GB, real-time arrivals")
+    out["workload"] = (f"first {args.serve_requests} requests of the seeded Poisson stream (20 requests/s, fluid "
+                       "log-uniform [4096, 2097152], steps U[100, 2000], rigid boxes off), HBM budget 200 GB, "
+                       "real-time arrivals")
     return out
 
 
@@ -261,7 +274,9 @@ def main():
                                f"grid budget 2^{args.budget_log2} (bin {float(g['bin']):.3g} m, dims {g['dims']})",
                    "fluid_particles": NF, "total_particles": N, "grid_cells": C - 1},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom, args),
+                     "algorithmic_bytes": dom_bytes, "kernel_ms": round(kern[dom], 4)},
         "cpu_baseline": cpu,
         "step_latency_ms": {"p50": p50, "p99": p99},
         "step_roofline": {"algorithmic_bytes_per_step": b_step, "achieved_GBs": round(step_gbs, 1),

@@ -194,6 +194,8 @@ void Engine::alloc_pool(uint32_t slots) {
     F4(P_.cA); F4(P_.cV); U(P_.cg);
     F4(spare_cA_); F4(spare_cV_); U(spare_cg_);
     F4(P_.sA); F4(P_.sV); U(P_.sg);
+    P_.part = dalloc<SceneBox>(slots / kBlock + 1);
+    pool_allocs_.push_back(P_.part);
     F(P_.c2); F(P_.press);
     U(P_.key); U(P_.slot); U(P_.items_m); U(P_.items_g);
     if (diagnostics_) {
@@ -391,6 +393,7 @@ void Engine::layout() {
     hip_check(hipMemcpy(d_box_, init.data(), sizeof(SceneBox) * init.size(), hipMemcpyHostToDevice), "box");
     if (P_.nblocks) {
         hipLaunchKernelGGL(k_box, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(k_box_reduce, dim3(std::min<uint32_t>(P_.nscenes, 4096)), dim3(kBlock), 0, stream_, P_);
         hip_check(hipGetLastError(), "k_box");
     }
     hip_check(hipStreamSynchronize(stream_), "layout sync");
@@ -428,6 +431,8 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
         default: hipLaunchKernelGGL(k_force_t<32>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
     }
     M("force");
+    hipLaunchKernelGGL(k_box_reduce, dim3(std::min<uint32_t>(P_.nscenes, 4096)), dim3(kBlock), 0, stream_, P_);
+    M("box_reduce");
     if (nspans_) {
         hipLaunchKernelGGL(k_bforce, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
         M("bforce");
@@ -642,20 +647,9 @@ void Engine::upload_boundary(uint32_t id, uint32_t nb, const float *pos, const f
     }
     hip_check(hipMemcpy(P_.cA + pb, a.data(), 16ull * n, hipMemcpyHostToDevice), "A");
     hip_check(hipMemcpy(P_.cV + pb, v.data(), 16ull * n, hipMemcpyHostToDevice), "V");
-    // Boundary motion changes the AABB of the next grid build.
-    std::vector<SceneBox> boxes(2 * scenes_.size());
-    hip_check(hipMemcpy(boxes.data(), d_box_, sizeof(SceneBox) * boxes.size(), hipMemcpyDeviceToHost), "box");
-    const uint32_t par = h_ctl_->step & 1u;
-    uint32_t k = 0;
-    for (; k < scenes_.size(); ++k)
-        if (scenes_[k].id == id) break;
-    SceneBox &b = boxes[par * scenes_.size() + k];
-    for (int a = 0; a < 3; ++a) {
-        b.lo[a] = 0xffffffffu;
-        b.hi[a] = 0u;
-    }
-    hip_check(hipMemcpy(d_box_, boxes.data(), sizeof(SceneBox) * boxes.size(), hipMemcpyHostToDevice), "box");
+    // Boundary motion changes the AABB of the next grid build: recompute it.
     hipLaunchKernelGGL(k_box, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_box_reduce, dim3(std::min<uint32_t>(P_.nscenes, 4096)), dim3(kBlock), 0, stream_, P_);
     hip_check(hipGetLastError(), "k_box");
     hip_check(hipStreamSynchronize(stream_), "sync");
 }

@@ -34,6 +34,7 @@ struct Pool {
     const SceneParams *params;
     SceneGrid *grid;
     SceneBox *box;  // [2][nscenes] ping-pong
+    SceneBox *part; // [nblocks] per-block partial boxes
     Control *ctl;
     const uint32_t *block_scene;
     uint32_t nblocks;  // pool slots / kBlock
@@ -47,6 +48,7 @@ struct Pool {
 };
 
 __global__ void k_box(Pool P);
+__global__ void k_box_reduce(Pool P);
 __global__ void k_grid_box(Pool P);
 __global__ void k_keys(Pool P);
 __global__ void k_scan_reduce(Pool P, uint32_t *tile_sums);

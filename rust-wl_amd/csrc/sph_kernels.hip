@@ -94,9 +94,11 @@ __device__ __forceinline__ void for_rows(const SceneGrid &g, const uint32_t *__r
     }
 }
 
-// Block-wide AABB of the positions handed in by live lanes, merged into one
-// scene's accumulator with 6 integer atomics (min/max are order free, exact).
-__device__ __forceinline__ void block_box(bool live, float x, float y, float z, SceneBox *dst) {
+// Block-wide AABB of the positions handed in by live lanes, written as this
+// block's partial box (ordered u32; the per-scene reduction runs after the
+// launch). No atomics: thousands of blocks merging into six words would
+// serialise on one L2 channel.
+__device__ __forceinline__ void block_box(bool live, float x, float y, float z, SceneBox *part) {
     __shared__ uint32_t sh[6][kBlock / 64];
     uint32_t lx = live ? f2o(x) : 0xffffffffu, ly = live ? f2o(y) : 0xffffffffu,
              lz = live ? f2o(z) : 0xffffffffu;
@@ -118,10 +120,8 @@ __device__ __forceinline__ void block_box(bool live, float x, float y, float z, 
             lx = min(lx, sh[0][k]); ly = min(ly, sh[1][k]); lz = min(lz, sh[2][k]);
             hx = max(hx, sh[3][k]); hy = max(hy, sh[4][k]); hz = max(hz, sh[5][k]);
         }
-        if (lx != 0xffffffffu) {
-            atomicMin(&dst->lo[0], lx); atomicMin(&dst->lo[1], ly); atomicMin(&dst->lo[2], lz);
-            atomicMax(&dst->hi[0], hx); atomicMax(&dst->hi[1], hy); atomicMax(&dst->hi[2], hz);
-        }
+        part->lo[0] = lx; part->lo[1] = ly; part->lo[2] = lz;
+        part->hi[0] = hx; part->hi[1] = hy; part->hi[2] = hz;
     }
     __syncthreads();
 }
@@ -138,7 +138,50 @@ __global__ __launch_bounds__(kBlock) void k_box(Pool P) {
         const bool live = m < sp.pbase + sp.n;
         float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (live) a = P.cA[m];
-        block_box(live, a.x, a.y, a.z, &P.box[par * P.nscenes + s]);
+        block_box(live, a.x, a.y, a.z, &P.part[b]);
+    }
+}
+
+// Per-scene AABB = reduction of its blocks' partial boxes, into box[parity].
+__global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
+    if (P.ctl->err) return;
+    const uint32_t par = P.ctl->step & 1u;
+    __shared__ uint32_t sh[6][kBlock / 64];
+    for (uint32_t s = blockIdx.x; s < P.nscenes; s += gridDim.x) {
+        const SceneParams &sp = P.params[s];
+        const uint32_t b0 = sp.pbase / kBlock, b1 = (sp.pbase + sp.alloc) / kBlock;
+        uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+        for (uint32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
+            const SceneBox q = P.part[b];
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = min(lo[a], q.lo[a]);
+                hi[a] = max(hi[a], q.hi[a]);
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = wave_min_u32(lo[a]);
+            hi[a] = wave_max_u32(hi[a]);
+        }
+        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+        if (l == 0)
+            for (int a = 0; a < 3; ++a) {
+                sh[a][w] = lo[a];
+                sh[3 + a][w] = hi[a];
+            }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            SceneBox &o = P.box[par * P.nscenes + s];
+            for (int a = 0; a < 3; ++a) {
+                uint32_t mn = sh[a][0], mx = sh[3 + a][0];
+                for (int k = 1; k < (int)(kBlock / 64); ++k) {
+                    mn = min(mn, sh[a][k]);
+                    mx = max(mx, sh[3 + a][k]);
+                }
+                o.lo[a] = mn;
+                o.hi[a] = mx;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -530,7 +573,6 @@ __device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const f
 template <uint32_t DEPTH>
 __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
     if (P.ctl->err) return;
-    const uint32_t par = P.ctl->step & 1u;
     __shared__ uint32_t lst[DEPTH > 0 ? DEPTH * kBlock : 1];
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
         const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
@@ -630,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
             P.cV[i] = vo;
             P.cg[i] = P.sg[i];
         }
-        block_box(live, ox, oy, oz, &P.box[par * P.nscenes + s]);
+        block_box(live, ox, oy, oz, &P.part[b]);
     }
 }
 
