@@ -159,7 +159,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--force-depth", type=int, default=None, help="override the force kernel's LDS list depth")
+    ap.add_argument("--force-kernel", type=int, default=None, choices=(1, 2),
+                    help="force kernel instead of the autotuner's choice: 1 candidate scan, 2 neighbour walk")
+    ap.add_argument("--density-kernel", type=int, default=None, choices=(1, 2),
+                    help="density kernel instead of the autotuner's choice: 1 fused, 2 split test/sum passes")
     ap.add_argument("--sweep", action="store_true", help="print per-kernel times for every tuning variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
@@ -200,19 +203,21 @@ def main():
 
     eng = V.Engine()
     sid = eng.add(cfg, data, reactions=False)
-    if args.force_depth is not None:
-        eng.set_tuning(force_depth=args.force_depth)
+    if args.force_kernel is not None or args.density_kernel is not None:
+        eng.set_tuning(force_kernel=args.force_kernel or 0, density_kernel=args.density_kernel or 0)
     else:
         tuned = eng.autotune(2)
         print(f"# autotuned: {tuned}", file=sys.stderr, flush=True)
     if args.sweep:
         eng.step(2)
-        for fd in (0, 16, 32, 64):
-            eng.set_tuning(force_depth=fd)
-            pr = eng.profile(3)
-            print(f"# sweep force_depth={fd}: force {pr['force'] / 3:.3f} ms, density {pr['density'] / 3:.3f} ms",
-                  file=sys.stderr, flush=True)
-        eng.autotune(2)
+        saved = eng.get_tuning()
+        for fk in (V.FORCE_SCAN, V.FORCE_WALK):
+            for dk in (V.DENSITY_FUSED, V.DENSITY_SPLIT):
+                eng.set_tuning(force_kernel=fk, density_kernel=dk)
+                pr = eng.profile(3)
+                print(f"# sweep force_kernel={fk} density_kernel={dk}: force {pr['force'] / 3:.3f} ms, "
+                      f"density {pr['density'] / 3:.3f} ms", file=sys.stderr, flush=True)
+        eng.set_tuning(**saved)
     eng.step(args.warmup)
     eng.sync()
     t0 = time.perf_counter()

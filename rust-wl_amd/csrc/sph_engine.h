@@ -83,13 +83,15 @@ public:
     hipStream_t stream() const { return stream_; }
     uint64_t substeps_done() const { return substeps_done_; }
 
-    // Launch geometry knobs (autotuned). 0 = default.
+    // Launch variants (autotuned; every variant gives identical results).
+    enum : uint32_t { kForceScan = 1, kForceWalk = 2, kDensityFused = 1, kDensitySplit = 2 };
     struct Tuning {
-        uint32_t persist_blocks = 0;  // grid-stride grid size cap for slot kernels
-        uint32_t force_depth = 32;    // LDS neighbour-list depth of the force kernel (0, 16, 32, 64)
-        uint32_t density_depth = 0;   // LDS neighbour-list depth of the density kernel (0, 16, 32)
+        uint32_t persist_blocks = 0;            // grid-stride grid size cap for slot kernels (0 = none)
+        uint32_t force_kernel = kForceWalk;     // kForceScan | kForceWalk
+        uint32_t density_kernel = kDensityFused;  // kDensityFused | kDensitySplit
     };
-    void set_tuning(const Tuning &t) { tuning_ = t; invalidate_graph(); }
+    // 0 in a kernel field selects its default.
+    void set_tuning(const Tuning &t);
     // Time every launch variant on the current pool for `substeps` substeps
     // each (they are real substeps: all variants give identical results) and
     // keep the fastest. Results are cached per (device arch, log2 particle

@@ -29,9 +29,9 @@ uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
 }  // namespace
 
 uint64_t Engine::footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins) {
-    // cur + spare (36 B each), grid-ordered copy 36, c2/press 8, sort 16 B per
-    // slot; counts + start 8 B per reserved cell.
-    const uint64_t slot_bytes = 36 + 36 + 36 + 8 + 16;
+    // cur + spare (36 B each), grid-ordered copy 36, c2/press 8, sort 16,
+    // neighbour record 40 B per slot; counts + start 8 B per reserved cell.
+    const uint64_t slot_bytes = 36 + 36 + 36 + 8 + 16 + 40;
     return slot_bytes * round_up(n, kBlock) + 8ull * cell_reserve(cells_hint, max_bins);
 }
 
@@ -198,6 +198,9 @@ void Engine::alloc_pool(uint32_t slots) {
     pool_allocs_.push_back(P_.part);
     F(P_.c2); F(P_.press);
     U(P_.key); U(P_.slot); U(P_.items_m); U(P_.items_g);
+    P_.nbm = dalloc<uint32_t>(9ull * slots);
+    pool_allocs_.push_back(P_.nbm);
+    U(P_.nbs);
     if (diagnostics_) {
         U(P_.ncount);
         F4(P_.acc); F4(P_.xsph);
@@ -422,14 +425,15 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     M("place");
     hipLaunchKernelGGL(k_canon_gather, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("canon_gather");
-    hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    if (tuning_.density_kernel == kDensitySplit)
+        hipLaunchKernelGGL(k_density_t<1>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    else
+        hipLaunchKernelGGL(k_density_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("density");
-    switch (tuning_.force_depth) {
-        case 0: hipLaunchKernelGGL(k_force_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
-        case 16: hipLaunchKernelGGL(k_force_t<16>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
-        case 64: hipLaunchKernelGGL(k_force_t<64>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
-        default: hipLaunchKernelGGL(k_force_t<32>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_); break;
-    }
+    if (tuning_.force_kernel == kForceScan)
+        hipLaunchKernelGGL(k_force_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    else
+        hipLaunchKernelGGL(k_force_t<1>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("force");
     hipLaunchKernelGGL(k_box_reduce, dim3(std::min<uint32_t>(P_.nscenes, 4096)), dim3(kBlock), 0, stream_, P_);
     M("box_reduce");
@@ -500,13 +504,15 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
     }
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
     std::vector<Tuning> space;
-    for (uint32_t fd : {0u, 16u, 32u, 64u})
-        for (uint32_t pb : {0u, 8u * cus}) {
-            Tuning t;
-            t.force_depth = fd;
-            t.persist_blocks = pb;
-            space.push_back(t);
-        }
+    for (uint32_t fk : {(uint32_t)kForceScan, (uint32_t)kForceWalk})
+        for (uint32_t dk : {(uint32_t)kDensityFused, (uint32_t)kDensitySplit})
+            for (uint32_t pb : {0u, 8u * cus}) {
+                Tuning t;
+                t.force_kernel = fk;
+                t.density_kernel = dk;
+                t.persist_blocks = pb;
+                space.push_back(t);
+            }
     const Tuning saved = tuning_;
     Tuning best = saved;
     float best_ms = 1e30f;
@@ -541,6 +547,15 @@ std::vector<float> Engine::step_timed(uint32_t ticks) {
     for (auto e : evs) (void)hipEventDestroy(e);
     check_error_state();
     return ms;
+}
+
+void Engine::set_tuning(const Tuning &t) {
+    tuning_ = t;
+    if (tuning_.force_kernel == 0) tuning_.force_kernel = Tuning{}.force_kernel;
+    if (tuning_.density_kernel == 0) tuning_.density_kernel = Tuning{}.density_kernel;
+    if (tuning_.force_kernel > kForceWalk || tuning_.density_kernel > kDensitySplit)
+        throw std::invalid_argument("unknown kernel variant");
+    invalidate_graph();
 }
 
 void Engine::build_graph() {

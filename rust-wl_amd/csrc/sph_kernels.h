@@ -24,6 +24,13 @@ struct Pool {
     uint32_t *sg;
     float *c2;     // p / (rho * rho)
     float *press;  // p
+    // accepted neighbours, written by the density pass, read by the force pass:
+    //   nbm[r * nblocks * kBlock + i]  bit k = candidate j0 + k of (dz, dy) row r
+    //                                  (r = 3 (dz + 1) + dy + 1) is a neighbour
+    //   nbs[i]  bits 0-8: row r holds a neighbour or a tail; 9-17: row r starts
+    //           at cell dx = -1; 18-26: row r has more than 32 candidates (the
+    //           tail j0 + 32 .. end is tested again by the reader)
+    uint32_t *nbm, *nbs;
     // sort
     uint32_t *key, *slot, *items_m, *items_g;
     uint32_t *counts, *start;  // counts[ccap], start[ccap + 1]
@@ -56,9 +63,10 @@ __global__ void k_scan_tiles(Pool P, uint32_t *tile_sums);
 __global__ void k_scan_apply(Pool P, const uint32_t *tile_sums);
 __global__ void k_place(Pool P);
 __global__ void k_canon_gather(Pool P);
-__global__ void k_density(Pool P);
-template <uint32_t DEPTH>
-__global__ void k_force_t(Pool P);  // DEPTH = LDS neighbour-list depth (0 = direct)
+template <uint32_t TWO_PASS>
+__global__ void k_density_t(Pool P);  // 0: sums while testing; 1: test pass, then sums over the masks
+template <uint32_t WALK>
+__global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks
 __global__ void k_bforce(Pool P);
 __global__ void k_reactions(Pool P);
 __global__ void k_unsort(Pool P, float *opos, float *ovel);

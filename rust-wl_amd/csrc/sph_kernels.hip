@@ -69,8 +69,10 @@ __device__ __forceinline__ Cursor make_cursor(const SceneGrid &g, float h, float
 }
 
 // Visit the 3x3 rows of the 27-cell neighbourhood in the reference order
-// (dz outer, dy inner); each row is handed over as one slot range covering
-// the cells dx = -1, 0, +1 that survive the conservative distance test.
+// (dz outer, dy inner). Each row is handed over as its index
+// r = 3 (dz + 1) + dy + 1, whether it starts at the cell dx = -1, and one slot
+// range covering the cells dx = -1, 0, +1 that survive the conservative
+// distance test.
 template <class F>
 __device__ __forceinline__ void for_rows(const SceneGrid &g, const uint32_t *__restrict__ start, const Cursor &c,
                                          F &&visit) {
@@ -89,7 +91,81 @@ __device__ __forceinline__ void for_rows(const SceneGrid &g, const uint32_t *__r
             const int xl = (c.cx > 0 && rd + lx <= c.cut) ? c.cx - 1 : c.cx;
             const int xr = (c.cx + 1 < nx && rd + rx <= c.cut) ? c.cx + 1 : c.cx;
             const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-            visit(start[row + (uint32_t)xl], start[row + (uint32_t)xr + 1u]);
+            visit((uint32_t)(3 * (dz + 1) + dy + 1), xl < c.cx, start[row + (uint32_t)xl],
+                  start[row + (uint32_t)xr + 1u]);
+        }
+    }
+}
+
+// Last cell (dx = 0 or +1) of row r as for_rows chooses it.
+__device__ __forceinline__ uint32_t row_last_cell(const SceneGrid &g, const Cursor &c, uint32_t r) {
+    const uint32_t q = (r * 11u) >> 5;  // r / 3 for r < 9
+    const int dz = (int)q - 1, dy = (int)(r - 3u * q) - 1;
+    const float ez = dz < 0 ? c.fz : (dz > 0 ? 1.0f - c.fz : 0.0f);
+    const float ey = dy < 0 ? c.fy : (dy > 0 ? 1.0f - c.fy : 0.0f);
+    const float rd = ez * ez + ey * ey;
+    const float rx = (1.0f - c.fx) * (1.0f - c.fx);
+    return (c.cx + 1 < (int)g.dims[0] && rd + rx <= c.cut) ? (uint32_t)c.cx + 1u : (uint32_t)c.cx;
+}
+
+// Accepted-neighbour record of one row, as the density pass writes it: bit k
+// of the mask = candidate j0 + k passed r^2 <= h^2 (and is not i itself);
+// candidates past j0 + 32 are left to the reader.
+constexpr uint32_t kRowsPresent = 0x1ffu;
+__device__ __forceinline__ uint32_t row_summary(uint32_t r, bool left, bool tail) {
+    return (1u << r) | (left ? 1u << (9u + r) : 0u) | (tail ? 1u << (18u + r) : 0u);
+}
+
+// Resumable walk over the neighbours the density pass accepted, in the
+// reference enumeration order: rows in ascending r, inside a row ascending
+// slot (= cell by cell, ascending original index).
+struct NbWalk {
+    uint32_t summ, pend;  // summary word; rows not yet opened
+    uint32_t mk, j0;      // accepted candidates j0 + k (bit k) of the open row
+    uint32_t jt, je;      // tail candidates [jt, je) of the open row, tested here
+};
+
+__device__ __forceinline__ NbWalk nb_begin(uint32_t summ) {
+    NbWalk w;
+    w.summ = summ;
+    w.pend = summ & kRowsPresent;
+    w.mk = w.j0 = w.jt = w.je = 0u;
+    return w;
+}
+
+__device__ __forceinline__ bool nb_next(NbWalk &w, const Pool &P, const SceneGrid &g, const Cursor &c, uint32_t i,
+                                        const float4 ai, float h2, uint32_t stride, uint32_t &jj) {
+    for (;;) {
+        if (w.mk) {
+            jj = w.j0 + (uint32_t)__builtin_ctz(w.mk);
+            w.mk &= w.mk - 1u;
+            return true;
+        }
+        if (w.jt < w.je) {
+            const uint32_t j = w.jt++;
+            const float4 aj = P.sA[j];
+            const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (r2 <= h2 && j != i) {
+                jj = j;
+                return true;
+            }
+            continue;
+        }
+        if (!w.pend) return false;
+        const uint32_t r = (uint32_t)__builtin_ctz(w.pend);
+        w.pend &= w.pend - 1u;
+        const uint32_t q = (r * 11u) >> 5;  // r / 3 for r < 9
+        const uint32_t z = (uint32_t)(c.cz + (int)q - 1), y = (uint32_t)(c.cy + (int)(r - 3u * q) - 1);
+        const uint32_t row = g.cbase + (z * g.dims[1] + y) * g.dims[0];
+        const uint32_t xl = (uint32_t)c.cx - ((w.summ >> (9u + r)) & 1u);
+        w.j0 = P.start[row + xl];
+        w.mk = P.nbm[r * stride + i];
+        if ((w.summ >> (18u + r)) & 1u) {
+            w.jt = w.j0 + 32u;
+            w.je = P.start[row + row_last_cell(g, c, r) + 1u];
+        } else {
+            w.jt = w.je = 0u;
         }
     }
 }
@@ -440,9 +516,19 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
 }
 
 // Density (fluid_density.rs:17-70, no providers) + Tait (fluid_force.rs:11-29,
-// gamma = 7) + the per-particle pressure term p / (rho * rho) used by the force.
-__global__ __launch_bounds__(kBlock) void k_density(Pool P) {
+// gamma = 7) + the per-particle pressure term p / (rho * rho) used by the
+// force. Records every fluid particle's accepted neighbours (nbm, nbs) for the
+// force pass.
+//
+// TWO_PASS = 0: the sums are taken while the candidates are tested, without
+// branches: a rejected candidate leaves them untouched through a select.
+// TWO_PASS = 1: a test-only pass records the neighbours of every particle,
+// then the sums run over the recorded neighbours only.
+// Either way the accumulation order and bits are those of the reference loop.
+template <uint32_t TWO_PASS>
+__global__ __launch_bounds__(kBlock) void k_density_t(Pool P) {
     if (P.ctl->err) return;
+    const uint32_t stride = P.nblocks * kBlock;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
         const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
         const uint32_t s = P.block_scene[b];
@@ -457,34 +543,52 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
         float sum = fluid_i ? sp.w0 : 0.0f;
         float sum_b = 0.0f;
         uint32_t cnt = 0;
+        uint32_t summ = 0;
         const float4 *__restrict__ sA = P.sA;
-        // Candidates are tested without branches: a rejected candidate leaves
-        // the sums untouched through a select, so the accumulation order and
-        // bits are those of the reference loop.
-        const float4 *__restrict__ sAc = sA;
-        for_rows(g, P.start, cur, [&](uint32_t j0, uint32_t j1) {
+        // One candidate's contribution (a select when `acc` is false).
+        auto add = [&](bool acc, const float4 aj) {
+            const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const float t = h2 - r2;
+            const float w = ((k6 * t) * t) * t;
+            const bool fj = aj.w > 0.0f;
+            const float mj = fabsf(aj.w);
+            const float add_sum = fluid_i ? w : mj * w;
+            sum = (acc && fj) ? sum + add_sum : sum;
+            sum_b = (acc && !fj && fluid_i) ? sum_b + mj * w : sum_b;
+            cnt += acc ? 1u : 0u;
+        };
+        const bool record = TWO_PASS || fluid_i;
+        for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
             constexpr uint32_t U = 4;
+            uint32_t mk = 0;
             for (uint32_t jb = j0; jb < j1; jb += U) {
                 float4 aj[U];
 #pragma unroll
-                for (uint32_t u = 0; u < U; ++u) aj[u] = sAc[min(jb + u, j1 - 1u)];
+                for (uint32_t u = 0; u < U; ++u) aj[u] = sA[min(jb + u, j1 - 1u)];
 #pragma unroll
                 for (uint32_t u = 0; u < U; ++u) {
                     const uint32_t j = jb + u;
                     const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
                     const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     const bool acc = (r2 <= h2) && (j != i) && (j < j1);
-                    const float t = h2 - r2;
-                    const float w = ((k6 * t) * t) * t;
-                    const bool fj = aj[u].w > 0.0f;
-                    const float mj = fabsf(aj[u].w);
-                    const float add_sum = fluid_i ? w : mj * w;
-                    sum = (acc && fj) ? sum + add_sum : sum;
-                    sum_b = (acc && !fj && fluid_i) ? sum_b + mj * w : sum_b;
-                    cnt += acc ? 1u : 0u;
+                    const uint32_t k = j - j0;
+                    mk |= ((acc && k < 32u) ? 1u : 0u) << (k & 31u);
+                    if (!TWO_PASS) add(acc, aj[u]);
                 }
             }
+            const bool tail = j1 - j0 > 32u;
+            if (record && (mk != 0u || tail)) {
+                P.nbm[r * stride + i] = mk;
+                summ |= row_summary(r, left, tail);
+            }
         });
+        if (record) P.nbs[i] = summ;
+        if (TWO_PASS) {
+            NbWalk w = nb_begin(summ);
+            uint32_t j;
+            while (nb_next(w, P, g, cur, i, ai, h2, stride, j)) add(true, sA[j]);
+        }
         const float rho = fluid_i ? sp.mass * sum + sum_b : (-ai.w) * sp.w0 + sum;
         const float q = rho / sp.rho0;
         const float q2 = q * q;
@@ -498,29 +602,8 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
     }
 }
 
-// Row-window iterator over the reference enumeration order (for_rows as a
-// resumable cursor): advances `r` to the next surviving (dz, dy) row and sets
-// [j, jend); returns false after the last row.
-__device__ __forceinline__ bool next_row(const SceneGrid &g, const uint32_t *__restrict__ start, const Cursor &c,
-                                         int &r, uint32_t &j, uint32_t &jend) {
-    const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
-    while (++r < 9) {
-        const int dz = r / 3 - 1, dy = r % 3 - 1;
-        const int z = c.cz + dz, y = c.cy + dy;
-        if (z < 0 || z >= nz || y < 0 || y >= ny) continue;
-        const float ez = dz < 0 ? c.fz : (dz > 0 ? 1.0f - c.fz : 0.0f);
-        const float ey = dy < 0 ? c.fy : (dy > 0 ? 1.0f - c.fy : 0.0f);
-        const float rd = ez * ez + ey * ey;
-        if (rd > c.cut) continue;
-        const int xl = (c.cx > 0 && rd + c.fx * c.fx <= c.cut) ? c.cx - 1 : c.cx;
-        const int xr = (c.cx + 1 < nx && rd + (1.0f - c.fx) * (1.0f - c.fx) <= c.cut) ? c.cx + 1 : c.cx;
-        const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-        j = start[row + (uint32_t)xl];
-        jend = start[row + (uint32_t)xr + 1u];
-        if (j < jend) return true;
-    }
-    return false;
-}
+template __global__ void k_density_t<0>(Pool P);
+template __global__ void k_density_t<1>(Pool P);
 
 // One neighbour pair of the force sum, fluid_force.rs:165-217 operation order.
 struct ForceAcc {
@@ -562,18 +645,18 @@ __device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const f
 // Force (fluid_force.rs:165-217: gravity + spiky pressure + Monaghan AV, XSPH)
 // fused with the semi-implicit Euler + CFL clamp of fluid_step.rs:110-123.
 // Writes the next state into the cur arrays (this substep's grid order) and
-// accumulates the next substep's AABB. Boundary slots are carried over.
+// the block's partial AABB of the next positions. Boundary slots are carried
+// over.
 //
-// DEPTH = 0: every lane evaluates the pair terms inside its candidate loop.
-// DEPTH > 0: in-wave compaction. Each lane scans its candidates and appends
-// the accepted neighbour slots (in enumeration order) to its own LDS list of
-// DEPTH entries; the wave then evaluates the pair terms over the lists, so the
-// expensive branch runs once per accepted pair instead of once per candidate
-// iteration of the slowest lane. Summation order is unchanged.
-template <uint32_t DEPTH>
+// WALK = 0: every lane tests its candidates and evaluates the accepted pairs
+// in its candidate loop.
+// WALK = 1: every lane walks the neighbours the density pass recorded (same
+// positions, same test), so the candidate scan is not repeated; the wave
+// evaluates one pair per lane per iteration.
+template <uint32_t WALK>
 __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
     if (P.ctl->err) return;
-    __shared__ uint32_t lst[DEPTH > 0 ? DEPTH * kBlock : 1];
+    const uint32_t stride = P.nblocks * kBlock;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
         const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
         const uint32_t s = P.block_scene[b];
@@ -593,11 +676,11 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
         ForceAcc f{sp.gx, sp.gy, sp.gz, 0.0f, 0.0f, 0.0f};
         float ci2 = 0.0f;
         if (fluid) ci2 = c2[i];
-        if constexpr (DEPTH == 0) {
+        const float h2 = sp.h2;
+        if constexpr (WALK == 0) {
             if (fluid) {
                 const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
-                const float h2 = sp.h2;
-                for_rows(g, P.start, cur, [&](uint32_t j0, uint32_t j1) {
+                for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t j0, uint32_t j1) {
                     for (uint32_t j = j0; j < j1; ++j) {
                         if (j == i) continue;
                         const float4 aj = sA[j];
@@ -610,39 +693,16 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
             }
         } else {
             Cursor cur{};
-            if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
-            const float h2 = sp.h2;
-            int r = -1;
-            uint32_t j = 0, jend = 0;
-            bool done = !fluid;
+            NbWalk w = nb_begin(0u);
+            if (fluid) {
+                cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+                w = nb_begin(P.nbs[i]);
+            }
             for (;;) {
-                uint32_t cnt = 0;
-                // Scan candidates 4 at a time (loads first), appending the
-                // accepted slots in order; stop once the list could overflow.
-                while (!done && cnt + 4u <= DEPTH) {
-                    if (j >= jend) {
-                        if (!next_row(g, P.start, cur, r, j, jend)) done = true;
-                        continue;
-                    }
-                    float4 aj[4];
-#pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) aj[u] = sA[min(j + u, jend - 1u)];
-#pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) {
-                        const uint32_t jj = j + u;
-                        const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
-                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        const bool acc = (r2 <= h2) && (jj != i) && (jj < jend);
-                        if (acc) lst[cnt * kBlock + threadIdx.x] = jj;
-                        cnt += acc ? 1u : 0u;
-                    }
-                    j = min(j + 4u, jend);
-                }
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    const uint32_t jj = lst[k * kBlock + threadIdx.x];
-                    force_pair(f, ai, vi, ci2, sA[jj], sV[jj], c2[jj], sp);
-                }
-                if (!__any(!done)) break;
+                uint32_t j = 0;
+                const bool have = nb_next(w, P, g, cur, i, ai, h2, stride, j);
+                if (!__any(have)) break;
+                if (have) force_pair(f, ai, vi, ci2, sA[j], sV[j], c2[j], sp);
             }
         }
         float ox = ai.x, oy = ai.y, oz = ai.z;
@@ -677,9 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
 }
 
 template __global__ void k_force_t<0>(Pool P);
-template __global__ void k_force_t<16>(Pool P);
-template __global__ void k_force_t<32>(Pool P);
-template __global__ void k_force_t<64>(Pool P);
+template __global__ void k_force_t<1>(Pool P);
 
 // Reaction on every boundary particle of a scene with reactions:
 // bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending
@@ -708,7 +766,7 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
         float kx[kReactMaxNeighbours], ky[kReactMaxNeighbours], kz[kReactMaxNeighbours];
         uint32_t cnt = 0;
         bool over = false;
-        for_rows(g, P.start, cur, [&](uint32_t i0, uint32_t i1) {
+        for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t i0, uint32_t i1) {
             for (uint32_t i = i0; i < i1; ++i) {
                 const float4 ai = P.sA[i];
                 if (!(ai.w > 0.0f)) continue;  // fluid neighbours only

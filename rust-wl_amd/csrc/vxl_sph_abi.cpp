@@ -202,14 +202,13 @@ VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t) {
     if (!e || !t) return fail(VXL_SPH_NULL_POINTER, "null argument");
     VxlSphStatus st = check_header(t->hdr, sizeof(VxlSphTuning));
     if (st != VXL_SPH_OK) return st;
-    const auto ok_depth = [](uint32_t d, bool allow64) { return d == 0 || d == 16 || d == 32 || (allow64 && d == 64); };
-    if (!ok_depth(t->force_depth, true) || !ok_depth(t->density_depth, false))
-        return fail(VXL_SPH_INVALID_ARGUMENT, "unsupported neighbour-list depth");
+    if (t->force_kernel > VXL_SPH_FORCE_WALK || t->density_kernel > VXL_SPH_DENSITY_SPLIT)
+        return fail(VXL_SPH_INVALID_ARGUMENT, "unknown kernel variant");
     return guard([&] {
         Engine::Tuning tu;
         tu.persist_blocks = t->persist_blocks;
-        tu.force_depth = t->force_depth;
-        tu.density_depth = t->density_depth;
+        tu.force_kernel = t->force_kernel;
+        tu.density_kernel = t->density_kernel;
         e->eng.set_tuning(tu);
     });
 }
@@ -221,8 +220,8 @@ VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out) {
     out->hdr.size = sizeof(VxlSphTuning);
     out->hdr.version = VXL_SPH_ABI_VERSION;
     out->persist_blocks = tu.persist_blocks;
-    out->force_depth = tu.force_depth;
-    out->density_depth = tu.density_depth;
+    out->force_kernel = tu.force_kernel;
+    out->density_kernel = tu.density_kernel;
     return VXL_SPH_OK;
 }
 

@@ -260,15 +260,19 @@ def test_reference_akinci_canary_on_gpu():
     np.testing.assert_array_equal(bits(gft), bits(ft))
 
 
-@pytest.mark.parametrize("depth", [0, 16, 32, 64])
-def test_force_kernel_variants_bit_exact(depth):
-    """Every autotuner choice of the force kernel (direct or LDS-compacted
-    neighbour lists of depth 16/32/64) gives the oracle's state bit for bit."""
+@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK])
+@pytest.mark.parametrize("density_kernel", [V.DENSITY_FUSED, V.DENSITY_SPLIT])
+@pytest.mark.parametrize("persist", [0, 7])
+def test_kernel_variants_bit_exact(force_kernel, density_kernel, persist):
+    """Every autotuner choice (force: candidate scan or walk of the recorded
+    neighbours; density: fused or split test/sum passes; one block per 256
+    slots or a grid-stride loop over a few blocks) gives the oracle's state
+    bit for bit."""
     cfg, data, tank, d = small_scene(seed=13, block=(16, 12, 10), tank=(0.6, 0.5, 0.4))
     orc = oracle_dam_break(cfg, data, tank, d)
     eng = V.Engine(diagnostics=True)
     sid = eng.add(cfg, data, reactions=False)
-    eng.set_tuning(force_depth=depth)
+    eng.set_tuning(persist_blocks=persist, force_kernel=force_kernel, density_kernel=density_kernel)
     for _ in range(6):
         eng.step(1)
         orc.step(cfg.dt_tick)
