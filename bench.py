@@ -161,8 +161,6 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--force-kernel", type=int, default=None, choices=(1, 2),
                     help="force kernel instead of the autotuner's choice: 1 candidate scan, 2 neighbour walk")
-    ap.add_argument("--density-kernel", type=int, default=None, choices=(1, 2),
-                    help="density kernel instead of the autotuner's choice: 1 fused, 2 split test/sum passes")
     ap.add_argument("--sweep", action="store_true", help="print per-kernel times for every tuning variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
@@ -203,8 +201,8 @@ def main():
 
     eng = V.Engine()
     sid = eng.add(cfg, data, reactions=False)
-    if args.force_kernel is not None or args.density_kernel is not None:
-        eng.set_tuning(force_kernel=args.force_kernel or 0, density_kernel=args.density_kernel or 0)
+    if args.force_kernel is not None:
+        eng.set_tuning(force_kernel=args.force_kernel)
     else:
         tuned = eng.autotune(2)
         print(f"# autotuned: {tuned}", file=sys.stderr, flush=True)
@@ -212,10 +210,10 @@ def main():
         eng.step(2)
         saved = eng.get_tuning()
         for fk in (V.FORCE_SCAN, V.FORCE_WALK):
-            for dk in (V.DENSITY_FUSED, V.DENSITY_SPLIT):
-                eng.set_tuning(force_kernel=fk, density_kernel=dk)
+            for pb in (0, 2048):
+                eng.set_tuning(force_kernel=fk, persist_blocks=pb)
                 pr = eng.profile(3)
-                print(f"# sweep force_kernel={fk} density_kernel={dk}: force {pr['force'] / 3:.3f} ms, "
+                print(f"# sweep force_kernel={fk} persist_blocks={pb}: force {pr['force'] / 3:.3f} ms, "
                       f"density {pr['density'] / 3:.3f} ms", file=sys.stderr, flush=True)
         eng.set_tuning(**saved)
     eng.step(args.warmup)

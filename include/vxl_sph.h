@@ -117,19 +117,13 @@ typedef struct VxlSphTuning {
     VxlSphHeader hdr;
     uint32_t persist_blocks; /* grid size cap of the per-slot kernels (0 = one block per 256 slots) */
     uint32_t force_kernel;   /* VXL_SPH_FORCE_* */
-    uint32_t density_kernel; /* VXL_SPH_DENSITY_* */
-    uint32_t reserved;
+    uint32_t reserved[2];
 } VxlSphTuning;
 /* Force pass: every particle tests its neighbour candidates again (SCAN), or
  * walks the accepted neighbours the density pass recorded (WALK). */
 #define VXL_SPH_FORCE_DEFAULT 0u
 #define VXL_SPH_FORCE_SCAN 1u
 #define VXL_SPH_FORCE_WALK 2u
-/* Density pass: sums taken while the candidates are tested (FUSED), or a
- * test pass followed by sums over the accepted neighbours (SPLIT). */
-#define VXL_SPH_DENSITY_DEFAULT 0u
-#define VXL_SPH_DENSITY_FUSED 1u
-#define VXL_SPH_DENSITY_SPLIT 2u
 VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t);
 VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out);
 /* Autotuner: times every variant for `substeps` real substeps each (results

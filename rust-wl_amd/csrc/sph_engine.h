@@ -84,11 +84,10 @@ public:
     uint64_t substeps_done() const { return substeps_done_; }
 
     // Launch variants (autotuned; every variant gives identical results).
-    enum : uint32_t { kForceScan = 1, kForceWalk = 2, kDensityFused = 1, kDensitySplit = 2 };
+    enum : uint32_t { kForceScan = 1, kForceWalk = 2 };
     struct Tuning {
-        uint32_t persist_blocks = 0;            // grid-stride grid size cap for slot kernels (0 = none)
-        uint32_t force_kernel = kForceWalk;     // kForceScan | kForceWalk
-        uint32_t density_kernel = kDensityFused;  // kDensityFused | kDensitySplit
+        uint32_t persist_blocks = 0;         // grid-stride grid size cap for slot kernels (0 = none)
+        uint32_t force_kernel = kForceWalk;  // kForceScan | kForceWalk
     };
     // 0 in a kernel field selects its default.
     void set_tuning(const Tuning &t);
@@ -108,6 +107,7 @@ public:
 
 private:
     void layout();            // assign slot ranges, (re)allocate, move + upload
+    uint32_t box_reduce_grid() const;
     // The launch sequence of one substep; `mark` (if set) is called after each
     // kernel launch with the kernel's name.
     void enqueue_substep(const std::function<void(const char *)> *mark = nullptr);

@@ -116,6 +116,10 @@ uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, cons
                            const float *pmass, const Span *spans, uint32_t nspans, bool reactions) {
     if (nf > n) throw std::invalid_argument("n_fluid exceeds particle count");
     if (n == 0) throw std::invalid_argument("empty scene");
+    uint64_t slots = round_up(n, kBlock);
+    for (const auto &s : scenes_) slots += round_up(s.params.n, kBlock);
+    if (n > kMaxPoolSlots || slots > kMaxPoolSlots - kBlock)
+        throw CapacityError("a pool holds at most 2^28 particle slots");
     if (fabsf(cfg.gamma_tait - 7.0f) >= 1e-6f)
         throw std::domain_error("only the gamma = 7 Tait branch (fluid_force.rs:17-21) is supported");
     SceneSlot sc;
@@ -193,7 +197,9 @@ void Engine::alloc_pool(uint32_t slots) {
     };
     F4(P_.cA); F4(P_.cV); U(P_.cg);
     F4(spare_cA_); F4(spare_cV_); U(spare_cg_);
-    F4(P_.sA); F4(P_.sV); U(P_.sg);
+    P_.sA = dalloc<float4>((size_t)slots + kReadPad);
+    pool_allocs_.push_back(P_.sA);
+    F4(P_.sV); U(P_.sg);
     P_.part = dalloc<SceneBox>(slots / kBlock + 1);
     pool_allocs_.push_back(P_.part);
     F(P_.c2); F(P_.press);
@@ -250,6 +256,9 @@ void Engine::layout() {
         for (auto &sc : scenes_) v.push_back(sc.dev_base);
         return v;
     }();
+    uint64_t total = 0;
+    for (const auto &sc : scenes_) total += round_up(sc.params.n, kBlock);
+    if (total > kMaxPoolSlots - kBlock) throw CapacityError("a pool holds at most 2^28 particle slots");
     uint32_t used = 0;
     uint64_t cells = 0;
     std::vector<uint32_t> block_scene;
@@ -273,7 +282,7 @@ void Engine::layout() {
         // New pool; the old one stays alive until the resident state is copied.
         old_allocs.swap(pool_allocs_);
         const uint32_t ccap_keep = ccap_;
-        const uint32_t cap = std::max<uint32_t>(need, slots_ + slots_ / 2);
+        const uint32_t cap = std::min<uint32_t>(std::max<uint32_t>(need, slots_ + slots_ / 2), kMaxPoolSlots - kBlock);
         want_reaction_buffers_ = want_reaction_buffers_ || reactions;
         diag_alloc_ = diagnostics_;
         alloc_pool(round_up(cap, kBlock));
@@ -396,7 +405,7 @@ void Engine::layout() {
     hip_check(hipMemcpy(d_box_, init.data(), sizeof(SceneBox) * init.size(), hipMemcpyHostToDevice), "box");
     if (P_.nblocks) {
         hipLaunchKernelGGL(k_box, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_);
-        hipLaunchKernelGGL(k_box_reduce, dim3(std::min<uint32_t>(P_.nscenes, 4096)), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
         hip_check(hipGetLastError(), "k_box");
     }
     hip_check(hipStreamSynchronize(stream_), "layout sync");
@@ -425,17 +434,14 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     M("place");
     hipLaunchKernelGGL(k_canon_gather, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("canon_gather");
-    if (tuning_.density_kernel == kDensitySplit)
-        hipLaunchKernelGGL(k_density_t<1>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
-    else
-        hipLaunchKernelGGL(k_density_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("density");
     if (tuning_.force_kernel == kForceScan)
         hipLaunchKernelGGL(k_force_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     else
         hipLaunchKernelGGL(k_force_t<1>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("force");
-    hipLaunchKernelGGL(k_box_reduce, dim3(std::min<uint32_t>(P_.nscenes, 4096)), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
     M("box_reduce");
     if (nspans_) {
         hipLaunchKernelGGL(k_bforce, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
@@ -505,14 +511,12 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
     std::vector<Tuning> space;
     for (uint32_t fk : {(uint32_t)kForceScan, (uint32_t)kForceWalk})
-        for (uint32_t dk : {(uint32_t)kDensityFused, (uint32_t)kDensitySplit})
-            for (uint32_t pb : {0u, 8u * cus}) {
-                Tuning t;
-                t.force_kernel = fk;
-                t.density_kernel = dk;
-                t.persist_blocks = pb;
-                space.push_back(t);
-            }
+        for (uint32_t pb : {0u, 8u * cus}) {
+            Tuning t;
+            t.force_kernel = fk;
+            t.persist_blocks = pb;
+            space.push_back(t);
+        }
     const Tuning saved = tuning_;
     Tuning best = saved;
     float best_ms = 1e30f;
@@ -549,12 +553,14 @@ std::vector<float> Engine::step_timed(uint32_t ticks) {
     return ms;
 }
 
+uint32_t Engine::box_reduce_grid() const {
+    return std::max<uint32_t>(1u, std::min<uint32_t>((P_.nblocks + kBoxReduceSpan - 1) / kBoxReduceSpan, 4096u));
+}
+
 void Engine::set_tuning(const Tuning &t) {
     tuning_ = t;
     if (tuning_.force_kernel == 0) tuning_.force_kernel = Tuning{}.force_kernel;
-    if (tuning_.density_kernel == 0) tuning_.density_kernel = Tuning{}.density_kernel;
-    if (tuning_.force_kernel > kForceWalk || tuning_.density_kernel > kDensitySplit)
-        throw std::invalid_argument("unknown kernel variant");
+    if (tuning_.force_kernel > kForceWalk) throw std::invalid_argument("unknown kernel variant");
     invalidate_graph();
 }
 

@@ -55,6 +55,7 @@ struct Pool {
 };
 
 __global__ void k_box(Pool P);
+constexpr uint32_t kBoxReduceSpan = kBlock * 8;  // partial boxes per k_box_reduce block
 __global__ void k_box_reduce(Pool P);
 __global__ void k_grid_box(Pool P);
 __global__ void k_keys(Pool P);
@@ -63,8 +64,7 @@ __global__ void k_scan_tiles(Pool P, uint32_t *tile_sums);
 __global__ void k_scan_apply(Pool P, const uint32_t *tile_sums);
 __global__ void k_place(Pool P);
 __global__ void k_canon_gather(Pool P);
-template <uint32_t TWO_PASS>
-__global__ void k_density_t(Pool P);  // 0: sums while testing; 1: test pass, then sums over the masks
+__global__ void k_density(Pool P);
 template <uint32_t WALK>
 __global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks
 __global__ void k_bforce(Pool P);

@@ -51,6 +51,14 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t nb) {
     return x * q + min(x, r) + k;
 }
 
+// Record `idx` of a per-slot array through a 32-bit byte offset from the
+// array base (a pool holds at most 2^28 slots; records are at most 16 bytes),
+// which lets the load use a scalar base and a 32-bit vector offset.
+template <class T>
+__device__ __forceinline__ const T *at(const T *__restrict__ base, uint32_t idx) {
+    return (const T *)((const char *)base + idx * (uint32_t)sizeof(T));
+}
+
 // Neighbour geometry of one particle: its cell and where it sits inside it.
 struct Cursor {
     int cx, cy, cz;
@@ -123,51 +131,54 @@ struct NbWalk {
     uint32_t summ, pend;  // summary word; rows not yet opened
     uint32_t mk, j0;      // accepted candidates j0 + k (bit k) of the open row
     uint32_t jt, je;      // tail candidates [jt, je) of the open row, tested here
+    uint32_t rowb;        // internal cell of (cx, cy - 1, cz - 1) (mod 2^32)
+    uint32_t nx, nxny;    // row strides of the scene grid
 };
 
-__device__ __forceinline__ NbWalk nb_begin(uint32_t summ) {
+__device__ __forceinline__ NbWalk nb_begin(uint32_t summ, const SceneGrid &g, const Cursor &c) {
     NbWalk w;
     w.summ = summ;
     w.pend = summ & kRowsPresent;
     w.mk = w.j0 = w.jt = w.je = 0u;
+    w.nx = g.dims[0];
+    w.nxny = g.dims[0] * g.dims[1];
+    w.rowb = g.cbase + ((uint32_t)(c.cz - 1) * g.dims[1] + (uint32_t)(c.cy - 1)) * g.dims[0] + (uint32_t)c.cx;
     return w;
 }
 
-__device__ __forceinline__ bool nb_next(NbWalk &w, const Pool &P, const SceneGrid &g, const Cursor &c, uint32_t i,
-                                        const float4 ai, float h2, uint32_t stride, uint32_t &jj) {
-    for (;;) {
-        if (w.mk) {
-            jj = w.j0 + (uint32_t)__builtin_ctz(w.mk);
-            w.mk &= w.mk - 1u;
-            return true;
-        }
+enum : int { kWalkDone = 0, kWalkPair = 1, kWalkIdle = 2 };
+
+// One step of the walk: kWalkPair with the next neighbour in jj, kWalkIdle
+// when this step opened a row without mask bits or tested a rejected tail
+// candidate (more may follow), kWalkDone after the last neighbour.
+__device__ __forceinline__ int nb_step(NbWalk &w, const Pool &P, const SceneGrid &g, const Cursor &c, uint32_t i,
+                                       const float4 ai, float h2, uint32_t stride, uint32_t &jj) {
+    if (w.mk == 0u) {
         if (w.jt < w.je) {
             const uint32_t j = w.jt++;
-            const float4 aj = P.sA[j];
+            const float4 aj = *at(P.sA, j);
             const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
             const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (r2 <= h2 && j != i) {
-                jj = j;
-                return true;
-            }
-            continue;
+            jj = j;
+            return (r2 <= h2 && j != i) ? kWalkPair : kWalkIdle;
         }
-        if (!w.pend) return false;
+        if (w.pend == 0u) return kWalkDone;
         const uint32_t r = (uint32_t)__builtin_ctz(w.pend);
         w.pend &= w.pend - 1u;
         const uint32_t q = (r * 11u) >> 5;  // r / 3 for r < 9
-        const uint32_t z = (uint32_t)(c.cz + (int)q - 1), y = (uint32_t)(c.cy + (int)(r - 3u * q) - 1);
-        const uint32_t row = g.cbase + (z * g.dims[1] + y) * g.dims[0];
-        const uint32_t xl = (uint32_t)c.cx - ((w.summ >> (9u + r)) & 1u);
-        w.j0 = P.start[row + xl];
-        w.mk = P.nbm[r * stride + i];
+        const uint32_t row = w.rowb + q * w.nxny + (r - 3u * q) * w.nx;
+        const uint32_t first = row - ((w.summ >> (9u + r)) & 1u);
+        w.j0 = P.start[first];
+        w.mk = P.nbm[(size_t)r * stride + i];
         if ((w.summ >> (18u + r)) & 1u) {
             w.jt = w.j0 + 32u;
-            w.je = P.start[row + row_last_cell(g, c, r) + 1u];
-        } else {
-            w.jt = w.je = 0u;
+            w.je = P.start[row - (uint32_t)c.cx + row_last_cell(g, c, r) + 1u];
         }
+        if (w.mk == 0u) return kWalkIdle;
     }
+    jj = w.j0 + (uint32_t)__builtin_ctz(w.mk);
+    w.mk &= w.mk - 1u;
+    return kWalkPair;
 }
 
 // Block-wide AABB of the positions handed in by live lanes, written as this
@@ -218,46 +229,55 @@ __global__ __launch_bounds__(kBlock) void k_box(Pool P) {
     }
 }
 
-// Per-scene AABB = reduction of its blocks' partial boxes, into box[parity].
+// Per-scene AABB = reduction of its blocks' partial boxes into box[parity]
+// (reset to the empty box by k_grid_box / the layout). Each thread folds
+// kBoxChunk consecutive partial boxes (scenes own contiguous block ranges, so
+// a thread meets few scene boundaries), each wave folds its lanes when they
+// share one scene, and one lane per wave merges the result with atomics.
+constexpr uint32_t kBoxChunk = kBoxReduceSpan / kBlock;
+
+__device__ __forceinline__ void box_merge(SceneBox *o, const uint32_t lo[3], const uint32_t hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        atomicMin(&o->lo[a], lo[a]);
+        atomicMax(&o->hi[a], hi[a]);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
     if (P.ctl->err) return;
     const uint32_t par = P.ctl->step & 1u;
-    __shared__ uint32_t sh[6][kBlock / 64];
-    for (uint32_t s = blockIdx.x; s < P.nscenes; s += gridDim.x) {
-        const SceneParams &sp = P.params[s];
-        const uint32_t b0 = sp.pbase / kBlock, b1 = (sp.pbase + sp.alloc) / kBlock;
+    SceneBox *out = P.box + par * P.nscenes;
+    for (uint32_t base = blockIdx.x * kBlock * kBoxChunk; base < P.nblocks; base += gridDim.x * kBlock * kBoxChunk) {
         uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
-        for (uint32_t b = b0 + threadIdx.x; b < b1; b += kBlock) {
+        uint32_t cur = 0xffffffffu;  // scene of the running fold (none yet)
+        const uint32_t b0 = base + threadIdx.x * kBoxChunk;
+        for (uint32_t k = 0; k < kBoxChunk && b0 + k < P.nblocks; ++k) {
+            const uint32_t b = b0 + k;
+            const uint32_t sc = P.block_scene[b];
+            if (sc != cur) {
+                if (cur != 0xffffffffu) box_merge(&out[cur], lo, hi);
+                cur = sc;
+                for (int a = 0; a < 3; ++a) {
+                    lo[a] = 0xffffffffu;
+                    hi[a] = 0u;
+                }
+            }
             const SceneBox q = P.part[b];
             for (int a = 0; a < 3; ++a) {
                 lo[a] = min(lo[a], q.lo[a]);
                 hi[a] = max(hi[a], q.hi[a]);
             }
         }
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = wave_min_u32(lo[a]);
-            hi[a] = wave_max_u32(hi[a]);
-        }
-        const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-        if (l == 0)
+        const uint32_t s0 = (uint32_t)__shfl((int)cur, 0, 64);
+        if (__all(cur == s0 || cur == 0xffffffffu)) {
             for (int a = 0; a < 3; ++a) {
-                sh[a][w] = lo[a];
-                sh[3 + a][w] = hi[a];
+                lo[a] = wave_min_u32(lo[a]);
+                hi[a] = wave_max_u32(hi[a]);
             }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            SceneBox &o = P.box[par * P.nscenes + s];
-            for (int a = 0; a < 3; ++a) {
-                uint32_t mn = sh[a][0], mx = sh[3 + a][0];
-                for (int k = 1; k < (int)(kBlock / 64); ++k) {
-                    mn = min(mn, sh[a][k]);
-                    mx = max(mx, sh[3 + a][k]);
-                }
-                o.lo[a] = mn;
-                o.hi[a] = mx;
-            }
+            if ((threadIdx.x & 63u) == 0u && s0 != 0xffffffffu) box_merge(&out[s0], lo, hi);
+        } else if (cur != 0xffffffffu) {
+            box_merge(&out[cur], lo, hi);
         }
-        __syncthreads();
     }
 }
 
@@ -515,18 +535,79 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
     }
 }
 
+// Density sums of one particle (fluid_density.rs:17-70): fluid i sums W over
+// fluid neighbours (times its mass at the end) and m_b W over boundary ones;
+// boundary i sums m_j W over fluid neighbours only. A candidate with
+// acc = false leaves both sums untouched (a select, not a branch).
+struct DensAcc {
+    float sum, sum_b;
+    bool fluid_i;
+    __device__ __forceinline__ void add(bool acc, float r2, float aw, float h2, float k6) {
+        const float t = h2 - r2;
+        const float w = ((k6 * t) * t) * t;
+        const bool fj = aw > 0.0f;
+        const float mw = fabsf(aw) * w;
+        const float add_sum = fluid_i ? w : mw;
+        sum = (acc && fj) ? sum + add_sum : sum;
+        sum_b = (acc && !fj && fluid_i) ? sum_b + mw : sum_b;
+    }
+};
+
+// Four consecutive candidates jb .. jb + 3 of a row [j0, j1) (loaded without
+// clamping: the arrays are padded). SELF: the range may hold i; PART: the
+// group may run past j1; MASK: set the mask bits (candidate k = j - j0 < 32),
+// else count the accepted ones.
+template <bool SELF, bool PART, bool MASK>
+__device__ __forceinline__ void dens_group(const float4 *__restrict__ sA, const float4 ai, uint32_t i, uint32_t jb,
+                                           uint32_t j0, uint32_t j1, float h2, float k6, DensAcc &da, uint32_t &mk,
+                                           uint32_t &cnt) {
+    constexpr uint32_t U = 4;
+    const float4 *p = at(sA, jb);
+    float4 aj[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) aj[u] = p[u];
+    uint32_t gb = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t j = jb + u;
+        const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
+        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        bool acc = r2 <= h2;
+        if (SELF) acc = acc && j != i;
+        if (PART) acc = acc && j < j1;
+        if (MASK)
+            gb |= acc ? 1u << u : 0u;
+        else
+            cnt += acc ? 1u : 0u;
+        da.add(acc, r2, aj[u].w, h2, k6);
+    }
+    if (MASK) mk |= gb << (jb - j0);
+}
+
+// One row of the fused density loop; returns the mask of its first 32
+// candidates (those past them are counted into cnt).
+template <bool SELF>
+__device__ __forceinline__ uint32_t dens_row(const float4 *__restrict__ sA, const float4 ai, uint32_t i, uint32_t j0,
+                                             uint32_t j1, float h2, float k6, DensAcc &da, uint32_t &cnt) {
+    uint32_t mk = 0;
+    const uint32_t jm = min(j1, j0 + 32u);
+    uint32_t jb = j0;
+    for (; jb + 4u <= jm; jb += 4u) dens_group<SELF, false, true>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+    if (jb < jm) {
+        dens_group<SELF, true, true>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+        jb += 4u;
+    }
+    for (; jb + 4u <= j1; jb += 4u) dens_group<SELF, false, false>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+    if (jb < j1) dens_group<SELF, true, false>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+    return mk;
+}
+
 // Density (fluid_density.rs:17-70, no providers) + Tait (fluid_force.rs:11-29,
 // gamma = 7) + the per-particle pressure term p / (rho * rho) used by the
-// force. Records every fluid particle's accepted neighbours (nbm, nbs) for the
-// force pass.
-//
-// TWO_PASS = 0: the sums are taken while the candidates are tested, without
-// branches: a rejected candidate leaves them untouched through a select.
-// TWO_PASS = 1: a test-only pass records the neighbours of every particle,
-// then the sums run over the recorded neighbours only.
-// Either way the accumulation order and bits are those of the reference loop.
-template <uint32_t TWO_PASS>
-__global__ __launch_bounds__(kBlock) void k_density_t(Pool P) {
+// force. The sums are taken while the candidates are tested (selects keep the
+// reference's accumulation order and bits). Records every fluid particle's
+// accepted neighbours (nbm, nbs) for the force pass.
+__global__ __launch_bounds__(kBlock) void k_density(Pool P) {
     if (P.ctl->err) return;
     const uint32_t stride = P.nblocks * kBlock;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
@@ -540,56 +621,27 @@ __global__ __launch_bounds__(kBlock) void k_density_t(Pool P) {
         const bool fluid_i = ai.w > 0.0f;
         const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
         const float h2 = sp.h2, k6 = sp.k6;
-        float sum = fluid_i ? sp.w0 : 0.0f;
-        float sum_b = 0.0f;
-        uint32_t cnt = 0;
-        uint32_t summ = 0;
+        DensAcc da{fluid_i ? sp.w0 : 0.0f, 0.0f, fluid_i};
+        uint32_t cnt = 0, summ = 0;
         const float4 *__restrict__ sA = P.sA;
-        // One candidate's contribution (a select when `acc` is false).
-        auto add = [&](bool acc, const float4 aj) {
-            const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
-            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const float t = h2 - r2;
-            const float w = ((k6 * t) * t) * t;
-            const bool fj = aj.w > 0.0f;
-            const float mj = fabsf(aj.w);
-            const float add_sum = fluid_i ? w : mj * w;
-            sum = (acc && fj) ? sum + add_sum : sum;
-            sum_b = (acc && !fj && fluid_i) ? sum_b + mj * w : sum_b;
-            cnt += acc ? 1u : 0u;
-        };
-        const bool record = TWO_PASS || fluid_i;
-        for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
-            constexpr uint32_t U = 4;
-            uint32_t mk = 0;
-            for (uint32_t jb = j0; jb < j1; jb += U) {
-                float4 aj[U];
-#pragma unroll
-                for (uint32_t u = 0; u < U; ++u) aj[u] = sA[min(jb + u, j1 - 1u)];
-#pragma unroll
-                for (uint32_t u = 0; u < U; ++u) {
-                    const uint32_t j = jb + u;
-                    const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
-                    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    const bool acc = (r2 <= h2) && (j != i) && (j < j1);
-                    const uint32_t k = j - j0;
-                    mk |= ((acc && k < 32u) ? 1u : 0u) << (k & 31u);
-                    if (!TWO_PASS) add(acc, aj[u]);
-                }
-            }
+        // Row r finished with mask mk: count, and record it for a fluid i.
+        auto close_row = [&](uint32_t r, bool left, uint32_t j0, uint32_t j1, uint32_t mk) {
+            cnt += (uint32_t)__builtin_popcount(mk);
             const bool tail = j1 - j0 > 32u;
-            if (record && (mk != 0u || tail)) {
+            if (fluid_i && (mk != 0u || tail)) {
                 P.nbm[r * stride + i] = mk;
                 summ |= row_summary(r, left, tail);
             }
+        };
+        for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
+            // Only the particle's own row (r = 4, uniform across the wave) can
+            // hold i itself.
+            const uint32_t mk = r == 4u ? dens_row<true>(sA, ai, i, j0, j1, h2, k6, da, cnt)
+                                        : dens_row<false>(sA, ai, i, j0, j1, h2, k6, da, cnt);
+            close_row(r, left, j0, j1, mk);
         });
-        if (record) P.nbs[i] = summ;
-        if (TWO_PASS) {
-            NbWalk w = nb_begin(summ);
-            uint32_t j;
-            while (nb_next(w, P, g, cur, i, ai, h2, stride, j)) add(true, sA[j]);
-        }
-        const float rho = fluid_i ? sp.mass * sum + sum_b : (-ai.w) * sp.w0 + sum;
+        if (fluid_i) P.nbs[i] = summ;
+        const float rho = fluid_i ? sp.mass * da.sum + da.sum_b : (-ai.w) * sp.w0 + da.sum;
         const float q = rho / sp.rho0;
         const float q2 = q * q;
         const float q4 = q2 * q2;
@@ -602,16 +654,22 @@ __global__ __launch_bounds__(kBlock) void k_density_t(Pool P) {
     }
 }
 
-template __global__ void k_density_t<0>(Pool P);
-template __global__ void k_density_t<1>(Pool P);
+
 
 // One neighbour pair of the force sum, fluid_force.rs:165-217 operation order.
 struct ForceAcc {
     float ax, ay, az, xx, xy, xz;
 };
+// Scene constants of the pair terms, held in registers for the whole walk.
+struct ForceConst {
+    float h, h2, k6, ks, alpha_c, h2c;  // h2c = 0.01 h^2 (the AV softening term)
+};
+__device__ __forceinline__ ForceConst force_const(const SceneParams &sp) {
+    return ForceConst{sp.h, sp.h2, sp.k6, sp.ks, sp.alpha_c, 0.01f * sp.h2};
+}
 __device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const float4 vi, const float ci2,
-                                           const float4 aj, const float4 vj, const float cj2, const SceneParams &sp) {
-    const float h = sp.h, h2 = sp.h2, k6 = sp.k6, ks = sp.ks, alpha_c = sp.alpha_c;
+                                           const float4 aj, const float4 vj, const float cj2, const ForceConst &fc) {
+    const float h = fc.h, h2 = fc.h2, k6 = fc.k6, ks = fc.ks, alpha_c = fc.alpha_c;
     const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
     const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
     const float rho_i = vi.w, rho_j = vj.w;
@@ -627,7 +685,7 @@ __device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const f
     const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
     const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
     if (vdn > 0.0f) {
-        const float mu = (vdn * h) / (r2 + 0.01f * h2);
+        const float mu = (vdn * h) / (r2 + fc.h2c);
         const float cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
         const float qv = cv / denom;
         f.ax = f.ax + ddx * qv;
@@ -680,6 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
         if constexpr (WALK == 0) {
             if (fluid) {
                 const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+                const ForceConst fc = force_const(sp);
                 for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t j0, uint32_t j1) {
                     for (uint32_t j = j0; j < j1; ++j) {
                         if (j == i) continue;
@@ -687,22 +746,20 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
                         const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
                         const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                         if (!(r2 <= h2)) continue;
-                        force_pair(f, ai, vi, ci2, aj, sV[j], c2[j], sp);
+                        force_pair(f, ai, vi, ci2, aj, sV[j], c2[j], fc);
                     }
                 });
             }
         } else {
             Cursor cur{};
-            NbWalk w = nb_begin(0u);
-            if (fluid) {
-                cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
-                w = nb_begin(P.nbs[i]);
-            }
+            if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+            NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur);
+            const ForceConst fc = force_const(sp);
             for (;;) {
                 uint32_t j = 0;
-                const bool have = nb_next(w, P, g, cur, i, ai, h2, stride, j);
-                if (!__any(have)) break;
-                if (have) force_pair(f, ai, vi, ci2, sA[j], sV[j], c2[j], sp);
+                const int st = nb_step(w, P, g, cur, i, ai, h2, stride, j);
+                if (!__any(st != kWalkDone)) break;
+                if (st == kWalkPair) force_pair(f, ai, vi, ci2, *at(sA, j), *at(sV, j), *at(c2, j), fc);
             }
         }
         float ox = ai.x, oy = ai.y, oz = ai.z;

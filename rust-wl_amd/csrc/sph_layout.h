@@ -20,6 +20,12 @@ namespace vxl {
 
 constexpr uint32_t kBlock = 256;        // slots per block (4 wave64)
 constexpr uint32_t kGridDimMax = 1u << 14;  // core/grid.rs:8
+// Slots of one pool: every per-slot array is addressed with 32-bit byte
+// offsets (16-byte records), so a pool holds at most 2^28 slots.
+constexpr uint32_t kMaxPoolSlots = 1u << 28;
+// Extra records after the last slot of the grid-ordered arrays: neighbour
+// candidates are loaded in groups of 4 without clamping to the range end.
+constexpr uint32_t kReadPad = 4;
 
 // Host-computed constants of one scene (FluidSystem::new, fluid_access.rs:8-13,
 // plus the substep constants of fluid_step.rs:110-113). Values are computed

@@ -202,13 +202,11 @@ VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t) {
     if (!e || !t) return fail(VXL_SPH_NULL_POINTER, "null argument");
     VxlSphStatus st = check_header(t->hdr, sizeof(VxlSphTuning));
     if (st != VXL_SPH_OK) return st;
-    if (t->force_kernel > VXL_SPH_FORCE_WALK || t->density_kernel > VXL_SPH_DENSITY_SPLIT)
-        return fail(VXL_SPH_INVALID_ARGUMENT, "unknown kernel variant");
+    if (t->force_kernel > VXL_SPH_FORCE_WALK) return fail(VXL_SPH_INVALID_ARGUMENT, "unknown kernel variant");
     return guard([&] {
         Engine::Tuning tu;
         tu.persist_blocks = t->persist_blocks;
         tu.force_kernel = t->force_kernel;
-        tu.density_kernel = t->density_kernel;
         e->eng.set_tuning(tu);
     });
 }
@@ -221,7 +219,6 @@ VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out) {
     out->hdr.version = VXL_SPH_ABI_VERSION;
     out->persist_blocks = tu.persist_blocks;
     out->force_kernel = tu.force_kernel;
-    out->density_kernel = tu.density_kernel;
     return VXL_SPH_OK;
 }
 

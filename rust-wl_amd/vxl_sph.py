@@ -20,9 +20,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvxl_sph.so")
 ABI_VERSION = 1
-# Kernel variants of VxlSphTuning (vxl_sph.h VXL_SPH_FORCE_* / VXL_SPH_DENSITY_*).
+# Force kernel variants of VxlSphTuning (vxl_sph.h VXL_SPH_FORCE_*).
 FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK = 0, 1, 2
-DENSITY_DEFAULT, DENSITY_FUSED, DENSITY_SPLIT = 0, 1, 2
 
 STATUS = {
     0: "OK", 1: "NULL_POINTER", 2: "VERSION_MISMATCH", 3: "SIZE_MISMATCH", 4: "NOT_READY",
@@ -67,7 +66,7 @@ class FluidConfig(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("hdr", Header), ("persist_blocks", C.c_uint32), ("force_kernel", C.c_uint32),
-                ("density_kernel", C.c_uint32), ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32 * 2)]
 
 
 class KernelTime(C.Structure):
@@ -433,15 +432,15 @@ class Engine:
         _check(lib().vxl_sph_engine_device_bytes(self.h, C.byref(v)))
         return v.value
 
-    def set_tuning(self, persist_blocks=0, force_kernel=FORCE_DEFAULT, density_kernel=DENSITY_DEFAULT):
-        """force_kernel: FORCE_SCAN | FORCE_WALK; density_kernel: DENSITY_FUSED |
-        DENSITY_SPLIT (0 = default). Every combination gives identical results."""
+    def set_tuning(self, persist_blocks=0, force_kernel=FORCE_DEFAULT):
+        """force_kernel: FORCE_SCAN | FORCE_WALK (0 = default); persist_blocks:
+        grid size cap of the per-slot kernels (0 = one block per 256 slots).
+        Every choice gives identical results."""
         t = Tuning()
         t.hdr.size = C.sizeof(Tuning)
         t.hdr.version = ABI_VERSION
         t.persist_blocks = persist_blocks
         t.force_kernel = force_kernel
-        t.density_kernel = density_kernel
         _check(lib().vxl_sph_engine_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, substeps=2):
@@ -449,12 +448,12 @@ class Engine:
         substeps x variants substeps; results are identical for every variant)."""
         t = Tuning()
         _check(lib().vxl_sph_engine_autotune(self.h, substeps, C.byref(t)))
-        return dict(persist_blocks=t.persist_blocks, force_kernel=t.force_kernel, density_kernel=t.density_kernel)
+        return dict(persist_blocks=t.persist_blocks, force_kernel=t.force_kernel)
 
     def get_tuning(self):
         t = Tuning()
         _check(lib().vxl_sph_engine_get_tuning(self.h, C.byref(t)))
-        return dict(persist_blocks=t.persist_blocks, force_kernel=t.force_kernel, density_kernel=t.density_kernel)
+        return dict(persist_blocks=t.persist_blocks, force_kernel=t.force_kernel)
 
     def profile(self, substeps):
         """{kernel name: total ms} over `substeps` eager substeps (HIP events)."""

@@ -261,18 +261,16 @@ def test_reference_akinci_canary_on_gpu():
 
 
 @pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK])
-@pytest.mark.parametrize("density_kernel", [V.DENSITY_FUSED, V.DENSITY_SPLIT])
 @pytest.mark.parametrize("persist", [0, 7])
-def test_kernel_variants_bit_exact(force_kernel, density_kernel, persist):
+def test_kernel_variants_bit_exact(force_kernel, persist):
     """Every autotuner choice (force: candidate scan or walk of the recorded
-    neighbours; density: fused or split test/sum passes; one block per 256
-    slots or a grid-stride loop over a few blocks) gives the oracle's state
-    bit for bit."""
+    neighbours; one block per 256 slots or a grid-stride loop over a few
+    blocks) gives the oracle's state bit for bit."""
     cfg, data, tank, d = small_scene(seed=13, block=(16, 12, 10), tank=(0.6, 0.5, 0.4))
     orc = oracle_dam_break(cfg, data, tank, d)
     eng = V.Engine(diagnostics=True)
     sid = eng.add(cfg, data, reactions=False)
-    eng.set_tuning(persist_blocks=persist, force_kernel=force_kernel, density_kernel=density_kernel)
+    eng.set_tuning(persist_blocks=persist, force_kernel=force_kernel)
     for _ in range(6):
         eng.step(1)
         orc.step(cfg.dt_tick)
