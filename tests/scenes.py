@@ -56,3 +56,46 @@ def max_rel(a, b, floor):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor))) if a.size else 0.0
+
+
+# --- golden fixtures (tests/golden/make_golden.py) ---------------------------
+
+GOLDEN_DIR = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)),
+                                        "golden")
+GOLDEN_SCENES = ("dam_512", "dam_4096")
+
+
+def load_golden(name):
+    """(meta dict, arrays dict) of one fixture; no pickles."""
+    import json
+    import os
+
+    with np.load(os.path.join(GOLDEN_DIR, name + ".npz"), allow_pickle=False) as z:
+        arrs = {k: z[k] for k in z.files}
+    return json.loads(str(arrs.pop("meta"))), arrs
+
+
+def golden_spans(arrs):
+    return [(int(b), tuple(float(x) for x in o), int(r[0]), int(r[1]))
+            for b, o, r in zip(arrs["in_span_body"], arrs["in_span_origin"], arrs["in_span_range"])]
+
+
+def fnv_words(words, h=0xcbf29ce484222325):
+    for v in np.asarray(words, np.uint32).tolist():
+        h = ((h ^ v) * 0x00000100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def grid_table_hash(start, items):
+    """FNV-1a-64 over start, 0xffffffff, items, 0xffffffff, per-particle cell
+    (the table hash of crates/vxl-phys-gpu/examples/gpu_grid_probe.rs:50-68)."""
+    start = np.asarray(start, np.int64)
+    items = np.asarray(items, np.uint32)
+    bins = np.full(len(items), 0xFFFFFFFF, np.uint32)
+    counts = np.diff(start)
+    bins[items] = np.repeat(np.arange(len(counts), dtype=np.uint32), counts)
+    h = fnv_words(start.astype(np.uint32))
+    h = fnv_words([0xFFFFFFFF], h)
+    h = fnv_words(items, h)
+    h = fnv_words([0xFFFFFFFF], h)
+    return fnv_words(bins, h)
