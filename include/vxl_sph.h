@@ -212,6 +212,43 @@ VxlSphStatus vxl_sph_box_particles(const VxlSphFluidConfig *cfg, uint32_t nbodie
 /* FluidSystem::new constants (fluid_access.rs:8-42): h, h2, k6, ks, w0, b_tait, mass, alpha_c. */
 VxlSphStatus vxl_sph_fluid_constants(const VxlSphFluidConfig *cfg, float out[8]);
 
+/* ---- coupled scenes: fluid + Box bodies, dynamic bodies advanced on the device ---- */
+/* The rigid leg of World::step for dynamic Box bodies
+ * (crates/vxl-phys/src/world_step.rs:13-123, crates/vxl-phys-integrate/src/lib.rs:18-64):
+ * per tick, `substeps` rigid substeps driven by gravity and the previous fluid
+ * tick's averaged reaction, then the bodies' Akinci particles are rebuilt from
+ * the new pose and the fluid advances one tick. Defaults are PhysConfig's
+ * (crates/vxl-phys-core/src/config.rs:206-233). `tank` holds the interior
+ * planes of an open tank [0,L0] x [0,inf) x [0,L2] that dynamic boxes cannot
+ * cross: a contact rule of this library (the reference has no contact between
+ * a box and a particle-only tank); L0 <= 0 disables it. */
+typedef struct VxlSphRigidConfig {
+    VxlSphHeader hdr;
+    uint32_t substeps;
+    float gravity[3];
+    float max_linear;
+    float max_angular;
+    float tank[3];
+    uint32_t reserved;
+} VxlSphRigidConfig;
+VxlSphStatus vxl_sph_default_rigid_config(VxlSphRigidConfig *out);
+/* A Box body of a coupled scene: density > 0 makes it dynamic (mass properties
+ * of crates/vxl-phys-core/src/mass.rs:35-48), otherwise it is static. */
+typedef struct VxlSphRigidBox {
+    VxlSphBoxBody box;
+    float density;
+} VxlSphRigidBox;
+/* Fluid particles (n_fluid x 3 floats; fluid_vel may be NULL) followed by the
+ * bodies' Akinci particles in body order (set_boundary_particles,
+ * fluid_step.rs:13-53). Body ids must be unique; reactions are computed. */
+VxlSphStatus vxl_sph_scene_add_coupled(VxlSphEngine *e, const VxlSphFluidConfig *cfg, const VxlSphRigidConfig *rc,
+                                       uint32_t n_fluid, const float *fluid_pos, const float *fluid_vel,
+                                       uint32_t nbodies, const VxlSphRigidBox *bodies, uint32_t *out_scene);
+/* Pose and velocities of a coupled scene's dynamic bodies, in creation order
+ * (call with out == NULL to query the count). */
+VxlSphStatus vxl_sph_scene_read_bodies(VxlSphEngine *e, uint32_t scene, uint32_t cap, VxlSphRigidBox *out,
+                                       uint32_t *count);
+
 /* Seeded dam-break scene (tank [0,L]^3 open at the top, fluid block in the
  * corner, 5 Akinci Box bodies of thickness 2d). Call with pos == NULL to
  * query n / n_fluid / nspans, then again with buffers of those sizes. */
@@ -231,8 +268,8 @@ VxlSphStatus vxl_sph_dam_break(const VxlSphDamBreak *spec, const VxlSphFluidConf
 
 /* ---- batched and served forms ---- */
 /* A sampled dam-break request: scene geometry and physics, requested steps,
- * arrival time (served form) and dynamic boxes (served form; not supported by
- * vxl_sph_serve yet: such requests are rejected with VXL_SPH_UNSUPPORTED). */
+ * arrival time (served form) and dynamic boxes (served form: a request with
+ * boxes is served as a coupled scene, see vxl_sph_scene_add_coupled). */
 typedef struct VxlSphRequest {
     VxlSphDamBreak dam;
     float sound_speed;
@@ -256,6 +293,11 @@ VxlSphStatus vxl_sph_sample_requests(uint64_t seed, uint32_t count, double rate,
 /* The fluid configuration a request's scene is stepped with
  * (h = 2d, mass = rho0 d^3, gamma 7, eps 0.5, one substep of dt). */
 VxlSphStatus vxl_sph_request_config(const VxlSphRequest *req, float dt, uint64_t max_bins, VxlSphFluidConfig *out);
+/* The Box bodies a request with boxes is served with (the tank's five static
+ * bodies, then the dynamic boxes) and its rigid configuration; count = 0 for a
+ * request without boxes. Call with out == NULL to query the count. */
+VxlSphStatus vxl_sph_request_bodies(const VxlSphRequest *req, uint32_t cap, VxlSphRigidBox *out, uint32_t *count,
+                                    VxlSphRigidConfig *rc);
 
 typedef struct VxlSphServeConfig {
     VxlSphHeader hdr;

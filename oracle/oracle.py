@@ -42,6 +42,29 @@ class OrBoxBody(C.Structure):
     ]
 
 
+class OrRigidBody(C.Structure):
+    _fields_ = [
+        ("half", C.c_float * 3),
+        ("pos", C.c_float * 3),
+        ("rot", C.c_float * 4),
+        ("linvel", C.c_float * 3),
+        ("angvel", C.c_float * 3),
+        ("inv_mass", C.c_float),
+        ("inv_inertia", C.c_float * 3),
+    ]
+
+
+class OrRigidConfig(C.Structure):
+    _fields_ = [
+        ("dt_tick", C.c_float),
+        ("substeps", C.c_uint32),
+        ("gravity", C.c_float * 3),
+        ("max_linear", C.c_float),
+        ("max_angular", C.c_float),
+        ("tank", C.c_float * 3),
+    ]
+
+
 _lib = None
 
 
@@ -101,6 +124,10 @@ def lib():
         L.or_boundary_self_sum_grid.argtypes = [fp, C.c_uint32, C.c_float]
         L.or_neighbors.restype = C.c_uint32
         L.or_neighbors.argtypes = [P, C.c_uint32, up, C.c_uint32]
+        L.or_box_mass_props.restype = None
+        L.or_box_mass_props.argtypes = [fp, C.c_float, fp, fp]
+        L.or_rigid_tick.restype = None
+        L.or_rigid_tick.argtypes = [C.POINTER(OrRigidConfig), C.c_uint32, C.POINTER(OrRigidBody), fp]
         _lib = L
     return _lib
 
@@ -303,3 +330,57 @@ class System:
         o = np.zeros(cap, np.uint32)
         n = lib().or_neighbors(self.p, i, _u(o), cap)
         return o[:min(n, cap)]
+
+
+# --- rigid-box leg (rigid_oracle.h) ------------------------------------------
+
+def box_mass_props(half, density):
+    """(inv_mass, inv_inertia[3]) of a box (mass.rs:35-48)."""
+    half = np.asarray(half, np.float32)
+    im = np.zeros(1, np.float32)
+    ii = np.zeros(3, np.float32)
+    lib().or_box_mass_props(_f(half), C.c_float(density), _f(im), _f(ii))
+    return im[0], ii
+
+
+def rigid_config(dt_tick, substeps=2, gravity=(0.0, -9.81, 0.0), max_linear=100.0, max_angular=50.0,
+                 tank=(0.0, 0.0, 0.0)):
+    c = OrRigidConfig()
+    c.dt_tick = dt_tick
+    c.substeps = substeps
+    c.gravity = (C.c_float * 3)(*gravity)
+    c.max_linear = max_linear
+    c.max_angular = max_angular
+    c.tank = (C.c_float * 3)(*tank)
+    return c
+
+
+class RigidBoxes:
+    """Dynamic boxes advanced by or_rigid_tick; `bodies` are dicts with half,
+    pos, rot, linvel, angvel, density."""
+
+    def __init__(self, bodies):
+        self.arr = (OrRigidBody * max(len(bodies), 1))()
+        self.n = len(bodies)
+        for k, b in enumerate(bodies):
+            r = self.arr[k]
+            r.half = (C.c_float * 3)(*b["half"])
+            r.pos = (C.c_float * 3)(*b["pos"])
+            r.rot = (C.c_float * 4)(*b.get("rot", (0.0, 0.0, 0.0, 1.0)))
+            r.linvel = (C.c_float * 3)(*b.get("linvel", (0.0, 0.0, 0.0)))
+            r.angvel = (C.c_float * 3)(*b.get("angvel", (0.0, 0.0, 0.0)))
+            im, ii = box_mass_props(b["half"], b["density"])
+            r.inv_mass = im
+            r.inv_inertia = (C.c_float * 3)(*ii)
+
+    def tick(self, cfg, react=None):
+        """react: (n, 6) float32 (F, tau) per body, or None for zeros."""
+        r = np.zeros((max(self.n, 1), 6), np.float32)
+        if react is not None and self.n:
+            r[:self.n] = np.asarray(react, np.float32).reshape(self.n, 6)
+        lib().or_rigid_tick(C.byref(cfg), self.n, self.arr, _f(r))
+
+    def state(self, k):
+        b = self.arr[k]
+        return dict(pos=np.array(b.pos[:], np.float32), rot=np.array(b.rot[:], np.float32),
+                    linvel=np.array(b.linvel[:], np.float32), angvel=np.array(b.angvel[:], np.float32))

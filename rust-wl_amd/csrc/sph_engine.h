@@ -38,6 +38,26 @@ struct SceneSlot {
     // reactions of the last tick (tick average), per span: F[3], tau[3]
     std::vector<float> reactions;
     bool want_reactions = false;
+    // dynamic rigid bodies: device record (state as of the last layout or
+    // readback), body-frame lattice (3 floats per particle), local span index
+    struct Rigid {
+        RigidDev dev{};
+        std::vector<float> lat;
+        uint32_t span_local = 0;
+        uint32_t body_id = 0;
+        float density = 0.0f;
+    };
+    std::vector<Rigid> rigid;
+};
+
+// Rigid configuration of a coupled scene (PhysConfig defaults,
+// crates/vxl-phys-core/src/config.rs:206-233) plus the tank contact planes.
+struct RigidConfig {
+    uint32_t substeps = 2;
+    float gravity[3] = {0.0f, -9.81f, 0.0f};
+    float max_linear = 100.0f;
+    float max_angular = 50.0f;
+    float tank[3] = {0.0f, 0.0f, 0.0f};  // open tank [0,L0] x [0,inf) x [0,L2]; L0 <= 0: no contacts
 };
 
 class Engine {
@@ -51,6 +71,15 @@ public:
     uint32_t add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos,
                        const float *vel, const float *pmass, const Span *spans, uint32_t nspans,
                        bool reactions = true);
+    // A scene whose boundary is sampled from Box bodies (set_boundary_particles,
+    // fluid_step.rs:13-53): bodies with density > 0 are dynamic and coupled
+    // both ways (advanced each tick by gravity and the fluid reaction, their
+    // particles rebuilt from the pose); the others are static.
+    uint32_t add_coupled_scene(const SceneConfig &cfg, uint32_t nf, const float *fpos, const float *fvel,
+                               const std::vector<BoxBody> &bodies, const std::vector<float> &density,
+                               const RigidConfig &rc);
+    // Current state of a scene's dynamic bodies (in the order given at creation).
+    std::vector<RigidDev> read_bodies(uint32_t id);
     void remove_scene(uint32_t id);
     bool has_scene(uint32_t id) const;
     const SceneSlot &scene(uint32_t id) const;
@@ -108,9 +137,15 @@ public:
 private:
     void layout();            // assign slot ranges, (re)allocate, move + upload
     uint32_t box_reduce_grid() const;
+    uint32_t pool_substeps() const;  // substeps per tick (shared by every scene of the pool)
     // The launch sequence of one substep; `mark` (if set) is called after each
     // kernel launch with the kernel's name.
     void enqueue_substep(const std::function<void(const char *)> *mark = nullptr);
+    // Substep i (counted from the last layout) with the rigid leg's launches at
+    // the tick's start and end when the pool has dynamic bodies.
+    // `head` = false skips the tick's rigid leg (a replay of a substep whose
+    // tick already ran it).
+    void enqueue_at(uint64_t i, const std::function<void(const char *)> *mark = nullptr, bool head = true);
     void build_graph();
     void invalidate_graph();
     void check_error_state();  // read the control block; grow tables on demand
@@ -145,6 +180,8 @@ private:
     uint32_t *d_tile_sums_ = nullptr;
     SpanDev *d_spans_ = nullptr;
     float *d_react_ = nullptr;
+    RigidDev *d_rigid_ = nullptr;
+    float4 *d_rlat_ = nullptr;
     uint32_t nspans_ = 0;
     std::vector<uint32_t> span_first_;  // first global span of each scene (by scene position)
     // spare state arrays: resident scenes move into them at a layout change

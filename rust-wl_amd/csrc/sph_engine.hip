@@ -87,8 +87,12 @@ void Engine::free_all() {
     dfree(d_tile_sums_);
     dfree(d_spans_);
     dfree(d_react_);
+    dfree(d_rigid_);
+    dfree(d_rlat_);
     d_spans_ = nullptr;
     d_react_ = nullptr;
+    d_rigid_ = nullptr;
+    d_rlat_ = nullptr;
     nspans_ = 0;
     d_params_ = nullptr;
     d_grid_ = nullptr;
@@ -158,6 +162,80 @@ uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, cons
     scenes_.push_back(std::move(sc));
     dirty_ = true;
     return scenes_.back().id;
+}
+
+uint32_t Engine::add_coupled_scene(const SceneConfig &cfg, uint32_t nf, const float *fpos, const float *fvel,
+                                   const std::vector<BoxBody> &bodies, const std::vector<float> &density,
+                                   const RigidConfig &rc) {
+    if (density.size() != bodies.size()) throw std::invalid_argument("one density per body");
+    std::vector<float> pos(fpos, fpos + 3ull * nf), vel, pmass(nf, 0.0f);
+    if (fvel)
+        vel.assign(fvel, fvel + 3ull * nf);
+    else
+        vel.assign(3ull * nf, 0.0f);
+    const SceneParams probe = scene_params(cfg, nf ? nf : 1, nf);
+    for (uint32_t i = 0; i < nf; ++i) pmass[i] = probe.mass;  // FluidSystem::mass (explicit or calibrated)
+    std::vector<Span> spans;
+    append_boundary_boxes(cfg, bodies, pos, vel, pmass, spans);
+    std::vector<SceneSlot::Rigid> rigid;
+    const float h = fmaxf(cfg.smoothing_radius, 1e-5f);
+    for (size_t b = 0; b < bodies.size(); ++b) {
+        if (!(density[b] > 0.0f)) continue;
+        const BoxBody &bd = bodies[b];
+        // append_boundary_boxes emits one span per body with particles, in order.
+        uint32_t span_local = 0;
+        bool found = false;
+        for (uint32_t k = 0; k < spans.size(); ++k)
+            if (spans[k].body == bd.id) {
+                if (found) throw std::invalid_argument("body ids must be unique");
+                span_local = k;
+                found = true;
+            }
+        if (!found) continue;  // no particles: nothing couples
+        SceneSlot::Rigid r;
+        r.span_local = span_local;
+        r.body_id = bd.id;
+        r.density = density[b];
+        uint32_t n0 = 0;
+        float vol = 0.0f;
+        box_lattice(bd.half, cfg.spacing, h, cfg.boundary_layers, r.lat, n0, vol);
+        RigidDev &d = r.dev;
+        for (int a = 0; a < 3; ++a) {
+            d.pos[a] = bd.pos[a];
+            d.linvel[a] = bd.linvel[a];
+            d.angvel[a] = bd.angvel[a];
+            d.half[a] = bd.half[a];
+            d.gravity[a] = rc.gravity[a];
+            d.tank[a] = rc.tank[a];
+        }
+        for (int a = 0; a < 4; ++a) d.rot[a] = bd.rot[a];
+        box_mass_props(bd.half, density[b], d.inv_mass, d.inv_inertia);
+        d.dt_tick = cfg.dt_tick;
+        d.max_linear = rc.max_linear;
+        d.max_angular = rc.max_angular;
+        d.substeps = std::max<uint32_t>(rc.substeps, 1u);
+        d.gstart = spans[span_local].start;
+        d.gend = spans[span_local].end;
+        if ((size_t)(d.gend - d.gstart) * 3 != r.lat.size()) throw std::logic_error("body lattice mismatch");
+        rigid.push_back(std::move(r));
+    }
+    const uint32_t n = (uint32_t)pmass.size();
+    const uint32_t id = add_scene(cfg, n, nf, pos.data(), vel.data(), pmass.data(), spans.data(),
+                                  (uint32_t)spans.size(), true);
+    scenes_.back().rigid = std::move(rigid);
+    return id;
+}
+
+std::vector<RigidDev> Engine::read_bodies(uint32_t id) {
+    if (dirty_) layout();
+    sync();
+    const SceneSlot &sc = scene(id);
+    std::vector<RigidDev> out(sc.rigid.size());
+    if (!out.empty())
+        hip_check(hipMemcpy(out.data(), d_rigid_ + sc.params.rbody0, sizeof(RigidDev) * out.size(),
+                            hipMemcpyDeviceToHost),
+                  "read bodies");
+    return out;
 }
 
 void Engine::remove_scene(uint32_t id) {
@@ -250,7 +328,20 @@ void *Engine::staging(size_t bytes) {
 void Engine::layout() {
     hip_check(hipSetDevice(device_), "hipSetDevice");
     hip_check(hipStreamSynchronize(stream_), "sync before layout");
+    // Finish (replay) any abandoned substep of the old layout first: the reset
+    // of the control block below would otherwise drop it.
+    if (P_.nblocks) check_error_state();
     invalidate_graph();
+    // Dynamic bodies of resident scenes continue from their device state.
+    if (P_.nrigid) {
+        std::vector<RigidDev> cur(P_.nrigid);
+        hip_check(hipMemcpy(cur.data(), d_rigid_, sizeof(RigidDev) * P_.nrigid, hipMemcpyDeviceToHost),
+                  "rigid readback");
+        for (auto &sc : scenes_) {
+            if (!sc.resident) continue;
+            for (size_t k = 0; k < sc.rigid.size(); ++k) sc.rigid[k].dev = cur[sc.params.rbody0 + k];
+        }
+    }
     const std::vector<uint32_t> old_base = [&] {
         std::vector<uint32_t> v;
         for (auto &sc : scenes_) v.push_back(sc.dev_base);
@@ -366,7 +457,39 @@ void Engine::layout() {
             for (int a = 0; a < 3; ++a) d.origin[a] = s.origin[a];
             spans.push_back(d);
         }
+        // A dynamic body's torque origin is its current position.
+        for (const auto &r : scenes_[k].rigid)
+            for (int a = 0; a < 3; ++a) spans[span_first_[k] + r.span_local].origin[a] = r.dev.pos[a];
     }
+    // Rigid table (scene order) and the bodies' body-frame lattices.
+    std::vector<RigidDev> rig;
+    std::vector<float4> lat;
+    for (uint32_t k = 0; k < scenes_.size(); ++k) {
+        SceneSlot &sc = scenes_[k];
+        sc.params.rbody0 = (uint32_t)rig.size();
+        sc.params.nrbody = (uint32_t)sc.rigid.size();
+        for (auto &r : sc.rigid) {
+            r.dev.scene = k;
+            r.dev.span = span_first_[k] + r.span_local;
+            r.dev.lat = (uint32_t)lat.size();
+            rig.push_back(r.dev);
+            for (size_t q = 0; q + 2 < r.lat.size(); q += 3)
+                lat.push_back(make_float4(r.lat[q], r.lat[q + 1], r.lat[q + 2], 0.0f));
+        }
+    }
+    dfree(d_rigid_);
+    dfree(d_rlat_);
+    d_rigid_ = nullptr;
+    d_rlat_ = nullptr;
+    if (!rig.empty()) {
+        d_rigid_ = dalloc<RigidDev>(rig.size());
+        hip_check(hipMemcpy(d_rigid_, rig.data(), sizeof(RigidDev) * rig.size(), hipMemcpyHostToDevice), "rigid");
+        d_rlat_ = dalloc<float4>(lat.size());
+        hip_check(hipMemcpy(d_rlat_, lat.data(), sizeof(float4) * lat.size(), hipMemcpyHostToDevice), "lattice");
+    }
+    P_.rigid = d_rigid_;
+    P_.nrigid = (uint32_t)rig.size();
+    P_.rlat = d_rlat_;
     nspans_ = (uint32_t)spans.size();
     if (nspans_) {
         d_spans_ = dalloc<SpanDev>(nspans_);
@@ -409,6 +532,7 @@ void Engine::layout() {
         hip_check(hipGetLastError(), "k_box");
     }
     hip_check(hipStreamSynchronize(stream_), "layout sync");
+    substeps_done_ = 0;  // substep index of the new layout (tick phase, replay)
     ++layouts_;
     dirty_ = false;
 }
@@ -452,6 +576,33 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     hip_check(hipGetLastError(), "substep launch");
 }
 
+void Engine::enqueue_at(uint64_t i, const std::function<void(const char *)> *mark, bool head) {
+    const uint32_t sub = pool_substeps();
+    auto M = [&](const char *name) {
+        if (mark) (*mark)(name);
+    };
+    if (P_.nrigid && head && i % sub == 0) {
+        const uint32_t gb = (P_.nrigid + 63) / 64;
+        hipLaunchKernelGGL(k_rigid, dim3(gb), dim3(64), 0, stream_, P_);
+        M("rigid");
+        hipLaunchKernelGGL(k_body_refresh, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
+        M("body_refresh");
+    }
+    enqueue_substep(mark);
+    if (P_.nrigid && i % sub == sub - 1) {
+        hipLaunchKernelGGL(k_rigid_collect, dim3((P_.nrigid + 63) / 64), dim3(64), 0, stream_, P_);
+        M("rigid");
+    }
+    hip_check(hipGetLastError(), "tick launch");
+}
+
+uint32_t Engine::pool_substeps() const {
+    uint32_t sub = 1;
+    for (auto &s : scenes_) sub = std::max(sub, s.substeps);
+    return sub;
+}
+
 std::vector<std::pair<std::string, float>> Engine::profile(uint32_t substeps) {
     sync();
     if (scenes_.empty()) return {};
@@ -470,7 +621,7 @@ std::vector<std::pair<std::string, float>> Engine::profile(uint32_t substeps) {
     for (uint32_t k = 0; k < substeps; ++k) {
         ev();
         names.push_back("");
-        enqueue_substep(&mark);
+        enqueue_at(substeps_done_ + k, &mark);
     }
     substeps_done_ += substeps;
     hip_check(hipStreamSynchronize(stream_), "profile sync");
@@ -564,10 +715,11 @@ void Engine::set_tuning(const Tuning &t) {
     invalidate_graph();
 }
 
+// One whole tick (rigid leg, substeps) as a graph; launched at tick starts.
 void Engine::build_graph() {
     invalidate_graph();
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-    enqueue_substep();
+    for (uint32_t i = 0; i < pool_substeps(); ++i) enqueue_at(i);
     hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
     hip_check(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0), "graph instantiate");
 }
@@ -577,20 +729,23 @@ void Engine::step(uint32_t ticks) {
     if (scenes_.empty() || P_.nblocks == 0) return;
     hip_check(hipSetDevice(device_), "hipSetDevice");
     // All scenes share one substep launch; a tick is `substeps` of them.
-    uint32_t sub = 1;
-    for (auto &s : scenes_) sub = std::max(sub, s.substeps);
+    const uint32_t sub = pool_substeps();
     for (auto &s : scenes_)
         if (s.substeps != sub)
             throw std::invalid_argument("scenes of one pool must share the substep count");
-    if (use_graph_ && !gexec_) build_graph();
-    const uint64_t n = (uint64_t)ticks * sub;
-    for (uint64_t k = 0; k < n; ++k) {
-        if (use_graph_)
-            hip_check(hipGraphLaunch(gexec_, stream_), "graph launch");
-        else
-            enqueue_substep();
+    uint64_t n = (uint64_t)ticks * sub;
+    // A tick left open by profile() is finished launch by launch.
+    while (n && substeps_done_ % sub) {
+        enqueue_at(substeps_done_++);
+        --n;
     }
-    substeps_done_ += n;
+    if (use_graph_) {
+        if (!gexec_) build_graph();
+        for (uint64_t t = 0; t < n / sub; ++t) hip_check(hipGraphLaunch(gexec_, stream_), "graph launch");
+        substeps_done_ += n;
+    } else {
+        for (uint64_t k = 0; k < n; ++k) enqueue_at(substeps_done_++);
+    }
 }
 
 void Engine::check_error_state() {
@@ -617,19 +772,12 @@ void Engine::check_error_state() {
         alloc_cells(new_cap);
         h_ctl_->err = 0;
         hip_check(hipMemcpy(&d_ctl_->err, &h_ctl_->err, sizeof(uint32_t), hipMemcpyHostToDevice), "ctl reset");
+        // Replay launch by launch. The rigid leg of the tick holding the failed
+        // substep already ran (it precedes the failure); later ticks' legs and
+        // the failed tick's closing collect returned early and run again.
         substeps_done_ = done;
-        if (missing) {
-            uint32_t sub = 1;
-            for (auto &s : scenes_) sub = std::max(sub, s.substeps);
-            if (use_graph_) build_graph();
-            for (uint64_t k = 0; k < missing; ++k) {
-                if (use_graph_)
-                    hip_check(hipGraphLaunch(gexec_, stream_), "graph launch");
-                else
-                    enqueue_substep();
-            }
-            substeps_done_ += missing;
-        }
+        for (uint64_t k = 0; k < missing; ++k) enqueue_at(done + k, nullptr, k != 0);
+        substeps_done_ += missing;
         check_error_state();
     }
 }

@@ -50,8 +50,12 @@ struct Pool {
     uint32_t nspans;   // reaction spans (0 = no reaction pass)
     // reactions (index order, pool slots): boundary force and position
     float4 *bf, *bp;
-    const SpanDev *spans;
+    SpanDev *spans;    // torque origins of dynamic bodies follow their pose
     float *react;      // [nspans][6]: F, tau (tick accumulator)
+    // dynamic rigid bodies (all scenes) and their body-frame Akinci lattices
+    RigidDev *rigid;
+    uint32_t nrigid;
+    const float4 *rlat;
 };
 
 __global__ void k_box(Pool P);
@@ -69,6 +73,12 @@ template <uint32_t WALK>
 __global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks
 __global__ void k_bforce(Pool P);
 __global__ void k_reactions(Pool P);
+// Rigid leg, once per tick: integrate the bodies (before the first substep),
+// rebuild their particles from the pose and the scenes' AABBs, keep the
+// tick's reaction for the next tick (after the last substep).
+__global__ void k_rigid(Pool P);
+__global__ void k_body_refresh(Pool P);
+__global__ void k_rigid_collect(Pool P);
 __global__ void k_unsort(Pool P, float *opos, float *ovel);
 __global__ void k_unsort_fields(Pool P, float *odens, float *opress, uint32_t *oncount, float *oacc, float *oxs);
 

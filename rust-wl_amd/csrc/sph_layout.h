@@ -43,6 +43,34 @@ struct SceneParams {
     float dt, vmax, vmax2;
     uint32_t clamp_neg;  // tensile_instability_suppression
     uint32_t react;      // compute boundary reactions for this scene's spans
+    uint32_t rbody0;     // first dynamic rigid body of the scene (pool rigid table)
+    uint32_t nrbody;     // dynamic rigid bodies of the scene (0: none)
+};
+
+// A dynamic Box body coupled to its scene's fluid (the rigid leg of
+// crates/vxl-phys/src/world_step.rs:13-123): advanced once per tick before
+// the fluid substeps by gravity and the fluid's tick-averaged reaction, then
+// its Akinci particles are rebuilt from the new pose.
+struct RigidDev {
+    float pos[3];
+    float rot[4];            // quaternion x, y, z, w
+    float linvel[3];
+    float angvel[3];
+    float inv_mass;
+    float inv_inertia[3];    // body-frame diagonal (mass.rs:35-48)
+    float half[3];
+    float react[6];          // (F, tau) of the last fluid tick (zero before the first)
+    // rigid configuration of the scene (PhysConfig, config.rs:206-233)
+    float dt_tick;
+    float gravity[3];
+    float max_linear, max_angular;
+    float tank[3];           // tank contact planes; tank[0] <= 0: none
+    uint32_t substeps;
+    uint32_t scene;          // pool scene index
+    uint32_t span;           // pool span index (reaction in, torque origin out)
+    uint32_t gstart, gend;   // the body's particles, scene index order
+    uint32_t lat;            // first body-frame lattice point in the pool lattice table
+    uint32_t pad[2];
 };
 
 // A boundary span of a scene with reactions (fluid_boundary.rs:10-31):

@@ -238,6 +238,19 @@ uint32_t box_lattice(const float half[3], float s_in, float h, uint32_t layers_i
     return n;
 }
 
+void box_mass_props(const float half[3], float density, float &inv_mass, float inv_inertia[3]) {
+    const float rho = density > 0.0f ? density : 1.0f;
+    const float ex = 2.0f * half[0], ey = 2.0f * half[1], ez = 2.0f * half[2];
+    const float m = rho * ex * ey * ez;
+    const float ix = m / 12.0f * (ey * ey + ez * ez);
+    const float iy = m / 12.0f * (ex * ex + ez * ez);
+    const float iz = m / 12.0f * (ex * ex + ey * ey);
+    inv_mass = 1.0f / m;
+    inv_inertia[0] = 1.0f / ix;
+    inv_inertia[1] = 1.0f / iy;
+    inv_inertia[2] = 1.0f / iz;
+}
+
 void append_boundary_boxes(const SceneConfig &cfg, const std::vector<BoxBody> &bodies,
                            std::vector<float> &pos, std::vector<float> &vel, std::vector<float> &pmass,
                            std::vector<Span> &spans) {

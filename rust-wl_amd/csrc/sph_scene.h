@@ -68,6 +68,10 @@ struct BoxBody {
     float angvel[3] = {0.0f, 0.0f, 0.0f};
 };
 
+// mass_props for Shape::Box (crates/vxl-phys-core/src/mass.rs:35-48):
+// m = density ex ey ez, I = m/12 (e_j^2 + e_k^2); density <= 0 falls back to 1.
+void box_mass_props(const float half[3], float density, float &inv_mass, float inv_inertia[3]);
+
 // Append the boundary particles of `bodies` after the fluid prefix
 // (body order x layer x face order), pmass = rho0 * V_b, spans per body.
 void append_boundary_boxes(const SceneConfig &cfg, const std::vector<BoxBody> &bodies,

@@ -48,14 +48,29 @@ struct Built {
     std::vector<Span> spans;
     uint32_t nf = 0;
     uint32_t cells_hint = 0;
+    // coupled requests (dynamic boxes)
+    std::vector<BoxBody> bodies;
+    std::vector<float> density;
+    RigidConfig rc;
 };
 
 }  // namespace
 
+void request_bodies(const SceneSample &req, std::vector<BoxBody> &bodies, std::vector<float> &density,
+                    RigidConfig &rc) {
+    bodies = req.dam.walls ? tank_bodies(req.dam.tank, req.dam.d) : std::vector<BoxBody>{};
+    density.assign(bodies.size(), 0.0f);
+    for (uint32_t b = 0; b < req.nboxes && b < 4; ++b) {
+        bodies.push_back(req.boxes[b]);
+        density.push_back(req.box_density[b]);
+    }
+    rc = RigidConfig{};
+    for (int a = 0; a < 3; ++a) rc.tank[a] = req.dam.tank[a];
+}
+
 void build_request(const SceneSample &req, const ServeConfig &cfg, SceneConfig &sc, std::vector<float> &pos,
-                   std::vector<float> &vel, std::vector<float> &pmass, std::vector<Span> &spans, uint32_t &nf) {
-    if (req.nboxes)
-        throw std::domain_error("requests with dynamic rigid boxes are not supported by the served path");
+                   std::vector<float> &vel, std::vector<float> &pmass, std::vector<Span> &spans, uint32_t &nf,
+                   std::vector<BoxBody> *bodies, std::vector<float> *density, RigidConfig *rc) {
     const float d = req.dam.d;
     sc = SceneConfig{};
     sc.smoothing_radius = 2.0f * d;
@@ -68,7 +83,16 @@ void build_request(const SceneSample &req, const ServeConfig &cfg, SceneConfig &
     sc.spacing = d;
     sc.max_bins = cfg.max_bins;
     sc.dt_tick = cfg.dt;
-    dam_break(req.dam, sc, pos, vel, pmass, spans, nf);
+    if (!req.nboxes) {
+        dam_break(req.dam, sc, pos, vel, pmass, spans, nf);
+        return;
+    }
+    if (!bodies || !density || !rc) throw std::invalid_argument("a request with boxes needs its body list");
+    DamBreakSpec fluid_only = req.dam;
+    fluid_only.walls = false;
+    dam_break(fluid_only, sc, pos, vel, pmass, spans, nf);
+    request_bodies(req, *bodies, *density, *rc);
+    append_boundary_boxes(sc, *bodies, pos, vel, pmass, spans);
 }
 
 ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeConfig &cfg,
@@ -106,7 +130,8 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
             Built b;
             b.index = order[pos_k];
             try {
-                build_request(reqs[b.index], cfg, b.sc, b.pos, b.vel, b.pmass, b.spans, b.nf);
+                build_request(reqs[b.index], cfg, b.sc, b.pos, b.vel, b.pmass, b.spans, b.nf, &b.bodies, &b.density,
+                              &b.rc);
                 float lo[3], hi[3], gmin[3], bin;
                 uint32_t dims[3];
                 for (int a = 0; a < 3; ++a) lo[a] = hi[a] = b.pos[a];
@@ -178,8 +203,11 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
                 const uint64_t need = Engine::footprint(n, b.cells_hint, cfg.max_bins);
                 if (need > cfg.budget_bytes)
                     throw CapacityError("a request alone exceeds the serving HBM budget");
-                const uint32_t sid = eng.add_scene(b.sc, n, b.nf, b.pos.data(), b.vel.data(), b.pmass.data(),
-                                                   b.spans.data(), (uint32_t)b.spans.size(), false);
+                const uint32_t sid =
+                    b.bodies.empty()
+                        ? eng.add_scene(b.sc, n, b.nf, b.pos.data(), b.vel.data(), b.pmass.data(), b.spans.data(),
+                                        (uint32_t)b.spans.size(), false)
+                        : eng.add_coupled_scene(b.sc, b.nf, b.pos.data(), b.vel.data(), b.bodies, b.density, b.rc);
                 const SceneSample &rq = reqs[b.index];
                 resident.push_back(Resident{b.index, sid, std::max<uint32_t>(rq.steps, 1), b.nf, n, need});
                 bytes_in_use += need;

@@ -50,10 +50,20 @@ struct ServeStats {
     uint64_t layouts = 0;
 };
 
-// Builds the scene of one request (dam break + tank; boxes are not
-// supported by the served path yet and are rejected).
+// Builds the scene of one request: the dam-break fluid and, with its tank,
+// all particles (pos/vel/pmass/spans). A request with dynamic boxes is a
+// coupled scene: `bodies`/`density` then hold the tank's static Box bodies
+// followed by the boxes (density > 0), and `rc` its rigid configuration; the
+// particle arrays still hold every particle (for the footprint and the grid
+// hint), the fluid prefix being the first nf.
 void build_request(const SceneSample &req, const ServeConfig &cfg, SceneConfig &sc, std::vector<float> &pos,
-                   std::vector<float> &vel, std::vector<float> &pmass, std::vector<Span> &spans, uint32_t &nf);
+                   std::vector<float> &vel, std::vector<float> &pmass, std::vector<Span> &spans, uint32_t &nf,
+                   std::vector<BoxBody> *bodies = nullptr, std::vector<float> *density = nullptr,
+                   RigidConfig *rc = nullptr);
+// The Box bodies of a request with dynamic boxes (tank, then boxes) and its
+// rigid configuration (PhysConfig defaults, contacts with the request's tank).
+void request_bodies(const SceneSample &req, std::vector<BoxBody> &bodies, std::vector<float> &density,
+                    RigidConfig &rc);
 
 ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeConfig &cfg,
                  std::vector<ServeResult> &results);

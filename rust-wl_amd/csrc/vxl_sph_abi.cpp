@@ -74,6 +74,19 @@ VxlSphStatus check_header(const VxlSphHeader &h, uint32_t size) {
     return VXL_SPH_OK;
 }
 
+BoxBody to_box_body(const VxlSphBoxBody &b) {
+    BoxBody o;
+    o.id = b.id;
+    for (int a = 0; a < 3; ++a) {
+        o.half[a] = b.half[a];
+        o.pos[a] = b.pos[a];
+        o.linvel[a] = b.linvel[a];
+        o.angvel[a] = b.angvel[a];
+    }
+    for (int a = 0; a < 4; ++a) o.rot[a] = b.rot[a];
+    return o;
+}
+
 SceneConfig to_scene_config(const VxlSphFluidConfig &c) {
     SceneConfig s;
     s.rest_density = c.rest_density;
@@ -438,6 +451,75 @@ VxlSphStatus vxl_sph_box_lattice(const float half[3], float sp, float h, uint32_
     });
 }
 
+VxlSphStatus vxl_sph_default_rigid_config(VxlSphRigidConfig *out) {
+    if (!out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    const RigidConfig d;
+    std::memset(out, 0, sizeof(*out));
+    out->hdr.size = sizeof(VxlSphRigidConfig);
+    out->hdr.version = VXL_SPH_ABI_VERSION;
+    out->substeps = d.substeps;
+    for (int a = 0; a < 3; ++a) {
+        out->gravity[a] = d.gravity[a];
+        out->tank[a] = d.tank[a];
+    }
+    out->max_linear = d.max_linear;
+    out->max_angular = d.max_angular;
+    return VXL_SPH_OK;
+}
+
+VxlSphStatus vxl_sph_scene_add_coupled(VxlSphEngine *e, const VxlSphFluidConfig *cfg, const VxlSphRigidConfig *rc,
+                                       uint32_t n_fluid, const float *fluid_pos, const float *fluid_vel,
+                                       uint32_t nbodies, const VxlSphRigidBox *bodies, uint32_t *out_scene) {
+    if (!e || !cfg || !rc || !out_scene || (n_fluid && !fluid_pos) || (nbodies && !bodies))
+        return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    st = check_header(rc->hdr, sizeof(VxlSphRigidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        RigidConfig r;
+        r.substeps = rc->substeps;
+        for (int a = 0; a < 3; ++a) {
+            r.gravity[a] = rc->gravity[a];
+            r.tank[a] = rc->tank[a];
+        }
+        r.max_linear = rc->max_linear;
+        r.max_angular = rc->max_angular;
+        std::vector<BoxBody> bs(nbodies);
+        std::vector<float> dens(nbodies);
+        for (uint32_t k = 0; k < nbodies; ++k) {
+            bs[k] = to_box_body(bodies[k].box);
+            dens[k] = bodies[k].density;
+        }
+        *out_scene = e->eng.add_coupled_scene(to_scene_config(*cfg), n_fluid, fluid_pos, fluid_vel, bs, dens, r);
+    });
+}
+
+VxlSphStatus vxl_sph_scene_read_bodies(VxlSphEngine *e, uint32_t scene, uint32_t cap, VxlSphRigidBox *out,
+                                       uint32_t *count) {
+    if (!e || !count) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        const SceneSlot &sc = e->eng.scene(scene);
+        *count = (uint32_t)sc.rigid.size();
+        if (!out) return;
+        if (cap < *count) throw std::length_error("output buffer too small");
+        const std::vector<RigidDev> cur = e->eng.read_bodies(scene);
+        for (size_t k = 0; k < cur.size(); ++k) {
+            VxlSphRigidBox &o = out[k];
+            std::memset(&o, 0, sizeof(o));
+            o.box.id = sc.rigid[k].body_id;
+            for (int a = 0; a < 3; ++a) {
+                o.box.half[a] = cur[k].half[a];
+                o.box.pos[a] = cur[k].pos[a];
+                o.box.linvel[a] = cur[k].linvel[a];
+                o.box.angvel[a] = cur[k].angvel[a];
+            }
+            for (int a = 0; a < 4; ++a) o.box.rot[a] = cur[k].rot[a];
+            o.density = sc.rigid[k].density;
+        }
+    });
+}
+
 VxlSphStatus vxl_sph_box_particles(const VxlSphFluidConfig *cfg, uint32_t nbodies, const VxlSphBoxBody *bodies,
                                    uint32_t first, uint32_t *n, uint32_t *nspans, float *pos, float *vel,
                                    float *pmass, VxlSphSpan *spans) {
@@ -446,16 +528,7 @@ VxlSphStatus vxl_sph_box_particles(const VxlSphFluidConfig *cfg, uint32_t nbodie
     if (st != VXL_SPH_OK) return st;
     return guard([&] {
         std::vector<BoxBody> bs(nbodies);
-        for (uint32_t k = 0; k < nbodies; ++k) {
-            bs[k].id = bodies[k].id;
-            for (int a = 0; a < 3; ++a) {
-                bs[k].half[a] = bodies[k].half[a];
-                bs[k].pos[a] = bodies[k].pos[a];
-                bs[k].linvel[a] = bodies[k].linvel[a];
-                bs[k].angvel[a] = bodies[k].angvel[a];
-            }
-            for (int a = 0; a < 4; ++a) bs[k].rot[a] = bodies[k].rot[a];
-        }
+        for (uint32_t k = 0; k < nbodies; ++k) bs[k] = to_box_body(bodies[k]);
         std::vector<float> p, v, m;
         std::vector<Span> sp;
         p.resize(3ull * first);  // spans index the whole particle array
@@ -622,6 +695,39 @@ VxlSphStatus vxl_sph_sample_requests(uint64_t seed, uint32_t count, double rate,
     });
 }
 
+VxlSphStatus vxl_sph_request_bodies(const VxlSphRequest *req, uint32_t cap, VxlSphRigidBox *out, uint32_t *count,
+                                    VxlSphRigidConfig *rc) {
+    if (!req || !count) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        const SceneSample s = from_request(*req);
+        std::vector<BoxBody> bodies;
+        std::vector<float> dens;
+        RigidConfig r;
+        if (s.nboxes) request_bodies(s, bodies, dens, r);
+        *count = (uint32_t)bodies.size();
+        if (rc) {
+            vxl_sph_default_rigid_config(rc);
+            for (int a = 0; a < 3; ++a) rc->tank[a] = r.tank[a];
+        }
+        if (!out) return;
+        if (cap < *count) throw std::length_error("output buffer too small");
+        for (size_t k = 0; k < bodies.size(); ++k) {
+            const BoxBody &b = bodies[k];
+            VxlSphRigidBox &o = out[k];
+            std::memset(&o, 0, sizeof(o));
+            o.box.id = b.id;
+            for (int a = 0; a < 3; ++a) {
+                o.box.half[a] = b.half[a];
+                o.box.pos[a] = b.pos[a];
+                o.box.linvel[a] = b.linvel[a];
+                o.box.angvel[a] = b.angvel[a];
+            }
+            for (int a = 0; a < 4; ++a) o.box.rot[a] = b.rot[a];
+            o.density = dens[k];
+        }
+    });
+}
+
 VxlSphStatus vxl_sph_request_config(const VxlSphRequest *req, float dt, uint64_t max_bins, VxlSphFluidConfig *out) {
     if (!req || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
     return guard([&] {
@@ -674,7 +780,6 @@ VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSp
         std::vector<SceneSample> v(n);
         for (uint32_t k = 0; k < n; ++k) {
             v[k] = from_request(reqs[k]);
-            if (v[k].nboxes) throw std::domain_error("requests with dynamic rigid boxes are not supported yet");
         }
         Engine eng(cfg->device);
         std::vector<ServeResult> res;

@@ -91,6 +91,15 @@ class BoxBody(C.Structure):
                 ("linvel", C.c_float * 3), ("angvel", C.c_float * 3)]
 
 
+class RigidConfig(C.Structure):
+    _fields_ = [("hdr", Header), ("substeps", C.c_uint32), ("gravity", C.c_float * 3), ("max_linear", C.c_float),
+                ("max_angular", C.c_float), ("tank", C.c_float * 3), ("reserved", C.c_uint32)]
+
+
+class RigidBox(C.Structure):
+    _fields_ = [("box", BoxBody), ("density", C.c_float)]
+
+
 class DamBreakSpec(C.Structure):
     _fields_ = [("hdr", Header), ("tank", C.c_float * 3), ("block", C.c_uint32 * 3), ("d", C.c_float),
                 ("jitter", C.c_float), ("seed", C.c_uint64), ("walls", C.c_uint32), ("reserved", C.c_uint32)]
@@ -140,6 +149,8 @@ EXPORTS = (
     "vxl_sph_stepper_create", "vxl_sph_stepper_destroy", "vxl_sph_stepper_step", "vxl_sph_stepper_reactions",
     "vxl_sph_stepper_bounds", "vxl_sph_stepper_read_state",
     "vxl_sph_box_lattice", "vxl_sph_box_particles", "vxl_sph_fluid_constants", "vxl_sph_dam_break",
+    "vxl_sph_default_rigid_config", "vxl_sph_scene_add_coupled", "vxl_sph_scene_read_bodies",
+    "vxl_sph_request_bodies",
     "vxl_sph_sample_batch", "vxl_sph_sample_requests", "vxl_sph_request_config", "vxl_sph_serve",
 )
 
@@ -218,6 +229,16 @@ def lib():
     L.vxl_sph_stepper_bounds.argtypes = [P, fp, fp]
     L.vxl_sph_stepper_read_state.restype = st
     L.vxl_sph_stepper_read_state.argtypes = [P, C.c_uint32, fp, fp]
+    L.vxl_sph_default_rigid_config.restype = st
+    L.vxl_sph_default_rigid_config.argtypes = [C.POINTER(RigidConfig)]
+    L.vxl_sph_scene_add_coupled.restype = st
+    L.vxl_sph_scene_add_coupled.argtypes = [P, C.POINTER(FluidConfig), C.POINTER(RigidConfig), C.c_uint32, fp, fp,
+                                            C.c_uint32, C.POINTER(RigidBox), up]
+    L.vxl_sph_scene_read_bodies.restype = st
+    L.vxl_sph_scene_read_bodies.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(RigidBox), up]
+    L.vxl_sph_request_bodies.restype = st
+    L.vxl_sph_request_bodies.argtypes = [C.POINTER(Request), C.c_uint32, C.POINTER(RigidBox), up,
+                                         C.POINTER(RigidConfig)]
     L.vxl_sph_box_lattice.restype = st
     L.vxl_sph_box_lattice.argtypes = [fp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, fp, up, up, fp]
     L.vxl_sph_box_particles.restype = st
@@ -297,17 +318,38 @@ def box_lattice(half, s, h, layers):
     return pts[:n.value], n0.value, np.float32(vol.value)
 
 
+def _fill_box(o, b):
+    o.id = b["id"]
+    o.half = (C.c_float * 3)(*b["half"])
+    o.pos = (C.c_float * 3)(*b["pos"])
+    o.rot = (C.c_float * 4)(*b.get("rot", (0.0, 0.0, 0.0, 1.0)))
+    o.linvel = (C.c_float * 3)(*b.get("linvel", (0.0, 0.0, 0.0)))
+    o.angvel = (C.c_float * 3)(*b.get("angvel", (0.0, 0.0, 0.0)))
+
+
+def rigid_config(substeps=None, gravity=None, max_linear=None, max_angular=None, tank=None):
+    """Rigid configuration of a coupled scene (PhysConfig defaults where None)."""
+    c = RigidConfig()
+    _check(lib().vxl_sph_default_rigid_config(C.byref(c)))
+    if substeps is not None:
+        c.substeps = substeps
+    if gravity is not None:
+        c.gravity = (C.c_float * 3)(*gravity)
+    if max_linear is not None:
+        c.max_linear = max_linear
+    if max_angular is not None:
+        c.max_angular = max_angular
+    if tank is not None:
+        c.tank = (C.c_float * 3)(*tank)
+    return c
+
+
 def box_particles(cfg, bodies, first):
     """set_boundary_particles for Box bodies (fluid_step.rs:13-53): returns
     (pos, vel, pmass, spans) with span indices starting at `first`."""
     arr = (BoxBody * max(len(bodies), 1))()
     for k, b in enumerate(bodies):
-        arr[k].id = b["id"]
-        arr[k].half = (C.c_float * 3)(*b["half"])
-        arr[k].pos = (C.c_float * 3)(*b["pos"])
-        arr[k].rot = (C.c_float * 4)(*b.get("rot", (0.0, 0.0, 0.0, 1.0)))
-        arr[k].linvel = (C.c_float * 3)(*b.get("linvel", (0.0, 0.0, 0.0)))
-        arr[k].angvel = (C.c_float * 3)(*b.get("angvel", (0.0, 0.0, 0.0)))
+        _fill_box(arr[k], b)
     n, ns = C.c_uint32(), C.c_uint32()
     _check(lib().vxl_sph_box_particles(C.byref(cfg), len(bodies), arr, first, C.byref(n), C.byref(ns),
                                        None, None, None, None))
@@ -505,6 +547,39 @@ class Engine:
         return dict(min=np.array(info.min, np.float32), inv=np.float32(info.inv), bin=np.float32(info.bin),
                     dims=[int(x) for x in info.dims], total=int(info.total), start=start, items=items, keys=keys)
 
+    def add_coupled(self, cfg, rigid_cfg, fluid_pos, bodies, fluid_vel=None):
+        """Scene of fluid particles and Box bodies (dicts with id, half, pos,
+        rot, linvel, angvel, density; density > 0 = dynamic, coupled both ways)."""
+        fp_ = np.ascontiguousarray(fluid_pos, np.float32).reshape(-1, 3)
+        fv = None if fluid_vel is None else np.ascontiguousarray(fluid_vel, np.float32).reshape(-1, 3)
+        arr = (RigidBox * max(len(bodies), 1))()
+        for k, b in enumerate(bodies):
+            _fill_box(arr[k].box, b)
+            arr[k].density = b.get("density", 0.0)
+        sid = C.c_uint32()
+        _check(lib().vxl_sph_scene_add_coupled(self.h, C.byref(cfg), C.byref(rigid_cfg), len(fp_), _f(fp_),
+                                               _f(fv) if fv is not None else None, len(bodies), arr,
+                                               C.byref(sid)))
+        # particle count: fluid + every body's lattice (one span per body with particles)
+        counts = [len(box_lattice(b["half"], cfg.spacing, max(cfg.smoothing_radius, 1e-5), cfg.boundary_layers)[0])
+                  for b in bodies]
+        self.scenes[sid.value] = (len(fp_) + sum(counts), len(fp_), sum(1 for c in counts if c))
+        return sid.value
+
+    def read_bodies(self, sid):
+        """Current state of the scene's dynamic bodies: list of dicts."""
+        cnt = C.c_uint32()
+        _check(lib().vxl_sph_scene_read_bodies(self.h, sid, 0, None, C.byref(cnt)))
+        arr = (RigidBox * max(cnt.value, 1))()
+        _check(lib().vxl_sph_scene_read_bodies(self.h, sid, cnt.value, arr, C.byref(cnt)))
+        out = []
+        for k in range(cnt.value):
+            b = arr[k].box
+            out.append(dict(id=b.id, half=np.array(b.half[:], np.float32), pos=np.array(b.pos[:], np.float32),
+                            rot=np.array(b.rot[:], np.float32), linvel=np.array(b.linvel[:], np.float32),
+                            angvel=np.array(b.angvel[:], np.float32), density=arr[k].density))
+        return out
+
     def read_bounds(self, sid):
         lo = np.zeros(3, np.float32)
         hi = np.zeros(3, np.float32)
@@ -585,12 +660,28 @@ def request_config(req, dt=5e-4, max_bins=1 << 21):
 
 
 def request_scene(req, dt=5e-4, max_bins=1 << 21):
-    """(config, SceneData) of one request, built exactly as the server builds it."""
+    """(config, SceneData) of one request, built exactly as the server builds it
+    (for a request with boxes: the fluid only; see request_bodies)."""
     cfg = request_config(req, dt, max_bins)
     dam = req.dam
-    data = dam_break(cfg, tuple(dam.tank), tuple(dam.block), dam.d, jitter=dam.jitter, seed=dam.seed,
-                     walls=bool(dam.walls))
+    walls = bool(dam.walls) and req.nboxes == 0
+    data = dam_break(cfg, tuple(dam.tank), tuple(dam.block), dam.d, jitter=dam.jitter, seed=dam.seed, walls=walls)
     return cfg, data
+
+
+def request_bodies(req):
+    """(rigid config, body dicts) a request with boxes is served with; ([], None) without boxes."""
+    cnt = C.c_uint32()
+    rc = RigidConfig()
+    _check(lib().vxl_sph_request_bodies(C.byref(req), 0, None, C.byref(cnt), C.byref(rc)))
+    arr = (RigidBox * max(cnt.value, 1))()
+    _check(lib().vxl_sph_request_bodies(C.byref(req), cnt.value, arr, C.byref(cnt), C.byref(rc)))
+    bodies = []
+    for k in range(cnt.value):
+        b = arr[k].box
+        bodies.append(dict(id=b.id, half=tuple(b.half), pos=tuple(b.pos), rot=tuple(b.rot), linvel=tuple(b.linvel),
+                           angvel=tuple(b.angvel), density=arr[k].density))
+    return (rc if cnt.value else None), bodies
 
 
 def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_resident=0, builders=4,

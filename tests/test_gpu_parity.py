@@ -298,14 +298,19 @@ def test_serve_results_equal_alone():
     """Served form: requests arriving over time, admitted as they arrive and
     co-stepped with whatever else is resident; every result is bit-identical
     to the same request stepped alone."""
-    reqs = V.sample_requests(5, 8, rate=200.0, nmin=1500, nmax=12000, smin=3, smax=12)
+    reqs = V.sample_requests(5, 8, rate=200.0, nmin=1500, nmax=12000, smin=3, smax=12, max_boxes=3)
+    assert any(r.nboxes for r in reqs) and any(not r.nboxes for r in reqs)
     stats, results = V.serve(reqs, budget_bytes=1 << 30, time_scale=1.0, builders=2)
     assert stats["completed"] == len(reqs)
     assert stats["peak_resident"] >= 2  # requests overlapped on the device
     for r, res in zip(reqs, results):
         cfg, data = V.request_scene(r)
+        rc, bodies = V.request_bodies(r)
         eng = V.Engine()
-        sid = eng.add(cfg, data, reactions=False)
+        if bodies:
+            sid = eng.add_coupled(cfg, rc, data.pos[:data.n_fluid], bodies)
+        else:
+            sid = eng.add(cfg, data, reactions=False)
         eng.step(r.steps)
         assert eng.state_hash(sid) == res["state_hash"]
         eng.close()
