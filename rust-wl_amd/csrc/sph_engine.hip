@@ -147,7 +147,7 @@ uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, cons
         sc.spans.push_back(spans[k]);
     }
     sc.want_reactions = reactions && nspans > 0;
-    sc.params.react = sc.want_reactions ? 1u : 0u;
+    sc.params.react = sc.want_reactions ? kReactAll : 0u;
     // Initial grid size (host grid_box over the initial AABB) sizes the tables.
     float lo[3] = {pos[0], pos[1], pos[2]}, hi[3] = {pos[0], pos[1], pos[2]};
     for (uint32_t i = 1; i < n; ++i)
@@ -221,7 +221,9 @@ uint32_t Engine::add_coupled_scene(const SceneConfig &cfg, uint32_t nf, const fl
     }
     const uint32_t n = (uint32_t)pmass.size();
     const uint32_t id = add_scene(cfg, n, nf, pos.data(), vel.data(), pmass.data(), spans.data(),
-                                  (uint32_t)spans.size(), true);
+                                  (uint32_t)spans.size(), !rigid.empty());
+    // Only the dynamic bodies consume a reaction; static bodies' spans are skipped.
+    if (!rigid.empty()) scenes_.back().params.react = kReactDynamic;
     scenes_.back().rigid = std::move(rigid);
     return id;
 }
@@ -455,11 +457,15 @@ void Engine::layout() {
             d.end = scenes_[k].params.pbase + s.end;
             d.substeps = scenes_[k].substeps;
             for (int a = 0; a < 3; ++a) d.origin[a] = s.origin[a];
+            d.active = scenes_[k].params.react == kReactAll ? 1u : 0u;
             spans.push_back(d);
         }
         // A dynamic body's torque origin is its current position.
-        for (const auto &r : scenes_[k].rigid)
-            for (int a = 0; a < 3; ++a) spans[span_first_[k] + r.span_local].origin[a] = r.dev.pos[a];
+        for (const auto &r : scenes_[k].rigid) {
+            SpanDev &d = spans[span_first_[k] + r.span_local];
+            for (int a = 0; a < 3; ++a) d.origin[a] = r.dev.pos[a];
+            d.active = 1u;
+        }
     }
     // Rigid table (scene order) and the bodies' body-frame lattices.
     std::vector<RigidDev> rig;

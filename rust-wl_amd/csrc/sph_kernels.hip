@@ -778,6 +778,12 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
         const float4 aj = P.sA[j];
         if (aj.w > 0.0f) continue;
         const uint32_t gj = P.sg[j];
+        if (sp.react == kReactDynamic) {  // only dynamic bodies consume a reaction
+            bool dyn = false;
+            for (uint32_t k = sp.rbody0; k < sp.rbody0 + sp.nrbody; ++k)
+                dyn = dyn || (gj >= P.rigid[k].gstart && gj < P.rigid[k].gend);
+            if (!dyn) continue;
+        }
         const SceneGrid &g = P.grid[s];
         const float4 vj = P.sV[j];
         const float rho_j = vj.w, cj2 = P.c2[j], mj = -aj.w;
@@ -845,16 +851,29 @@ __global__ __launch_bounds__(64) void k_reactions(Pool P) {
     const uint32_t k = blockIdx.x * 64 + threadIdx.x;
     if (k >= P.nspans) return;
     const SpanDev sd = P.spans[k];
+    if (!sd.active) return;
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
-    for (uint32_t q = sd.start; q < sd.end; ++q) {
-        const float4 f = P.bf[q];
-        const float4 p = P.bp[q];
+    auto fold = [&](const float4 f, const float4 p) {
         Fx = Fx + f.x; Fy = Fy + f.y; Fz = Fz + f.z;
         const float rx = p.x - sd.origin[0], ry = p.y - sd.origin[1], rz = p.z - sd.origin[2];
         Tx = Tx + (ry * f.z - rz * f.y);
         Ty = Ty + (rz * f.x - rx * f.z);
         Tz = Tz + (rx * f.y - ry * f.x);
+    };
+    // Serial in index order; loads are issued kU at a time ahead of the fold.
+    constexpr uint32_t kU = 16;
+    uint32_t q = sd.start;
+    for (; q + kU <= sd.end; q += kU) {
+        float4 f[kU], p[kU];
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) {
+            f[u] = P.bf[q + u];
+            p[u] = P.bp[q + u];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kU; ++u) fold(f[u], p[u]);
     }
+    for (; q < sd.end; ++q) fold(P.bf[q], P.bp[q]);
     const uint32_t sub = sd.substeps;
     const uint32_t phase = (P.ctl->step - 1u) % sub;  // substep index inside the tick
     float *o = P.react + 6ull * k;

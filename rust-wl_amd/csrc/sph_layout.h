@@ -42,7 +42,7 @@ struct SceneParams {
     float gx, gy, gz;
     float dt, vmax, vmax2;
     uint32_t clamp_neg;  // tensile_instability_suppression
-    uint32_t react;      // compute boundary reactions for this scene's spans
+    uint32_t react;      // boundary reactions: 0 none, kReactAll every span, kReactDynamic dynamic bodies' spans
     uint32_t rbody0;     // first dynamic rigid body of the scene (pool rigid table)
     uint32_t nrbody;     // dynamic rigid bodies of the scene (0: none)
 };
@@ -80,8 +80,11 @@ struct SpanDev {
     uint32_t start, end;
     uint32_t substeps;   // tick = substeps substeps; reactions are tick averages
     float origin[3];
-    uint32_t pad;
+    uint32_t active;     // 0: the span's reaction is not computed (left at zero)
 };
+
+constexpr uint32_t kReactAll = 1;      // every span (FluidStepper semantics)
+constexpr uint32_t kReactDynamic = 2;  // only spans of dynamic rigid bodies (coupled scenes)
 
 constexpr uint32_t kReactMaxNeighbours = 96;  // fluid neighbours per boundary particle
 

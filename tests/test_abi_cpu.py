@@ -123,3 +123,48 @@ def test_gamma_other_than_seven_is_unsupported_not_silent():
     L = V.lib()
     out = np.zeros(8, np.float32)
     assert L.vxl_sph_fluid_constants(C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_float))) == 0
+
+
+def test_rigid_config_defaults_are_physconfig():
+    """vxl_sph_default_rigid_config = PhysConfig::default (config.rs:206-233):
+    2 substeps, |v| <= 100, |w| <= 50, g = (0, -9.81, 0); no tank planes."""
+    rc = V.rigid_config()
+    assert rc.hdr.size == C.sizeof(V.RigidConfig) and rc.hdr.version == V.ABI_VERSION
+    assert rc.substeps == 2 and rc.max_linear == 100.0 and rc.max_angular == 50.0
+    assert list(rc.gravity) == [0.0, np.float32(-9.81), 0.0] and list(rc.tank) == [0.0, 0.0, 0.0]
+
+
+def test_coupled_scene_argument_errors():
+    L = V.lib()
+    cfg = V.fluid_config()
+    rc = V.rigid_config()
+    sid = C.c_uint32()
+    assert L.vxl_sph_scene_add_coupled(None, C.byref(cfg), C.byref(rc), 0, None, None, 0, None, C.byref(sid)) == 1
+    cnt = C.c_uint32()
+    assert L.vxl_sph_scene_read_bodies(None, 0, 0, None, C.byref(cnt)) == 1
+    assert L.vxl_sph_request_bodies(None, 0, None, C.byref(cnt), None) == 1
+
+
+def test_request_bodies_are_tank_then_boxes():
+    """A served request with boxes runs with the tank's five static Box bodies
+    (the same particles as the request's own tank) followed by its dynamic
+    boxes; a request without boxes has none."""
+    reqs = V.sample_requests(5, 8, rate=200.0, nmin=1500, nmax=12000, smin=3, smax=12, max_boxes=3)
+    for r in reqs:
+        rc, bodies = V.request_bodies(r)
+        if not r.nboxes:
+            assert rc is None and bodies == []
+            continue
+        assert len(bodies) == 5 + r.nboxes
+        assert all(b["density"] == 0.0 for b in bodies[:5])
+        for k in range(r.nboxes):
+            b = bodies[5 + k]
+            assert b["density"] == np.float32(r.box_density[k]) and 500.0 <= b["density"] <= 2000.0
+            assert b["id"] == r.boxes[k].id and tuple(b["pos"]) == tuple(r.boxes[k].pos)
+        assert list(rc.tank) == list(r.dam.tank) and rc.substeps == 2
+        # The static bodies' particles equal the request's tank built by dam_break.
+        cfg, _ = V.request_scene(r)
+        tank_pos, _, _, _ = V.box_particles(cfg, bodies[:5], 0)
+        with_tank = V.dam_break(cfg, tuple(r.dam.tank), tuple(r.dam.block), r.dam.d, jitter=r.dam.jitter,
+                                seed=r.dam.seed, walls=True)
+        np.testing.assert_array_equal(tank_pos, with_tank.pos[with_tank.n_fluid:])
