@@ -151,6 +151,7 @@ private:
     void check_error_state();  // read the control block; grow tables on demand
     void alloc_pool(uint32_t slots);
     void alloc_cells(uint32_t ccap);
+    void free_cells();  // counts, start, fcount and the scan's tile sums
     void free_all();
     template <class T>
     T *dalloc(size_t n);

@@ -307,10 +307,24 @@ void Engine::alloc_cells(uint32_t ccap) {
     pool_allocs_.push_back(P_.counts);
     P_.start = dalloc<uint32_t>((size_t)ccap + 1);
     pool_allocs_.push_back(P_.start);
+    P_.fcount = dalloc<uint32_t>(ccap);
+    pool_allocs_.push_back(P_.fcount);
     hip_check(hipMemset(P_.counts, 0, sizeof(uint32_t) * ccap), "memset counts");
     tiles_cap_ = (ccap + 2047) / 2048 + 1;
     d_tile_sums_ = dalloc<uint32_t>(tiles_cap_);
     P_.ccap = ccap;
+}
+
+void Engine::free_cells() {
+    void *cell_arrays[3] = {P_.counts, P_.start, P_.fcount};
+    pool_allocs_.erase(std::remove_if(pool_allocs_.begin(), pool_allocs_.end(),
+                                      [&](void *p) { return p == cell_arrays[0] || p == cell_arrays[1] ||
+                                                            p == cell_arrays[2]; }),
+                       pool_allocs_.end());
+    for (void *p : cell_arrays) dfree(p);
+    dfree(d_tile_sums_);
+    P_.counts = P_.start = P_.fcount = nullptr;
+    d_tile_sums_ = nullptr;
 }
 
 void *Engine::staging(size_t bytes) {
@@ -385,12 +399,7 @@ void Engine::layout() {
         alloc_cells((uint32_t)std::min<uint64_t>(std::max<uint64_t>({cells, (uint64_t)ccap_keep, 4096ull}),
                                                  0xfffffff0ull));
     } else if (cells > ccap_) {
-        pool_allocs_.erase(std::remove_if(pool_allocs_.begin(), pool_allocs_.end(),
-                                          [&](void *p) { return p == P_.counts || p == P_.start; }),
-                           pool_allocs_.end());
-        dfree(P_.counts);
-        dfree(P_.start);
-        dfree(d_tile_sums_);
+        free_cells();
         alloc_cells((uint32_t)std::min<uint64_t>(cells, 0xfffffff0ull));
     }
     // Move resident scenes: old position (src arrays) -> new position (dst).
@@ -769,12 +778,7 @@ void Engine::check_error_state() {
         const uint32_t new_cap = (uint32_t)std::min<uint64_t>((uint64_t)ccap_ * 2ull, 0xfffffff0ull);
         if (new_cap <= ccap_) throw CapacityError("cell table capacity exhausted");
         invalidate_graph();
-        dfree(P_.counts);
-        dfree(P_.start);
-        dfree(d_tile_sums_);
-        pool_allocs_.erase(std::remove_if(pool_allocs_.begin(), pool_allocs_.end(),
-                                          [&](void *p) { return p == P_.counts || p == P_.start; }),
-                           pool_allocs_.end());
+        free_cells();
         alloc_cells(new_cap);
         h_ctl_->err = 0;
         hip_check(hipMemcpy(&d_ctl_->err, &h_ctl_->err, sizeof(uint32_t), hipMemcpyHostToDevice), "ctl reset");

@@ -34,6 +34,7 @@ struct Pool {
     // sort
     uint32_t *key, *slot, *items_m, *items_g;
     uint32_t *counts, *start;  // counts[ccap], start[ccap + 1]
+    uint32_t *fcount;          // [ccap] fluid items heading each non-empty cell (this substep)
     // optional diagnostics (grid order; null = off)
     uint32_t *ncount;
     float4 *acc, *xsph;

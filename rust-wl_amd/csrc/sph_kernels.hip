@@ -74,12 +74,11 @@ __device__ __forceinline__ Cursor make_cursor(const SceneGrid &g, float h, float
 
 // Visit the 3x3 rows of the 27-cell neighbourhood in the reference order
 // (dz outer, dy inner). Each row is handed over as its index
-// r = 3 (dz + 1) + dy + 1, whether it starts at the cell dx = -1, and one slot
-// range covering the cells dx = -1, 0, +1 that survive the conservative
-// distance test.
+// r = 3 (dz + 1) + dy + 1, its first and last internal cell (the cells
+// dx = -1, 0, +1 that survive the conservative distance test) and whether it
+// starts at the cell dx = -1.
 template <class F>
-__device__ __forceinline__ void for_rows(const SceneGrid &g, const uint32_t *__restrict__ start, const Cursor &c,
-                                         F &&visit) {
+__device__ __forceinline__ void for_row_windows(const SceneGrid &g, const Cursor &c, F &&visit) {
     const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
     const float lx = c.fx * c.fx, rx = (1.0f - c.fx) * (1.0f - c.fx);
     for (int dz = -1; dz <= 1; ++dz) {
@@ -95,10 +94,19 @@ __device__ __forceinline__ void for_rows(const SceneGrid &g, const uint32_t *__r
             const int xl = (c.cx > 0 && rd + lx <= c.cut) ? c.cx - 1 : c.cx;
             const int xr = (c.cx + 1 < nx && rd + rx <= c.cut) ? c.cx + 1 : c.cx;
             const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-            visit((uint32_t)(3 * (dz + 1) + dy + 1), xl < c.cx, start[row + (uint32_t)xl],
-                  start[row + (uint32_t)xr + 1u]);
+            visit((uint32_t)(3 * (dz + 1) + dy + 1), row + (uint32_t)xl, row + (uint32_t)xr, xl < c.cx);
         }
     }
+}
+
+// for_row_windows with each row as one slot range [j0, j1) (x-fastest cells
+// make the row's cells contiguous).
+template <class F>
+__device__ __forceinline__ void for_rows(const SceneGrid &g, const uint32_t *__restrict__ start, const Cursor &c,
+                                         F &&visit) {
+    for_row_windows(g, c, [&](uint32_t r, uint32_t cf, uint32_t cl, bool left) {
+        visit(r, left, start[cf], start[cl + 1u]);
+    });
 }
 
 // Last cell (dx = 0 or +1) of row r as for_rows chooses it.
@@ -478,7 +486,8 @@ __global__ __launch_bounds__(kBlock) void k_place(Pool P) {
 
 // Canonical position = cell start + number of cell members with a smaller
 // original index (cell order = ascending index, fluid grid.rs:53-75), then
-// gather the particle into the grid-ordered arrays.
+// gather the particle into the grid-ordered arrays; also the count of fluid
+// items per cell (they come first: fluid indices precede boundary ones).
 __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
     if (P.ctl->err) return;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
@@ -490,12 +499,19 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
         const uint32_t g = P.items_g[u];
         const uint32_t c = P.key[m];
         const uint32_t s0 = P.start[c], e0 = P.start[c + 1];
-        uint32_t rank = 0;
-        for (uint32_t k = s0; k < e0; ++k) rank += P.items_g[k] < g ? 1u : 0u;
+        uint32_t rank = 0, nfl = 0;
+        for (uint32_t k = s0; k < e0; ++k) {
+            const uint32_t gk = P.items_g[k];
+            rank += gk < g ? 1u : 0u;
+            nfl += gk < sp.nf ? 1u : 0u;
+        }
         const uint32_t d = s0 + rank;
         P.sA[d] = P.cA[m];
         P.sV[d] = P.cV[m];
         P.sg[d] = g;
+        // Fluid items (smaller indices) head the cell; the cell's first item
+        // records how many there are (stale for empty cells: readers clamp).
+        if (rank == 0) P.fcount[c] = nfl;
     }
 }
 
@@ -597,13 +613,33 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
                 summ |= row_summary(r, left, tail);
             }
         };
-        for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
-            // Only the particle's own row (r = 4, uniform across the wave) can
-            // hold i itself.
-            const uint32_t mk = r == 4u ? dens_row<true>(sA, ai, i, j0, j1, h2, k6, da, cnt)
-                                        : dens_row<false>(sA, ai, i, j0, j1, h2, k6, da, cnt);
-            close_row(r, left, j0, j1, mk);
-        });
+        // Waves holding a fluid particle take the full scan (a wave running
+        // both scans would pay for both); all-boundary waves scan fluid heads.
+        if (__any(fluid_i) || P.ncount) {
+            for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
+                // Only the particle's own row (r = 4, uniform across the wave)
+                // can hold i itself.
+                const uint32_t mk = r == 4u ? dens_row<true>(sA, ai, i, j0, j1, h2, k6, da, cnt)
+                                            : dens_row<false>(sA, ai, i, j0, j1, h2, k6, da, cnt);
+                close_row(r, left, j0, j1, mk);
+            });
+        } else {
+            // Boundary particles sum fluid neighbours only
+            // (fluid_density.rs:59-69): scan just the fluid head of each cell,
+            // in cell order. (With neighbour counts requested, the full scan
+            // above counts boundary neighbours too.)
+            for_row_windows(g, cur, [&](uint32_t, uint32_t cf, uint32_t cl, bool) {
+                uint32_t lo = P.start[cf];
+                for (uint32_t c = cf; c <= cl; ++c) {
+                    const uint32_t hi = P.start[c + 1u];
+                    const uint32_t jn = lo + min(P.fcount[c], hi - lo);
+                    uint32_t unused = 0;
+                    for (uint32_t jb = lo; jb < jn; jb += 4u)
+                        dens_group<false, true, false>(sA, ai, i, jb, lo, jn, h2, k6, da, unused, cnt);
+                    lo = hi;
+                }
+            });
+        }
         if (fluid_i) P.nbs[i] = summ;
         const float rho = fluid_i ? sp.mass * da.sum + da.sum_b : (-ai.w) * sp.w0 + da.sum;
         const float q = rho / sp.rho0;

@@ -69,6 +69,24 @@ def test_one_step_bit_exact():
     assert_state_equal(vel, orc.vel(), "velocity", cfg.sound_speed * 1e-3)
 
 
+def test_boundary_density_from_fluid_heads_bit_exact():
+    """Without neighbour-count diagnostics a boundary particle scans only the
+    fluid items heading each cell (fluid indices precede boundary ones);
+    density and pressure of every particle, boundary included, equal the
+    oracle's bit for bit over several steps."""
+    cfg, data, tank, d = small_scene(seed=21, block=(14, 12, 10), tank=(0.5, 0.4, 0.35))
+    eng = V.Engine(diagnostics=False)
+    sid = eng.add(cfg, data, reactions=False)
+    orc = oracle_dam_break(cfg, data, tank, d)
+    for _ in range(5):
+        eng.step(1)
+        orc.step(cfg.dt_tick)
+        f = eng.read_fields(sid)
+        np.testing.assert_array_equal(bits(f["dens"]), bits(orc.dens()))
+        np.testing.assert_array_equal(bits(f["press"]), bits(orc.press()))
+    assert eng.state_hash(sid) == orc.state_hash()
+
+
 def test_many_steps_bit_exact_and_rerun_hash():
     cfg, data, tank, d = small_scene(seed=11)
     orc = oracle_dam_break(cfg, data, tank, d)
