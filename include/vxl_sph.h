@@ -175,6 +175,12 @@ VxlSphStatus vxl_sph_scene_read_bounds(VxlSphEngine *e, uint32_t scene, float lo
  * (crates/vxl-phys-fluid/examples/sph_scale.rs:52-63). */
 VxlSphStatus vxl_sph_scene_state_hash(VxlSphEngine *e, uint32_t scene, uint64_t *out);
 
+/* FNV-1a-64 byte-wise over every particle's position (index order, x y z, f32
+ * little-endian bytes) then every velocity: the A/B state hash of
+ * crates/vxl-phys-gpu/tests/packet_grid_canary.rs:102-111 (the other
+ * convention, vxl_sph_scene_state_hash, follows sph_scale.rs). */
+VxlSphStatus vxl_sph_scene_state_hash_all(VxlSphEngine *e, uint32_t scene, uint64_t *out);
+
 /* ---- FluidStepper drop-in (single scene; GpuFluidStepper semantics) ---- */
 VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,
                                     VxlSphStepper **out);
@@ -265,6 +271,28 @@ typedef struct VxlSphDamBreak {
 VxlSphStatus vxl_sph_dam_break(const VxlSphDamBreak *spec, const VxlSphFluidConfig *cfg, uint32_t *n,
                                uint32_t *n_fluid, uint32_t *nspans, float *pos, float *vel, float *pmass,
                                VxlSphSpan *spans);
+
+/* ---- frame dumps (VXLD v3) ---- */
+/* The binary frame format of the reference demos and renderer
+ * (crates/vxl-phys/examples/dam_break.rs:14-44, :84-128; body records as
+ * showcase.rs:311-360; read by scripts/render_demo.py:39-100): header, then per
+ * frame the tick, a time in ms, Box body records (kind, awake, pose, half
+ * extents) and `nfluid` fluid systems of particle positions. No voxels, splats
+ * or meshes are written. */
+typedef struct VxlSphVxld VxlSphVxld;
+VxlSphStatus vxl_sph_vxld_open(const char *path, uint32_t ticks, uint32_t ticks_per_frame, float dt,
+                               uint32_t nfluid, VxlSphVxld **out);
+/* A frame of engine scenes (nscenes == nfluid): each coupled scene's static
+ * bodies (asleep) and dynamic bodies (awake, current pose), then each scene's
+ * fluid positions (index order) as one fluid system. */
+VxlSphStatus vxl_sph_vxld_frame(VxlSphVxld *w, VxlSphEngine *e, uint32_t tick, float ms, uint32_t nscenes,
+                                const uint32_t *scenes);
+/* A frame from host arrays: bodies (density > 0 written awake), counts[nfluid]
+ * and the systems' positions back to back (3 floats per particle). */
+VxlSphStatus vxl_sph_vxld_frame_raw(VxlSphVxld *w, uint32_t tick, float ms, uint32_t nbodies,
+                                    const VxlSphRigidBox *bodies, uint32_t nfluid, const uint32_t *counts,
+                                    const float *pos);
+VxlSphStatus vxl_sph_vxld_close(VxlSphVxld *w);
 
 /* ---- batched and served forms ---- */
 /* A sampled dam-break request: scene geometry and physics, requested steps,

@@ -48,6 +48,7 @@ struct SceneSlot {
         float density = 0.0f;
     };
     std::vector<Rigid> rigid;
+    std::vector<BoxBody> statics;  // static Box bodies of a coupled scene (for frame dumps)
 };
 
 // Rigid configuration of a coupled scene (PhysConfig defaults,

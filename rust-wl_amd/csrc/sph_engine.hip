@@ -225,6 +225,8 @@ uint32_t Engine::add_coupled_scene(const SceneConfig &cfg, uint32_t nf, const fl
     // Only the dynamic bodies consume a reaction; static bodies' spans are skipped.
     if (!rigid.empty()) scenes_.back().params.react = kReactDynamic;
     scenes_.back().rigid = std::move(rigid);
+    for (size_t b = 0; b < bodies.size(); ++b)
+        if (!(density[b] > 0.0f)) scenes_.back().statics.push_back(bodies[b]);
     return id;
 }
 

@@ -11,6 +11,7 @@
 
 #include "sph_engine.h"
 #include "sph_serve.h"
+#include "sph_vxld.h"
 
 using namespace vxl;
 
@@ -364,6 +365,26 @@ VxlSphStatus vxl_sph_scene_state_hash(VxlSphEngine *e, uint32_t scene, uint64_t 
     });
 }
 
+VxlSphStatus vxl_sph_scene_state_hash_all(VxlSphEngine *e, uint32_t scene, uint64_t *out) {
+    if (!e || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        const SceneSlot &sc = e->eng.scene(scene);
+        std::vector<float> p(3ull * sc.params.n), v(3ull * sc.params.n);
+        e->eng.read_state(scene, p.data(), v.data());
+        uint64_t h = 0xcbf29ce484222325ull;
+        auto bytes = [&](const std::vector<float> &a) {
+            const unsigned char *b = reinterpret_cast<const unsigned char *>(a.data());
+            for (size_t k = 0; k < 4 * a.size(); ++k) {
+                h ^= (uint64_t)b[k];
+                h *= 0x00000100000001b3ull;
+            }
+        };
+        bytes(p);
+        bytes(v);
+        *out = h;
+    });
+}
+
 // ---- FluidStepper drop-in ----
 
 VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,
@@ -693,6 +714,89 @@ VxlSphStatus vxl_sph_sample_requests(uint64_t seed, uint32_t count, double rate,
         const auto v = sample_requests(seed, count, rate, nmin, nmax, smin, smax, max_boxes);
         for (uint32_t k = 0; k < count; ++k) to_request(v[k], out[k]);
     });
+}
+
+// ---- frame dumps ----
+
+struct VxlSphVxld {
+    vxl::VxldWriter w;
+    VxlSphVxld(const char *path, uint32_t ticks, uint32_t tpf, float dt, uint32_t nfluid)
+        : w(path, ticks, tpf, dt, nfluid) {}
+};
+
+namespace {
+VxldBody to_vxld(const BoxBody &b, uint8_t awake) {
+    VxldBody o;
+    o.awake = awake;
+    for (int a = 0; a < 3; ++a) {
+        o.pos[a] = b.pos[a];
+        o.half[a] = b.half[a];
+    }
+    for (int a = 0; a < 4; ++a) o.rot[a] = b.rot[a];
+    return o;
+}
+}  // namespace
+
+VxlSphStatus vxl_sph_vxld_open(const char *path, uint32_t ticks, uint32_t ticks_per_frame, float dt,
+                               uint32_t nfluid, VxlSphVxld **out) {
+    if (!path || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    *out = nullptr;
+    return guard([&] { *out = new VxlSphVxld(path, ticks, ticks_per_frame, dt, nfluid); });
+}
+
+VxlSphStatus vxl_sph_vxld_frame(VxlSphVxld *w, VxlSphEngine *e, uint32_t tick, float ms, uint32_t nscenes,
+                                const uint32_t *scenes) {
+    if (!w || !e || (nscenes && !scenes)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        std::vector<VxldBody> bodies;
+        std::vector<std::vector<float>> pos(nscenes);
+        std::vector<std::pair<const float *, uint32_t>> fl;
+        for (uint32_t k = 0; k < nscenes; ++k) {
+            const SceneSlot &sc = e->eng.scene(scenes[k]);
+            for (const BoxBody &b : sc.statics) bodies.push_back(to_vxld(b, 0u));
+            const std::vector<RigidDev> dyn = e->eng.read_bodies(scenes[k]);
+            for (const RigidDev &d : dyn) {
+                VxldBody o;
+                o.awake = 1u;
+                for (int a = 0; a < 3; ++a) {
+                    o.pos[a] = d.pos[a];
+                    o.half[a] = d.half[a];
+                }
+                for (int a = 0; a < 4; ++a) o.rot[a] = d.rot[a];
+                bodies.push_back(o);
+            }
+            pos[k].resize(3ull * sc.params.n);
+            e->eng.read_state(scenes[k], pos[k].data(), nullptr);
+            fl.push_back({pos[k].data(), sc.params.nf});
+        }
+        w->w.frame(tick, ms, bodies, fl);
+    });
+}
+
+VxlSphStatus vxl_sph_vxld_frame_raw(VxlSphVxld *w, uint32_t tick, float ms, uint32_t nbodies,
+                                    const VxlSphRigidBox *bodies, uint32_t nfluid, const uint32_t *counts,
+                                    const float *pos) {
+    if (!w || (nbodies && !bodies) || (nfluid && !counts)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        std::vector<VxldBody> bs;
+        for (uint32_t k = 0; k < nbodies; ++k)
+            bs.push_back(to_vxld(to_box_body(bodies[k].box), bodies[k].density > 0.0f ? 1u : 0u));
+        std::vector<std::pair<const float *, uint32_t>> fl;
+        size_t at = 0;
+        for (uint32_t k = 0; k < nfluid; ++k) {
+            if (counts[k] && !pos) throw std::invalid_argument("positions are null");
+            fl.push_back({pos ? pos + 3 * at : nullptr, counts[k]});
+            at += counts[k];
+        }
+        w->w.frame(tick, ms, bs, fl);
+    });
+}
+
+VxlSphStatus vxl_sph_vxld_close(VxlSphVxld *w) {
+    if (!w) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = guard([&] { w->w.close(); });
+    delete w;
+    return st;
 }
 
 VxlSphStatus vxl_sph_request_bodies(const VxlSphRequest *req, uint32_t cap, VxlSphRigidBox *out, uint32_t *count,

@@ -150,7 +150,8 @@ EXPORTS = (
     "vxl_sph_stepper_bounds", "vxl_sph_stepper_read_state",
     "vxl_sph_box_lattice", "vxl_sph_box_particles", "vxl_sph_fluid_constants", "vxl_sph_dam_break",
     "vxl_sph_default_rigid_config", "vxl_sph_scene_add_coupled", "vxl_sph_scene_read_bodies",
-    "vxl_sph_request_bodies",
+    "vxl_sph_request_bodies", "vxl_sph_scene_state_hash_all", "vxl_sph_vxld_open", "vxl_sph_vxld_frame",
+    "vxl_sph_vxld_frame_raw", "vxl_sph_vxld_close",
     "vxl_sph_sample_batch", "vxl_sph_sample_requests", "vxl_sph_request_config", "vxl_sph_serve",
 )
 
@@ -239,6 +240,17 @@ def lib():
     L.vxl_sph_request_bodies.restype = st
     L.vxl_sph_request_bodies.argtypes = [C.POINTER(Request), C.c_uint32, C.POINTER(RigidBox), up,
                                          C.POINTER(RigidConfig)]
+    L.vxl_sph_scene_state_hash_all.restype = st
+    L.vxl_sph_scene_state_hash_all.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint64)]
+    L.vxl_sph_vxld_open.restype = st
+    L.vxl_sph_vxld_open.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(P)]
+    L.vxl_sph_vxld_frame.restype = st
+    L.vxl_sph_vxld_frame.argtypes = [P, P, C.c_uint32, C.c_float, C.c_uint32, up]
+    L.vxl_sph_vxld_frame_raw.restype = st
+    L.vxl_sph_vxld_frame_raw.argtypes = [P, C.c_uint32, C.c_float, C.c_uint32, C.POINTER(RigidBox), C.c_uint32, up,
+                                         fp]
+    L.vxl_sph_vxld_close.restype = st
+    L.vxl_sph_vxld_close.argtypes = [P]
     L.vxl_sph_box_lattice.restype = st
     L.vxl_sph_box_lattice.argtypes = [fp, C.c_float, C.c_float, C.c_uint32, C.c_uint32, fp, up, up, fp]
     L.vxl_sph_box_particles.restype = st
@@ -586,6 +598,12 @@ class Engine:
         _check(lib().vxl_sph_scene_read_bounds(self.h, sid, _f(lo), _f(hi)))
         return lo, hi
 
+    def state_hash_all(self, sid):
+        """Byte-wise FNV-1a over all positions then all velocities (packet_grid_canary.rs:102-111)."""
+        v = C.c_uint64()
+        _check(lib().vxl_sph_scene_state_hash_all(self.h, sid, C.byref(v)))
+        return v.value
+
     def state_hash(self, sid):
         v = C.c_uint64()
         _check(lib().vxl_sph_scene_state_hash(self.h, sid, C.byref(v)))
@@ -707,3 +725,34 @@ def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_r
     results = [dict(index=r.index, n=r.n, n_fluid=r.n_fluid, steps=r.steps, arrival_s=r.arrival_s,
                     admit_s=r.admit_s, finish_s=r.finish_s, state_hash=r.state_hash) for r in res[:n]]
     return stats, results
+
+
+class VxldWriter:
+    """VXLD v3 frame dump (the reference demos' format, read by its renderer)."""
+
+    def __init__(self, path, ticks, ticks_per_frame, dt, nfluid):
+        self.h = C.c_void_p()
+        _check(lib().vxl_sph_vxld_open(str(path).encode(), ticks, ticks_per_frame, C.c_float(dt), nfluid,
+                                       C.byref(self.h)))
+
+    def frame(self, engine, tick, ms, scenes):
+        arr = (C.c_uint32 * max(len(scenes), 1))(*scenes)
+        _check(lib().vxl_sph_vxld_frame(self.h, engine.h, tick, C.c_float(ms), len(scenes), arr))
+
+    def frame_raw(self, tick, ms, bodies, fluids):
+        """bodies: dicts as Engine.add_coupled takes; fluids: list of (n, 3) arrays."""
+        arr = (RigidBox * max(len(bodies), 1))()
+        for k, b in enumerate(bodies):
+            _fill_box(arr[k].box, b)
+            arr[k].density = b.get("density", 0.0)
+        counts = np.array([len(f) for f in fluids], np.uint32)
+        pos = (np.concatenate([np.asarray(f, np.float32).reshape(-1, 3) for f in fluids])
+               if fluids else np.zeros((0, 3), np.float32))
+        pos = np.ascontiguousarray(pos, np.float32)
+        _check(lib().vxl_sph_vxld_frame_raw(self.h, tick, C.c_float(ms), len(bodies), arr, len(fluids), _u(counts),
+                                            _f(pos) if len(pos) else None))
+
+    def close(self):
+        if self.h:
+            _check(lib().vxl_sph_vxld_close(self.h))
+            self.h = C.c_void_p()

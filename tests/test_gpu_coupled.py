@@ -121,3 +121,36 @@ def test_coupled_scene_is_unchanged_by_relayouts_and_neighbours():
     for ga, gm in zip(alone.read_bodies(a), mixed.read_bodies(m)):
         for f in ("pos", "rot", "linvel", "angvel"):
             np.testing.assert_array_equal(bits(ga[f]), bits(gm[f]))
+
+
+def test_vxld_frames_of_a_coupled_scene_and_byte_hash(tmp_path):
+    """VXLD v3 frames written from the engine hold the scene's bodies (static
+    tank asleep, dynamic boxes awake at their current pose) and its fluid
+    positions; the byte-wise state hash equals a Python restatement of
+    packet_grid_canary.rs:102-111 over the read-back arrays."""
+    from test_vxld import read_vxld
+
+    cfg, fluid, statics, boxes, tank, d = coupled_scene(seed=13)
+    nf = fluid.n_fluid
+    eng = V.Engine()
+    sid = eng.add_coupled(cfg, V.rigid_config(tank=tank), fluid.pos[:nf], statics + boxes)
+    path = tmp_path / "coupled.bin"
+    w = V.VxldWriter(path, ticks=20, ticks_per_frame=10, dt=cfg.dt_tick, nfluid=1)
+    for t in (10, 20):
+        eng.step(10)
+        w.frame(eng, t, 0.0, [sid])
+    w.close()
+    dump = read_vxld(path)
+    assert [f["tick"] for f in dump["frames"]] == [10, 20]
+    last = dump["frames"][-1]
+    pos, vel = eng.read_state(sid)
+    np.testing.assert_array_equal(last["fluids"][0], pos[:nf])
+    assert [b["awake"] for b in last["bodies"]] == [0] * len(statics) + [1] * len(boxes)
+    for got, want in zip(last["bodies"][len(statics):], eng.read_bodies(sid)):
+        np.testing.assert_array_equal(np.array(got["pos"], F), want["pos"])
+        np.testing.assert_array_equal(np.array(got["rot"], F), want["rot"])
+        np.testing.assert_array_equal(np.array(got["half"], F), want["half"])
+    h = 0xCBF29CE484222325
+    for b in np.concatenate([pos.ravel(), vel.ravel()]).astype("<f4").tobytes():
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    assert eng.state_hash_all(sid) == h
