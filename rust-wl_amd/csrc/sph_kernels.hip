@@ -519,17 +519,27 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
 // fluid neighbours (times its mass at the end) and m_b W over boundary ones;
 // boundary i sums m_j W over fluid neighbours only. A candidate with
 // acc = false leaves both sums untouched (a select, not a branch).
+//
+// The class of j is the sign of its packed mass (masses are validated positive
+// and finite, so the sign bit alone decides). ALL_FLUID: every lane of the
+// wave is a fluid particle, which drops the per-candidate class select of i.
+template <bool ALL_FLUID>
 struct DensAcc {
     float sum, sum_b;
     bool fluid_i;
     __device__ __forceinline__ void add(bool acc, float r2, float aw, float h2, float k6) {
         const float t = h2 - r2;
         const float w = ((k6 * t) * t) * t;
-        const bool fj = aw > 0.0f;
+        const bool fj = __float_as_int(aw) >= 0;
         const float mw = fabsf(aw) * w;
-        const float add_sum = fluid_i ? w : mw;
-        sum = (acc && fj) ? sum + add_sum : sum;
-        sum_b = (acc && !fj && fluid_i) ? sum_b + mw : sum_b;
+        if (ALL_FLUID) {
+            sum = (acc && fj) ? sum + w : sum;
+            sum_b = (acc && !fj) ? sum_b + mw : sum_b;
+        } else {
+            const float add_sum = fluid_i ? w : mw;
+            sum = (acc && fj) ? sum + add_sum : sum;
+            sum_b = (acc && !fj && fluid_i) ? sum_b + mw : sum_b;
+        }
     }
 };
 
@@ -537,9 +547,9 @@ struct DensAcc {
 // clamping: the arrays are padded). SELF: the range may hold i; PART: the
 // group may run past j1; MASK: set the mask bits (candidate k = j - j0 < 32),
 // else count the accepted ones.
-template <bool SELF, bool PART, bool MASK>
+template <bool SELF, bool PART, bool MASK, class Acc>
 __device__ __forceinline__ void dens_group(const float4 *__restrict__ sA, const float4 ai, uint32_t i, uint32_t jb,
-                                           uint32_t j0, uint32_t j1, float h2, float k6, DensAcc &da, uint32_t &mk,
+                                           uint32_t j0, uint32_t j1, float h2, float k6, Acc &da, uint32_t &mk,
                                            uint32_t &cnt) {
     constexpr uint32_t U = 4;
     const float4 *p = at(sA, jb);
@@ -566,9 +576,9 @@ __device__ __forceinline__ void dens_group(const float4 *__restrict__ sA, const 
 
 // One row of the fused density loop; returns the mask of its first 32
 // candidates (those past them are counted into cnt).
-template <bool SELF>
+template <bool SELF, class Acc>
 __device__ __forceinline__ uint32_t dens_row(const float4 *__restrict__ sA, const float4 ai, uint32_t i, uint32_t j0,
-                                             uint32_t j1, float h2, float k6, DensAcc &da, uint32_t &cnt) {
+                                             uint32_t j1, float h2, float k6, Acc &da, uint32_t &cnt) {
     uint32_t mk = 0;
     const uint32_t jm = min(j1, j0 + 32u);
     uint32_t jb = j0;
@@ -601,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
         const bool fluid_i = ai.w > 0.0f;
         const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
         const float h2 = sp.h2, k6 = sp.k6;
-        DensAcc da{fluid_i ? sp.w0 : 0.0f, 0.0f, fluid_i};
+        float sum = fluid_i ? sp.w0 : 0.0f, sum_b = 0.0f;
         uint32_t cnt = 0, summ = 0;
         const float4 *__restrict__ sA = P.sA;
         // Row r finished with mask mk: count, and record it for a fluid i.
@@ -613,9 +623,8 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
                 summ |= row_summary(r, left, tail);
             }
         };
-        // Waves holding a fluid particle take the full scan (a wave running
-        // both scans would pay for both); all-boundary waves scan fluid heads.
-        if (__any(fluid_i) || P.ncount) {
+        // The full scan: every candidate tested, sums taken through selects.
+        auto full_scan = [&](auto &da) {
             for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
                 // Only the particle's own row (r = 4, uniform across the wave)
                 // can hold i itself.
@@ -623,11 +632,23 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
                                             : dens_row<false>(sA, ai, i, j0, j1, h2, k6, da, cnt);
                 close_row(r, left, j0, j1, mk);
             });
+            sum = da.sum;
+            sum_b = da.sum_b;
+        };
+        if (__all(fluid_i)) {
+            DensAcc<true> da{sum, sum_b, true};
+            full_scan(da);
+        } else if (__any(fluid_i) || P.ncount) {
+            // Waves holding a fluid particle take the full scan (a wave running
+            // both scans would pay for both).
+            DensAcc<false> da{sum, sum_b, fluid_i};
+            full_scan(da);
         } else {
             // Boundary particles sum fluid neighbours only
             // (fluid_density.rs:59-69): scan just the fluid head of each cell,
             // in cell order. (With neighbour counts requested, the full scan
             // above counts boundary neighbours too.)
+            DensAcc<false> da{sum, sum_b, false};
             for_row_windows(g, cur, [&](uint32_t, uint32_t cf, uint32_t cl, bool) {
                 uint32_t lo = P.start[cf];
                 for (uint32_t c = cf; c <= cl; ++c) {
@@ -639,9 +660,10 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
                     lo = hi;
                 }
             });
+            sum = da.sum;
         }
         if (fluid_i) P.nbs[i] = summ;
-        const float rho = fluid_i ? sp.mass * da.sum + da.sum_b : (-ai.w) * sp.w0 + da.sum;
+        const float rho = fluid_i ? sp.mass * sum + sum_b : (-ai.w) * sp.w0 + sum;
         const float q = rho / sp.rho0;
         const float q2 = q * q;
         const float q4 = q2 * q2;
