@@ -151,6 +151,72 @@ def bench_served(V, args):
     return out
 
 
+def tank_walls(V, tank):
+    """The dam-break tank [0, L0] x [0, inf) x [0, L2] as one half-space
+    provider: floor, then the x and z walls."""
+    return V.HalfspaceProvider([(1, 0.0, 1.0), (0, 0.0, 1.0), (0, tank[0], -1.0), (2, 0.0, 1.0), (2, tank[2], -1.0)])
+
+
+def bench_walls(V, args):
+    """The 1M dam break (100^3 fluid, d = 0.02 m, tank 4 x 2 x 2 m) held by
+    provider walls instead of Akinci particles. Half-space provider: the
+    device queries the walls every substep (mirror ghosts, projection,
+    coincident separation). Plane tables: the host reads the fluid back,
+    gathers both tables and uploads them once per tick (the GpuFluidStepper
+    contract); its device time and its host share are reported apart."""
+    d, tank = 0.02, (4.0, 2.0, 2.0)
+    cfg = V.dam_break_config(d)
+    data = V.dam_break(cfg, tank, (100, 100, 100), d, seed=2, walls=False)
+    nf = data.n_fluid
+    walls = tank_walls(V, tank)
+    eng = V.Engine()
+    sid = eng.add(cfg, data, reactions=False)
+    eng.set_halfspace_walls(sid, [walls])
+    tuned = eng.autotune(2)
+    eng.step(args.warmup)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(args.walls_steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    prof = eng.profile(args.profile_steps)
+    kern = {k: round(v / args.profile_steps, 4) for k, v in prof.items()}
+    wall_ms = sum(v for k, v in kern.items() if k.startswith("wall_"))
+    eng.close()
+    half = {"value": nf * args.walls_steps / wall, "unit": "particle-updates/s",
+            "ms_per_step": 1e3 * wall / args.walls_steps, "wall_kernels_ms_per_step": round(wall_ms, 4),
+            "kernel_ms_per_step": kern, "tuning": tuned}
+    # Plane tables, one gather per tick.
+    provs = V.Providers({0: walls})
+    eng = V.Engine()
+    sid = eng.add(cfg, data, reactions=False)
+    h = np.float32(cfg.smoothing_radius)
+    host_s = dev_s = 0.0
+    ticks = args.walls_table_ticks
+    for t in range(ticks + 2):
+        t0 = time.perf_counter()
+        pos, _ = eng.read_state(sid)
+        for side, fn in ((0, V.gather_wall_contacts), (1, V.gather_wall_project_contacts)):
+            eng.set_wall_tables(sid, side, *fn([0], h, pos[:nf], provs))
+        t1 = time.perf_counter()
+        eng.step(1)
+        eng.sync()
+        t2 = time.perf_counter()
+        if t >= 2:
+            host_s += t1 - t0
+            dev_s += t2 - t1
+    eng.close()
+    table = {"device_ms_per_step": 1e3 * dev_s / ticks, "host_gather_upload_ms_per_tick": 1e3 * host_s / ticks,
+             "value_device_only": nf * ticks / dev_s, "value_with_host_gather": nf * ticks / (dev_s + host_s),
+             "ticks": ticks}
+    print(f"# walls: half-space {half['ms_per_step']:.3f} ms/step ({wall_ms:.3f} ms in wall kernels), "
+          f"tables {table['device_ms_per_step']:.3f} ms/step + host {table['host_gather_upload_ms_per_tick']:.1f} "
+          f"ms/tick", file=sys.stderr, flush=True)
+    return {"workload": f"dam break 100^3 = {nf} fluid particles, d = {d} m, tank {tank[0]}x{tank[1]}x{tank[2]} m "
+                        "as half-space provider walls (floor + 4 walls), dt = 5e-4 s, one substep per tick",
+            "halfspace": half, "tables": table}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +236,9 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--no-walls", action="store_true", help="skip the provider-wall measurement")
+    ap.add_argument("--walls-steps", type=int, default=200)
+    ap.add_argument("--walls-table-ticks", type=int, default=5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device copy of BYTES (scale for the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -252,6 +321,7 @@ def main():
     eng.close()
     batched = None if args.no_batch else bench_batched(V, args)
     served = None if args.no_serve else bench_served(V, args)
+    walls = None if args.no_walls else bench_walls(V, args)
 
     cpu = None
     if not args.no_cpu:
@@ -292,6 +362,7 @@ def main():
         "tuning": tuning,
         "batched": batched,
         "served": served,
+        "walls": walls,
     }
     print(json.dumps(out), flush=True)
 

@@ -181,6 +181,50 @@ VxlSphStatus vxl_sph_scene_state_hash(VxlSphEngine *e, uint32_t scene, uint64_t 
  * convention, vxl_sph_scene_state_hash, follows sph_scale.rs). */
 VxlSphStatus vxl_sph_scene_state_hash_all(VxlSphEngine *e, uint32_t scene, uint64_t *out);
 
+/* ---- provider walls (crates/vxl-phys-core/src/interop.rs:166-224) ---- */
+
+/* A half-space wall provider: the AABB of ProviderColliders::bounds
+ * (has_bounds = 0: None, no prefilter) and its faces. Face (axis, plane,
+ * sign) is the half-space whose fluid side is sign * (x[axis] - plane) > 0;
+ * its contact for a query point p is p with p[axis] = plane and the outward
+ * normal sign * e_axis, reported while sign * (p[axis] - plane) <= 2 * probe
+ * (the reference's test doubles FloorY and Tank, fluid/src/tests.rs:11-110,
+ * are such providers). */
+typedef struct VxlSphWallProvider {
+    uint32_t id;
+    uint32_t has_bounds;
+    float lo[3];
+    float hi[3];
+} VxlSphWallProvider;
+typedef struct VxlSphWallFace {
+    uint32_t axis;      /* 0, 1, 2 */
+    float plane;
+    float sign;         /* +1 or -1 */
+    uint32_t provider;  /* index into the provider array; faces listed provider by provider */
+} VxlSphWallFace;
+
+/* Half-space providers queried on the device every substep with the CPU
+ * engine's semantics: FluidSystem::set_boundaries (fluid_access.rs:184-186)
+ * + FluidSystem::step(dt, providers) (fluid_step.rs:75-127): mirror ghosts in
+ * the density pass (fluid_density.rs:17-51, :76-121), projection and
+ * coincident separation after the integration (fluid_boundary.rs:45-131).
+ * At most 8 providers and 16 faces; nprov = 0 removes the walls. Takes effect
+ * at the next step. */
+VxlSphStatus vxl_sph_scene_set_halfspace_walls(VxlSphEngine *e, uint32_t scene, uint32_t nprov,
+                                               const VxlSphWallProvider *provs, uint32_t nface,
+                                               const VxlSphWallFace *faces);
+/* Sparse plane table of side 0 (mirror) or 1 (project), gathered by the host
+ * from any provider (gather_wall_contacts / gather_wall_project_contacts,
+ * pipeline/walls.rs:145-216) and held until replaced, with the GPU pipeline's
+ * semantics (WallStage::upload, walls.rs:104-116; wall_ghost.wgsl:61-178).
+ * ids: ne strictly ascending fluid indices; start: ne + 1 plane offsets from
+ * 0 to np; planes: np x (point xyz, outward normal xyz). */
+VxlSphStatus vxl_sph_scene_set_wall_tables(VxlSphEngine *e, uint32_t scene, uint32_t side, uint32_t ne,
+                                           const uint32_t *ids, const uint32_t *start, uint32_t np,
+                                           const float *planes);
+/* Remove a scene's walls (either kind). */
+VxlSphStatus vxl_sph_scene_clear_walls(VxlSphEngine *e, uint32_t scene);
+
 /* ---- FluidStepper drop-in (single scene; GpuFluidStepper semantics) ---- */
 VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,
                                     VxlSphStepper **out);
@@ -193,6 +237,16 @@ VxlSphStatus vxl_sph_stepper_step(VxlSphStepper *s, float dt_tick, uint32_t n_fl
 VxlSphStatus vxl_sph_stepper_reactions(VxlSphStepper *s, uint32_t cap, VxlSphReaction *out, uint32_t *count);
 VxlSphStatus vxl_sph_stepper_bounds(VxlSphStepper *s, float lo[3], float hi[3]);
 VxlSphStatus vxl_sph_stepper_read_state(VxlSphStepper *s, uint32_t n, float *pos, float *vel);
+/* FluidStepper::gather_walls (interop.rs:313-315; GpuFluidStepper, pipeline/stepper.rs:118-137):
+ * the host gathers both tables from its providers at the stepper's fluid
+ * positions (vxl_sph_stepper_read_state; before the first step, the positions
+ * it will pass) and hands them over here; they are used from the next step
+ * on. Same arguments as vxl_sph_scene_set_wall_tables. */
+VxlSphStatus vxl_sph_stepper_set_wall_tables(VxlSphStepper *s, uint32_t side, uint32_t ne, const uint32_t *ids,
+                                             const uint32_t *start, uint32_t np, const float *planes);
+/* Half-space providers for the stepper's scene (vxl_sph_scene_set_halfspace_walls). */
+VxlSphStatus vxl_sph_stepper_set_halfspace_walls(VxlSphStepper *s, uint32_t nprov, const VxlSphWallProvider *provs,
+                                                 uint32_t nface, const VxlSphWallFace *faces);
 
 /* ---- scene construction ---- */
 /* boundary::lattice for Shape::Box (boundary.rs:67-88, :208-232): writes up to cap

@@ -65,6 +65,24 @@ class OrRigidConfig(C.Structure):
     ]
 
 
+class OrWallProvider(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32),
+        ("has_bounds", C.c_uint32),
+        ("lo", C.c_float * 3),
+        ("hi", C.c_float * 3),
+    ]
+
+
+class OrWallFace(C.Structure):
+    _fields_ = [
+        ("axis", C.c_uint32),
+        ("plane", C.c_float),
+        ("sign", C.c_float),
+        ("provider", C.c_uint32),
+    ]
+
+
 _lib = None
 
 
@@ -128,6 +146,16 @@ def lib():
         L.or_box_mass_props.argtypes = [fp, C.c_float, fp, fp]
         L.or_rigid_tick.restype = None
         L.or_rigid_tick.argtypes = [C.POINTER(OrRigidConfig), C.c_uint32, C.POINTER(OrRigidBody), fp]
+        L.or_set_halfspace_walls.restype = C.c_int
+        L.or_set_halfspace_walls.argtypes = [P, C.c_uint32, C.POINTER(OrWallProvider), C.c_uint32,
+                                             C.POINTER(OrWallFace)]
+        L.or_gather_wall_tables.restype = C.c_uint32
+        L.or_gather_wall_tables.argtypes = [P, C.c_uint32, C.c_uint32, up, up, C.c_uint32, fp, up]
+        L.or_set_wall_tables.restype = C.c_int
+        L.or_set_wall_tables.argtypes = [P, C.c_uint32, C.c_uint32, up, up, C.c_uint32, fp]
+        L.or_clear_walls.argtypes = [P]
+        L.or_last_pushed.restype = C.c_uint32
+        L.or_last_pushed.argtypes = [P]
         _lib = L
     return _lib
 
@@ -330,6 +358,73 @@ class System:
         o = np.zeros(cap, np.uint32)
         n = lib().or_neighbors(self.p, i, _u(o), cap)
         return o[:min(n, cap)]
+
+    # --- provider walls ------------------------------------------------------
+
+    def set_halfspace_walls(self, providers):
+        """Half-space providers queried every substep (the CPU path). Each is a
+        dict: id, bounds ((lo), (hi)) or None, faces [(axis, plane, sign), ...]."""
+        provs, faces = _wall_structs(providers)
+        rc = lib().or_set_halfspace_walls(self.p, len(providers), provs, len(faces), faces)
+        if rc != 0:
+            raise ValueError("malformed half-space walls")
+
+    def gather_wall_tables(self, side):
+        """(ids, start, planes[np, 6]) of the current fluid positions (side 0
+        mirror, 1 project; wall_gather.rs:52-100)."""
+        npl = np.zeros(1, np.uint32)
+        ne = lib().or_gather_wall_tables(self.p, side, 0, None, None, 0, None, _u(npl))
+        ids = np.zeros(max(ne, 1), np.uint32)
+        start = np.zeros(ne + 1, np.uint32)
+        planes = np.zeros((max(int(npl[0]), 1), 6), np.float32)
+        lib().or_gather_wall_tables(self.p, side, ne, _u(ids), _u(start), int(npl[0]), _f(planes), _u(npl))
+        return ids[:ne], start, planes[:int(npl[0])]
+
+    def set_wall_tables(self, side, ids, start, planes):
+        """Plane tables held until replaced (the GPU path)."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        start = np.ascontiguousarray(start, np.uint32)
+        planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 6)
+        ids_b = ids if len(ids) else np.zeros(1, np.uint32)
+        pl_b = planes if len(planes) else np.zeros((1, 6), np.float32)
+        rc = lib().or_set_wall_tables(self.p, side, len(ids), _u(ids_b), _u(start), len(planes), _f(pl_b))
+        if rc != 0:
+            raise ValueError("malformed wall table")
+
+    def clear_walls(self):
+        lib().or_clear_walls(self.p)
+
+    def last_pushed(self):
+        return int(lib().or_last_pushed(self.p))
+
+
+def _wall_structs(providers):
+    provs = (OrWallProvider * max(len(providers), 1))()
+    flist = []
+    for k, pv in enumerate(providers):
+        provs[k].id = pv.get("id", k)
+        b = pv.get("bounds")
+        provs[k].has_bounds = 0 if b is None else 1
+        if b is not None:
+            provs[k].lo = (C.c_float * 3)(*b[0])
+            provs[k].hi = (C.c_float * 3)(*b[1])
+        for (axis, plane, sign) in pv["faces"]:
+            flist.append((axis, plane, sign, k))
+    faces = (OrWallFace * max(len(flist), 1))()
+    for q, (axis, plane, sign, k) in enumerate(flist):
+        faces[q].axis = axis
+        faces[q].plane = plane
+        faces[q].sign = sign
+        faces[q].provider = k
+    return provs, faces
+
+
+# The reference's half-space test doubles (crates/vxl-phys-fluid/src/tests.rs):
+# FloorY (:11-47): solid y < 0; Tank (:49-110): floor y = 0 and walls at
+# x, z = +-0.2, faces in the order floor, left, right, back, front.
+FLOOR_Y = dict(id=0, bounds=((-50.0, -1.0, -50.0), (50.0, 0.0, 50.0)), faces=[(1, 0.0, 1.0)])
+TANK = dict(id=0, bounds=((-50.0, -50.0, -50.0), (50.0, 50.0, 50.0)),
+            faces=[(1, 0.0, 1.0), (0, -0.2, 1.0), (0, 0.2, -1.0), (2, -0.2, 1.0), (2, 0.2, -1.0)])
 
 
 # --- rigid-box leg (rigid_oracle.h) ------------------------------------------

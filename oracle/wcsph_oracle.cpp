@@ -203,6 +203,30 @@ struct OrSystem {
     };
     std::vector<React> breact;
     Grid grid;
+    // Provider walls. wall_mode 1: half-space providers queried every substep
+    // (the CPU path: fluid_density.rs:17-51, fluid_boundary.rs:45-131);
+    // wall_mode 2: host-gathered plane tables held for the tick (the GPU path:
+    // wall_ghost.wgsl:61-178). The providers also serve the table gather.
+    struct WallProv {
+        uint32_t id;
+        bool has_bounds;
+        V3 lo, hi;
+        uint32_t face0, nface;  // its faces, in contact order
+    };
+    struct WallFace {
+        uint32_t axis;
+        float plane, sign;
+    };
+    std::vector<WallProv> provs;
+    std::vector<WallFace> faces;
+    int wall_mode = 0;
+    struct WallTable {
+        std::vector<uint32_t> ids, start;
+        std::vector<V3> pt, nrm;
+    };
+    WallTable tables[2];  // 0: mirror, 1: project
+    float skin = 0.0f;    // fluid_access.rs:47
+    uint32_t last_pushed = 0;
 };
 
 namespace {
@@ -216,6 +240,7 @@ void init_constants(OrSystem *s, const OrConfig *cfg) {
     s->k6 = 315.0f / ((64.0f * kPi) * powi_rs(h, 9));
     s->ks = 45.0f / (kPi * powi_rs(h, 6));
     s->w0 = ((s->k6 * s->h2) * s->h2) * s->h2;
+    s->skin = 0.15f * h;
     s->b_tait = ((cfg->sound_speed * cfg->sound_speed) * cfg->rest_density) / cfg->gamma_tait;
 }
 
@@ -262,8 +287,56 @@ void parallel_chunks(size_t lo, size_t hi, uint32_t threads, F &&body) {
     for (auto &t : ts) t.join();
 }
 
-// fluid_density.rs:17-70 (providers absent: no mirror ghosts). Fluid i uses
-// two accumulators; boundary i takes fluid neighbours only.
+inline float &comp(V3 &v, uint32_t a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// A half-space provider's point query (the reference's test doubles FloorY and
+// Tank, crates/vxl-phys-fluid/src/tests.rs:11-110): face (axis, plane, sign)
+// within 2 * probe of p yields the contact point p with p[axis] = plane and the
+// outward normal sign * e_axis, in face order.
+template <class F>
+void provider_contacts(const OrSystem *s, const OrSystem::WallProv &pv, V3 p, float probe, F &&emit) {
+    for (uint32_t k = 0; k < pv.nface; ++k) {
+        const OrSystem::WallFace &f = s->faces[pv.face0 + k];
+        V3 pt = p;
+        const float sdf = (comp(pt, f.axis) - f.plane) * f.sign;
+        if (sdf > 2.0f * probe) continue;
+        comp(pt, f.axis) = f.plane;
+        V3 n = v3(0.0f, 0.0f, 0.0f);
+        comp(n, f.axis) = f.sign;
+        emit(pt, n);
+    }
+}
+
+// The AABB prefilter of every provider query, margin m (fluid_density.rs:88-99,
+// fluid_boundary.rs:62-73, wall_gather.rs:63-73): true = skip the provider.
+inline bool outside(const OrSystem::WallProv &pv, V3 p, float m) {
+    if (!pv.has_bounds) return false;
+    return p.x < pv.lo.x - m || p.x > pv.hi.x + m || p.y < pv.lo.y - m || p.y > pv.hi.y + m ||
+           p.z < pv.lo.z - m || p.z > pv.hi.z + m;
+}
+
+// wall_planes_in (fluid_density.rs:76-121): the planes within the h band of
+// pi (0 < sdf < h), at most 8, providers in boundary order.
+uint32_t wall_planes(const OrSystem *s, V3 pi, V3 pt[8], V3 nrm[8]) {
+    uint32_t np = 0;
+    for (const auto &pv : s->provs) {
+        if (outside(pv, pi, s->h)) continue;
+        provider_contacts(s, pv, pi, s->h, [&](V3 point, V3 n) {
+            const float sdf = dot(sub(pi, point), n);
+            if (sdf > 0.0f && sdf < s->h && np < 8) {
+                pt[np] = point;
+                nrm[np] = n;
+                ++np;
+            }
+        });
+    }
+    return np;
+}
+
+// fluid_density.rs:17-70. Fluid i uses two accumulators; boundary i takes
+// fluid neighbours only. With half-space providers (wall_mode 1) the mirror
+// ghosts of every neighbour are summed into `sum` right after it
+// (fluid_density.rs:36-45).
 void density_pass(OrSystem *s) {
     const size_t nf = s->n_fluid, nt = s->pos.size();
     const float h2 = s->h2, k6 = s->k6, w0 = s->w0, mass = s->mass;
@@ -272,7 +345,10 @@ void density_pass(OrSystem *s) {
         for (size_t i = a; i < b; ++i) {
             float sum = w0, sum_b = 0.0f;
             uint32_t cnt = 0;
-            g.for_neighbors(s->pos, h2, i, [&](size_t j, V3, float r2) {
+            const V3 pi = s->pos[i];
+            V3 pl_pt[8], pl_n[8];
+            const uint32_t np = s->wall_mode == 1 ? wall_planes(s, pi, pl_pt, pl_n) : 0u;
+            g.for_neighbors(s->pos, h2, i, [&](size_t j, V3 d, float r2) {
                 float t = h2 - r2;
                 float w = s->fault == 1 ? k6 * ((t * t) * t) : ((k6 * t) * t) * t;
                 if (j < nf)
@@ -280,6 +356,16 @@ void density_pass(OrSystem *s) {
                 else
                     sum_b += s->pmass[j] * w;
                 ++cnt;
+                for (uint32_t k = 0; k < np; ++k) {
+                    const V3 pj = sub(pi, d);
+                    const float dn = dot(sub(pj, pl_pt[k]), pl_n[k]);
+                    const V3 gh = s->fault == 4 ? add(pj, mul(pl_n[k], 2.0f * dn)) : sub(pj, mul(pl_n[k], 2.0f * dn));
+                    const float rg2 = len2(sub(pi, gh));
+                    if (rg2 <= h2) {
+                        const float tg = h2 - rg2;
+                        sum += ((k6 * tg) * tg) * tg;
+                    }
+                }
             });
             s->dens[i] = mass * sum + sum_b;
             s->ncount[i] = cnt;
@@ -414,12 +500,129 @@ void force_pass(OrSystem *s) {
     aggregate_reactions(s);
 }
 
-// fluid_step.rs:91-127 (providers absent: boundary_pass is a no-op).
+// separate_coincident (fluid_boundary.rs:110-131): pushed particles in
+// ascending index; the later of a pair closer than 0.2 h moves along its own
+// contact normal sum by r_min - r.
+void separate_coincident(OrSystem *s, std::vector<std::pair<size_t, V3>> pushed) {
+    if (s->fault == 5) std::reverse(pushed.begin(), pushed.end());
+    const float r_min = 0.2f * s->h;
+    for (size_t a = 0; a < pushed.size(); ++a) {
+        const size_t ia = pushed[a].first;
+        const V3 na = pushed[a].second;
+        const float nl = sqrtf(len2(na));
+        if (nl <= 1e-9f) continue;
+        const V3 dir = mul(na, 1.0f / nl);
+        V3 pa = s->pos[ia];
+        for (size_t b = 0; b < a; ++b) {
+            const V3 d = sub(pa, s->pos[pushed[b].first]);
+            const float r2 = len2(d);
+            if (r2 < r_min * r_min) {
+                const float r = sqrtf(r2);
+                pa = add(pa, mul(dir, r_min - r));
+            }
+        }
+        s->pos[ia] = pa;
+    }
+}
+
+// boundary_pass (fluid_boundary.rs:45-104): every fluid particle queried at
+// its integrated position pi; each contact with sdf(pi) < 0 pushes the
+// particle to the rest line sdf = +skin (at most h per contact) and removes
+// the inward normal velocity; then separate_coincident.
+void boundary_pass(OrSystem *s) {
+    if (s->provs.empty()) return;
+    std::vector<std::pair<size_t, V3>> pushed;
+    for (size_t i = 0; i < s->n_fluid; ++i) {
+        const V3 pi = s->pos[i];
+        V3 p = pi, v = s->vel[i], n_sum = v3(0.0f, 0.0f, 0.0f);
+        bool hit = false;
+        for (const auto &pv : s->provs) {
+            if (outside(pv, pi, s->h)) continue;
+            provider_contacts(s, pv, pi, s->skin, [&](V3 point, V3 n) {
+                const float sdf = dot(sub(pi, point), n);
+                const float pen = -sdf;
+                if (pen <= 0.0f) return;
+                p = add(p, mul(n, fminf(pen + s->skin, s->h)));
+                n_sum = add(n_sum, n);
+                hit = true;
+                const float vn = dot(v, n);
+                if (vn < 0.0f) v = sub(v, mul(n, vn));
+            });
+            s->pos[i] = p;
+            s->vel[i] = v;
+        }
+        if (hit) pushed.emplace_back(i, n_sum);
+    }
+    s->last_pushed = (uint32_t)pushed.size();
+    separate_coincident(s, pushed);
+}
+
+// wall_ghost (wall_ghost.wgsl:61-136): per mirror-table entry, the ghosts of
+// every neighbour over the entry's planes whose band holds p_i now
+// (0 < sdf < h), summed apart and added once: dens += mass * add.
+void wall_ghost_tables(OrSystem *s) {
+    const auto &T = s->tables[0];
+    const float h = s->h, h2 = s->h2, k6 = s->k6;
+    for (size_t e = 0; e < T.ids.size(); ++e) {
+        const uint32_t i = T.ids[e], st = T.start[e], en = T.start[e + 1];
+        if (st >= en) continue;
+        const V3 pi = s->pos[i];
+        float acc = 0.0f;
+        s->grid.for_neighbors(s->pos, h2, i, [&](size_t, V3 d, float) {
+            const V3 pj = sub(pi, d);
+            for (uint32_t q = st; q < en; ++q) {
+                const float sdf_i = dot(sub(pi, T.pt[q]), T.nrm[q]);
+                if (sdf_i <= 0.0f || sdf_i >= h) continue;
+                const float dn = dot(sub(pj, T.pt[q]), T.nrm[q]);
+                const V3 gh = sub(pj, mul(T.nrm[q], 2.0f * dn));
+                const float rg2 = len2(sub(pi, gh));
+                if (rg2 <= h2) {
+                    const float t = h2 - rg2;
+                    acc = acc + ((k6 * t) * t) * t;
+                }
+            }
+        });
+        if (acc != 0.0f) s->dens[i] = s->dens[i] + s->mass * acc;
+    }
+}
+
+// wall_project (wall_ghost.wgsl:137-178): per project-table entry, planes in
+// order against the current (already pushed) position; no coincident
+// separation.
+void wall_project_tables(OrSystem *s) {
+    const auto &T = s->tables[1];
+    for (size_t e = 0; e < T.ids.size(); ++e) {
+        const uint32_t i = T.ids[e], st = T.start[e], en = T.start[e + 1];
+        if (st >= en) continue;
+        V3 p = s->pos[i], v = s->vel[i];
+        bool hit = false;
+        for (uint32_t q = st; q < en; ++q) {
+            const float sdf = dot(sub(p, T.pt[q]), T.nrm[q]);
+            const float pen = -sdf;
+            if (pen > 0.0f) {
+                const float push = fminf(pen + s->skin, s->h);
+                p = add(p, mul(T.nrm[q], push));
+                const float vn = dot(v, T.nrm[q]);
+                if (vn < 0.0f) v = sub(v, mul(T.nrm[q], vn));
+                hit = true;
+            }
+        }
+        if (hit) {
+            s->pos[i] = p;
+            s->vel[i] = v;
+        }
+    }
+}
+
+// fluid_step.rs:91-127; with plane tables the ghosts follow the density pass
+// and the projection follows the integration, as the GPU pipeline encodes
+// them (pipeline.rs:515-555).
 void substep(OrSystem *s, float dt) {
     const size_t nf = s->n_fluid;
     if (nf == 0) return;
     s->grid.rebuild(s->pos, s->h, s->cfg.max_bins);
     density_pass(s);
+    if (s->wall_mode == 2) wall_ghost_tables(s);
     pressure_pass(s);
     force_pass(s);
     const float vmax = (s->cfg.max_speed_frac * s->h) / dt;
@@ -432,6 +635,8 @@ void substep(OrSystem *s, float dt) {
         s->vel[i] = v;
         s->pos[i] = add(s->pos[i], mul(v, dt));
     }
+    if (s->wall_mode == 1) boundary_pass(s);
+    if (s->wall_mode == 2) wall_project_tables(s);
 }
 
 // boundary.rs:192-194: round half away from zero, at least 1.
@@ -701,6 +906,101 @@ extern "C" void or_set_fault(OrSystem *s, int fault) {
     s->fault = fault;
     s->grid.fault = fault;
 }
+
+extern "C" int or_set_halfspace_walls(OrSystem *s, uint32_t nprov, const OrWallProvider *provs,
+                                      uint32_t nface, const OrWallFace *faces) {
+    std::vector<OrSystem::WallProv> pv;
+    std::vector<OrSystem::WallFace> fc;
+    for (uint32_t k = 0; k < nprov; ++k) {
+        OrSystem::WallProv p;
+        p.id = provs[k].id;
+        p.has_bounds = provs[k].has_bounds != 0;
+        p.lo = v3(provs[k].lo[0], provs[k].lo[1], provs[k].lo[2]);
+        p.hi = v3(provs[k].hi[0], provs[k].hi[1], provs[k].hi[2]);
+        p.face0 = (uint32_t)fc.size();
+        p.nface = 0;
+        for (uint32_t f = 0; f < nface; ++f) {
+            if (faces[f].provider != k) continue;
+            if (faces[f].axis > 2) return -1;
+            fc.push_back({faces[f].axis, faces[f].plane, faces[f].sign});
+            ++p.nface;
+        }
+        pv.push_back(p);
+    }
+    if (fc.size() != nface) return -1;  // a face names no provider
+    s->provs = std::move(pv);
+    s->faces = std::move(fc);
+    s->wall_mode = s->provs.empty() ? 0 : 1;
+    return 0;
+}
+
+// gather_in (wall_gather.rs:52-100): per fluid particle at its current
+// position, the planes of every provider within the h band (side 0 mirror:
+// 0 < sdf < h; side 1 project: sdf < h), at most 8, as sparse rows.
+extern "C" uint32_t or_gather_wall_tables(const OrSystem *s, uint32_t side, uint32_t cap_e, uint32_t *ids,
+                                          uint32_t *start, uint32_t cap_p, float *planes, uint32_t *out_np) {
+    uint32_t ne = 0, np_total = 0;
+    if (start) start[0] = 0;
+    for (size_t i = 0; i < s->n_fluid; ++i) {
+        const V3 p = s->pos[i];
+        uint32_t np = 0;
+        for (const auto &pv : s->provs) {
+            if (outside(pv, p, s->h)) continue;
+            provider_contacts(s, pv, p, s->h, [&](V3 point, V3 n) {
+                const float sdf = dot(sub(p, point), n);
+                const bool keep = side == 0 ? (sdf > 0.0f && sdf < s->h) : (sdf < s->h);
+                if (keep && np < 8) {
+                    if (planes && np_total < cap_p) {
+                        float *o = planes + 6ull * np_total;
+                        o[0] = point.x; o[1] = point.y; o[2] = point.z;
+                        o[3] = n.x; o[4] = n.y; o[5] = n.z;
+                    }
+                    ++np_total;
+                    ++np;
+                }
+            });
+        }
+        if (np > 0) {
+            if (ids && ne < cap_e) {
+                ids[ne] = (uint32_t)i;
+                start[ne + 1] = np_total;
+            }
+            ++ne;
+        }
+    }
+    if (out_np) *out_np = np_total;
+    return ne;
+}
+
+extern "C" int or_set_wall_tables(OrSystem *s, uint32_t side, uint32_t ne, const uint32_t *ids,
+                                  const uint32_t *start, uint32_t np, const float *planes) {
+    if (side > 1) return -1;
+    OrSystem::WallTable t;
+    t.ids.assign(ids, ids + ne);
+    t.start.assign(start, start + ne + 1);
+    if (t.start[0] != 0 || t.start[ne] != np) return -1;
+    for (uint32_t e = 0; e < ne; ++e) {
+        if (t.ids[e] >= s->n_fluid || (e && t.ids[e] <= t.ids[e - 1]) || t.start[e + 1] < t.start[e]) return -1;
+    }
+    for (uint32_t q = 0; q < np; ++q) {
+        const float *o = planes + 6ull * q;
+        t.pt.push_back(v3(o[0], o[1], o[2]));
+        t.nrm.push_back(v3(o[3], o[4], o[5]));
+    }
+    s->tables[side] = std::move(t);
+    s->wall_mode = 2;
+    return 0;
+}
+
+extern "C" void or_clear_walls(OrSystem *s) {
+    s->provs.clear();
+    s->faces.clear();
+    s->tables[0] = OrSystem::WallTable{};
+    s->tables[1] = OrSystem::WallTable{};
+    s->wall_mode = 0;
+}
+
+extern "C" uint32_t or_last_pushed(const OrSystem *s) { return s->last_pushed; }
 
 // fluid_step.rs:75-89
 extern "C" void or_step(OrSystem *s, float dt_tick) {

@@ -77,8 +77,52 @@ void or_set_boundary_raw(OrSystem *s, uint32_t nb, const float *pos, const float
 void or_set_velocities(OrSystem *s, uint32_t n, const float *vel);
 void or_set_threads(OrSystem *s, uint32_t threads);
 /* Test-only planted mistakes (1 = poly6 reassociated, 2 = XSPH dropped,
- * 3 = neighbour order transposed); 0 restores the reference algorithm. */
+ * 3 = neighbour order transposed, 4 = mirror ghost reflected the wrong way,
+ * 5 = coincident separation moves the earlier particle); 0 restores the
+ * reference algorithm. */
 void or_set_fault(OrSystem *s, int fault);
+
+/* Provider walls (crates/vxl-phys-core/src/interop.rs:166-224). A provider is
+ * a set of axis-aligned half-space faces, as the reference's test doubles
+ * FloorY and Tank (crates/vxl-phys-fluid/src/tests.rs:11-110): for a query
+ * point p, face (axis, plane, sign) within 2 * probe of p ((p[axis] - plane) *
+ * sign <= 2 probe) yields the contact point p with p[axis] = plane and the
+ * outward normal sign * e_axis. Providers are queried in the order given (the
+ * boundaries list of set_boundaries, fluid_access.rs:184-186), a provider's
+ * faces in the order given. */
+typedef struct OrWallProvider {
+    uint32_t id;
+    uint32_t has_bounds;   /* 0: bounds() = None, the AABB prefilter passes */
+    float lo[3], hi[3];
+} OrWallProvider;
+typedef struct OrWallFace {
+    uint32_t axis;         /* 0, 1, 2 */
+    float plane;
+    float sign;            /* +1 or -1: the fluid side */
+    uint32_t provider;     /* index into the provider list */
+} OrWallFace;
+
+/* Half-space providers queried every substep, the CPU path: mirror ghosts in
+ * the density pass (fluid_density.rs:17-51, :76-121) and the projection with
+ * coincident separation after the integration (fluid_boundary.rs:45-131).
+ * Returns 0, or -1 for a face naming no provider or an axis > 2. */
+int or_set_halfspace_walls(OrSystem *s, uint32_t nprov, const OrWallProvider *provs,
+                           uint32_t nface, const OrWallFace *faces);
+/* The sparse plane tables of the current fluid positions (gather_in,
+ * wall_gather.rs:52-100; side 0 mirror, 1 project): returns the entry count;
+ * writes up to cap_e ids and start[0..ne] and up to cap_p planes (point xyz,
+ * normal xyz); *out_np = plane count. */
+uint32_t or_gather_wall_tables(const OrSystem *s, uint32_t side, uint32_t cap_e, uint32_t *ids,
+                               uint32_t *start, uint32_t cap_p, float *planes, uint32_t *out_np);
+/* Plane tables held until replaced, the GPU path (WallStage::upload,
+ * walls.rs:104-116; kernels wall_ghost.wgsl:61-178): ghosts summed apart after
+ * the density pass, projection after the integration without separation.
+ * ids strictly ascending fluid indices. Returns 0 or -1 (malformed). */
+int or_set_wall_tables(OrSystem *s, uint32_t side, uint32_t ne, const uint32_t *ids,
+                       const uint32_t *start, uint32_t np, const float *planes);
+void or_clear_walls(OrSystem *s);
+/* Particles the last boundary_pass pushed (half-space walls). */
+uint32_t or_last_pushed(const OrSystem *s);
 
 /* FluidSystem::step (fluid_step.rs:75-89). */
 void or_step(OrSystem *s, float dt_tick);

@@ -49,6 +49,32 @@ struct SceneSlot {
     };
     std::vector<Rigid> rigid;
     std::vector<BoxBody> statics;  // static Box bodies of a coupled scene (for frame dumps)
+    // Provider walls: half-space providers (faces grouped by provider) or
+    // plane tables (side 0 mirror, 1 project) with their device copies.
+    struct WallTable {
+        std::vector<uint32_t> ids, start;
+        std::vector<float4> planes;  // (point, normal) per plane
+        uint32_t *d_ids = nullptr, *d_start = nullptr;
+        float4 *d_planes = nullptr;
+        size_t cap_e = 0, cap_p = 0;  // device capacity (entries, planes)
+    };
+    uint32_t wall_mode = kWallsOff;
+    std::vector<WallProvDev> wall_prov;
+    std::vector<WallFaceDev> wall_face;
+    WallTable wall_tab[2];
+};
+
+// A half-space wall provider as the host describes it: the AABB of bounds()
+// (has_bounds = 0: none) and its axis-aligned faces.
+struct WallProvider {
+    uint32_t id = 0;
+    bool has_bounds = false;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    struct Face {
+        uint32_t axis;
+        float plane, sign;
+    };
+    std::vector<Face> faces;
 };
 
 // Rigid configuration of a coupled scene (PhysConfig defaults,
@@ -85,6 +111,19 @@ public:
     bool has_scene(uint32_t id) const;
     const SceneSlot &scene(uint32_t id) const;
     size_t scene_count() const { return scenes_.size(); }
+
+    // Provider walls of a scene (see WallSceneDev): half-space providers
+    // queried on the device every substep (FluidSystem::set_boundaries with
+    // providers, fluid_access.rs:184-186), or sparse plane tables of side 0
+    // (mirror) / 1 (project) that the host gathers from any provider and
+    // replaces once per tick (WallStage::upload, walls.rs:104-116): ids
+    // strictly ascending fluid indices, start[ne + 1] plane ranges, planes as
+    // point xyz + normal xyz. Switching a scene's wall mode takes effect at
+    // the next step; replacing a table of a resident scene is immediate.
+    void set_halfspace_walls(uint32_t id, const std::vector<WallProvider> &providers);
+    void set_wall_tables(uint32_t id, uint32_t side, const uint32_t *ids, uint32_t ne, const uint32_t *start,
+                         const float *planes, uint32_t np);
+    void clear_walls(uint32_t id);
 
     // Overwrite the boundary segment pos/vel of a resident scene (index order).
     void upload_boundary(uint32_t id, uint32_t nb, const float *pos, const float *vel);
@@ -153,6 +192,12 @@ private:
     void alloc_pool(uint32_t slots);
     void alloc_cells(uint32_t ccap);
     void free_cells();  // counts, start, fcount and the scan's tile sums
+    void free_walls();  // wall pool arrays and the wall scene table
+    void free_wall_tables(SceneSlot &sc);
+    void upload_wall_table(SceneSlot &sc, uint32_t side);
+    WallSceneDev wall_scene(const SceneSlot &sc) const;
+    void index_wall_table(uint32_t k, uint32_t side);  // went of scene position k (stream-ordered)
+    SceneSlot &slot_of(uint32_t id);
     void free_all();
     template <class T>
     T *dalloc(size_t n);
@@ -185,6 +230,9 @@ private:
     RigidDev *d_rigid_ = nullptr;
     float4 *d_rlat_ = nullptr;
     uint32_t nspans_ = 0;
+    WallSceneDev *d_walls_ = nullptr;
+    uint32_t nwall_ = 0;           // scenes with walls (wall kernels launched when > 0)
+    bool wall_halfspace_ = false;  // some scene has half-space walls (separation launched)
     std::vector<uint32_t> span_first_;  // first global span of each scene (by scene position)
     // spare state arrays: resident scenes move into them at a layout change
     float4 *spare_cA_ = nullptr, *spare_cV_ = nullptr;

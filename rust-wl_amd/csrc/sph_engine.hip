@@ -78,6 +78,8 @@ void Engine::dfree(void *p) {
 }
 
 void Engine::free_all() {
+    free_walls();
+    for (auto &sc : scenes_) free_wall_tables(sc);
     for (void *p : pool_allocs_) dfree(p);
     pool_allocs_.clear();
     dfree(d_params_);
@@ -245,10 +247,10 @@ std::vector<RigidDev> Engine::read_bodies(uint32_t id) {
 void Engine::remove_scene(uint32_t id) {
     auto it = std::find_if(scenes_.begin(), scenes_.end(), [&](const SceneSlot &s) { return s.id == id; });
     if (it == scenes_.end()) throw std::out_of_range("unknown scene");
-    if (it->resident) {
-        // Only the slot map changes; device state of the remaining scenes is
-        // moved by the next layout.
-    }
+    // Only the slot map changes; device state of the remaining scenes is moved
+    // by the next layout. Kernels may still read the scene's wall tables.
+    hip_check(hipStreamSynchronize(stream_), "sync before remove");
+    free_wall_tables(*it);
     scenes_.erase(it);
     dirty_ = true;
 }
@@ -517,6 +519,35 @@ void Engine::layout() {
     P_.nspans = nspans_;
     P_.spans = d_spans_;
     P_.react = d_react_;
+    // Provider walls: scene table, per-slot arrays, device tables.
+    free_walls();
+    nwall_ = 0;
+    wall_halfspace_ = false;
+    std::vector<WallSceneDev> walls;
+    for (auto &sc : scenes_) {
+        for (uint32_t side = 0; side < 2 && sc.wall_mode == kWallsTable; ++side) upload_wall_table(sc, side);
+        walls.push_back(wall_scene(sc));
+        nwall_ += sc.wall_mode != kWallsOff ? 1u : 0u;
+        wall_halfspace_ = wall_halfspace_ || sc.wall_mode == kWallsHalfspace;
+    }
+    if (nwall_) {
+        d_walls_ = dalloc<WallSceneDev>(walls.size());
+        hip_check(hipMemcpy(d_walls_, walls.data(), sizeof(WallSceneDev) * walls.size(), hipMemcpyHostToDevice),
+                  "walls");
+        P_.went = dalloc<uint32_t>(2ull * slots_);
+        P_.wsep = dalloc<uint64_t>(2ull * slots_);
+        P_.wn = dalloc<float4>(slots_);
+        P_.sep = dalloc<SepState>(walls.size());
+        hip_check(hipMemset(P_.sep, 0, sizeof(SepState) * walls.size()), "separation state");
+        P_.wslots = slots_;
+        if (wall_halfspace_) {
+            P_.fpos = dalloc<float4>(2ull * slots_);
+            P_.hhead = dalloc<uint32_t>(6ull * slots_);
+            P_.hnext = dalloc<uint32_t>(3ull * slots_);
+            hip_check(hipMemset(P_.hhead, 0xff, 24ull * slots_), "separation buckets");
+        }
+    }
+    P_.walls = d_walls_;
     std::vector<SceneParams> hp;
     for (auto &sc : scenes_) hp.push_back(sc.params);
     if (!hp.empty())
@@ -548,10 +579,195 @@ void Engine::layout() {
         hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
         hip_check(hipGetLastError(), "k_box");
     }
+    for (uint32_t k = 0; k < scenes_.size(); ++k)
+        if (scenes_[k].wall_mode == kWallsTable) {
+            index_wall_table(k, 0);
+            index_wall_table(k, 1);
+        }
     hip_check(hipStreamSynchronize(stream_), "layout sync");
     substeps_done_ = 0;  // substep index of the new layout (tick phase, replay)
     ++layouts_;
     dirty_ = false;
+}
+
+SceneSlot &Engine::slot_of(uint32_t id) {
+    for (auto &s : scenes_)
+        if (s.id == id) return s;
+    throw std::out_of_range("unknown scene");
+}
+
+void Engine::free_walls() {
+    dfree(d_walls_);
+    dfree(P_.went);
+    dfree(P_.wsep);
+    dfree(P_.wn);
+    dfree(P_.sep);
+    dfree(P_.fpos);
+    dfree(P_.hhead);
+    dfree(P_.hnext);
+    d_walls_ = nullptr;
+    P_.walls = nullptr;
+    P_.went = nullptr;
+    P_.wsep = nullptr;
+    P_.wn = nullptr;
+    P_.sep = nullptr;
+    P_.fpos = nullptr;
+    P_.hhead = nullptr;
+    P_.hnext = nullptr;
+    P_.wslots = 0;
+}
+
+void Engine::free_wall_tables(SceneSlot &sc) {
+    for (auto &t : sc.wall_tab) {
+        dfree(t.d_ids);
+        dfree(t.d_start);
+        dfree(t.d_planes);
+        t.d_ids = t.d_start = nullptr;
+        t.d_planes = nullptr;
+        t.cap_e = t.cap_p = 0;
+    }
+}
+
+// Device copy of one table (grown geometrically; the caller has synchronised
+// the stream, so no kernel reads the old buffers).
+void Engine::upload_wall_table(SceneSlot &sc, uint32_t side) {
+    auto &t = sc.wall_tab[side];
+    if (t.start.empty()) t.start.assign(1, 0u);
+    const size_t ne = t.ids.size(), np = t.planes.size() / 2;
+    if (ne > t.cap_e || !t.d_start) {
+        dfree(t.d_ids);
+        dfree(t.d_start);
+        t.cap_e = std::max<size_t>(ne, t.cap_e + t.cap_e / 2);
+        t.d_ids = dalloc<uint32_t>(t.cap_e);
+        t.d_start = dalloc<uint32_t>(t.cap_e + 1);
+    }
+    if (np > t.cap_p || !t.d_planes) {
+        dfree(t.d_planes);
+        t.cap_p = std::max<size_t>(np, t.cap_p + t.cap_p / 2);
+        t.d_planes = dalloc<float4>(2 * std::max<size_t>(t.cap_p, 1));
+    }
+    if (ne) hip_check(hipMemcpy(t.d_ids, t.ids.data(), 4 * ne, hipMemcpyHostToDevice), "wall ids");
+    hip_check(hipMemcpy(t.d_start, t.start.data(), 4 * (ne + 1), hipMemcpyHostToDevice), "wall start");
+    if (np) hip_check(hipMemcpy(t.d_planes, t.planes.data(), 32 * np, hipMemcpyHostToDevice), "wall planes");
+}
+
+WallSceneDev Engine::wall_scene(const SceneSlot &sc) const {
+    WallSceneDev w;
+    std::memset(&w, 0, sizeof(w));
+    w.mode = sc.wall_mode;
+    if (sc.wall_mode == kWallsHalfspace) {
+        w.nprov = (uint32_t)sc.wall_prov.size();
+        std::copy(sc.wall_prov.begin(), sc.wall_prov.end(), w.prov);
+        std::copy(sc.wall_face.begin(), sc.wall_face.end(), w.face);
+    } else if (sc.wall_mode == kWallsTable) {
+        for (uint32_t side = 0; side < 2; ++side) {
+            w.ne[side] = (uint32_t)sc.wall_tab[side].ids.size();
+            w.tstart[side] = sc.wall_tab[side].d_start;
+            w.tplane[side] = (const float *)sc.wall_tab[side].d_planes;
+        }
+    }
+    return w;
+}
+
+void Engine::index_wall_table(uint32_t k, uint32_t side) {
+    const SceneSlot &sc = scenes_[k];
+    const uint32_t nf = sc.params.nf, ne = (uint32_t)sc.wall_tab[side].ids.size();
+    const uint32_t gc = std::max<uint32_t>(1, std::min<uint32_t>((nf + kBlock - 1) / kBlock, 4096));
+    hipLaunchKernelGGL(k_wall_index_clear, dim3(gc), dim3(kBlock), 0, stream_, P_, k, side);
+    if (ne) {
+        const uint32_t gs = std::min<uint32_t>((ne + kBlock - 1) / kBlock, 4096);
+        hipLaunchKernelGGL(k_wall_index_set, dim3(gs), dim3(kBlock), 0, stream_, P_, k, side,
+                           (const uint32_t *)sc.wall_tab[side].d_ids, ne);
+    }
+    hip_check(hipGetLastError(), "wall index");
+}
+
+void Engine::set_halfspace_walls(uint32_t id, const std::vector<WallProvider> &providers) {
+    SceneSlot &sc = slot_of(id);
+    if (providers.size() > kWallMaxProviders) throw std::invalid_argument("at most 8 wall providers per scene");
+    std::vector<WallProvDev> prov;
+    std::vector<WallFaceDev> face;
+    for (const auto &p : providers) {
+        WallProvDev d{};
+        d.has_bounds = p.has_bounds ? 1u : 0u;
+        for (int a = 0; a < 3; ++a) {
+            d.lo[a] = p.lo[a];
+            d.hi[a] = p.hi[a];
+        }
+        d.face0 = (uint32_t)face.size();
+        d.nface = (uint32_t)p.faces.size();
+        for (const auto &f : p.faces) {
+            if (f.axis > 2) throw std::invalid_argument("wall face axis must be 0, 1 or 2");
+            if (f.sign != 1.0f && f.sign != -1.0f) throw std::invalid_argument("wall face sign must be +1 or -1");
+            if (!std::isfinite(f.plane)) throw std::invalid_argument("wall face plane must be finite");
+            face.push_back(WallFaceDev{f.axis, f.plane, f.sign, 0u});
+        }
+        prov.push_back(d);
+    }
+    if (face.size() > kWallMaxFaces) throw std::invalid_argument("at most 16 wall faces per scene");
+    hip_check(hipStreamSynchronize(stream_), "sync before walls");
+    free_wall_tables(sc);
+    for (auto &t : sc.wall_tab) t = SceneSlot::WallTable{};
+    sc.wall_prov = std::move(prov);
+    sc.wall_face = std::move(face);
+    sc.wall_mode = sc.wall_prov.empty() ? kWallsOff : kWallsHalfspace;
+    dirty_ = true;
+}
+
+void Engine::set_wall_tables(uint32_t id, uint32_t side, const uint32_t *ids, uint32_t ne, const uint32_t *start,
+                             const float *planes, uint32_t np) {
+    SceneSlot &sc = slot_of(id);
+    if (side > 1) throw std::invalid_argument("wall table side must be 0 (mirror) or 1 (project)");
+    if (ne && (!ids || !start)) throw std::invalid_argument("null wall table");
+    if (np && !planes) throw std::invalid_argument("null wall planes");
+    if ((start ? start[0] : 0u) != 0u || (start ? start[ne] : 0u) != np)
+        throw std::invalid_argument("wall table start must run from 0 to the plane count");
+    for (uint32_t e = 0; e < ne; ++e) {
+        if (ids[e] >= sc.params.nf) throw std::invalid_argument("wall table id is not a fluid index");
+        if (e && ids[e] <= ids[e - 1]) throw std::invalid_argument("wall table ids must be strictly ascending");
+        if (start[e + 1] < start[e]) throw std::invalid_argument("wall table start must not decrease");
+    }
+    for (size_t q = 0; q < 6ull * np; ++q)
+        if (!std::isfinite(planes[q])) throw std::invalid_argument("wall planes must be finite");
+    SceneSlot::WallTable t;
+    t.ids.assign(ids, ids + ne);
+    t.start.assign(1, 0u);
+    if (ne) t.start.assign(start, start + ne + 1);
+    t.planes.resize(2ull * np);
+    for (uint32_t q = 0; q < np; ++q) {
+        const float *o = planes + 6ull * q;
+        t.planes[2 * q] = make_float4(o[0], o[1], o[2], 0.0f);
+        t.planes[2 * q + 1] = make_float4(o[3], o[4], o[5], 0.0f);
+    }
+    hip_check(hipStreamSynchronize(stream_), "sync before wall table");
+    auto &dst = sc.wall_tab[side];
+    std::swap(dst.ids, t.ids);
+    std::swap(dst.start, t.start);
+    std::swap(dst.planes, t.planes);
+    if (sc.wall_mode != kWallsTable) {
+        sc.wall_prov.clear();
+        sc.wall_face.clear();
+        sc.wall_mode = kWallsTable;
+        dirty_ = true;  // the layout uploads and indexes both tables
+        return;
+    }
+    if (dirty_ || !sc.resident || !d_walls_) return;
+    upload_wall_table(sc, side);
+    const uint32_t k = (uint32_t)(&sc - scenes_.data());
+    const WallSceneDev w = wall_scene(sc);
+    hip_check(hipMemcpy(d_walls_ + k, &w, sizeof(w), hipMemcpyHostToDevice), "wall scene");
+    index_wall_table(k, side);
+}
+
+void Engine::clear_walls(uint32_t id) {
+    SceneSlot &sc = slot_of(id);
+    hip_check(hipStreamSynchronize(stream_), "sync before walls");
+    free_wall_tables(sc);
+    for (auto &t : sc.wall_tab) t = SceneSlot::WallTable{};
+    sc.wall_prov.clear();
+    sc.wall_face.clear();
+    if (sc.wall_mode != kWallsOff) dirty_ = true;
+    sc.wall_mode = kWallsOff;
 }
 
 void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
@@ -577,11 +793,30 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     M("canon_gather");
     hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("density");
+    if (nwall_) {
+        hipLaunchKernelGGL(k_wall_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        M("wall_density");
+    }
     if (tuning_.force_kernel == kForceScan)
         hipLaunchKernelGGL(k_force_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     else
         hipLaunchKernelGGL(k_force_t<1>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("force");
+    if (nwall_) {
+        hipLaunchKernelGGL(k_wall_project, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        M("wall_project");
+        if (wall_halfspace_) {
+            const uint32_t gs = std::min<uint32_t>(nb, 2048);
+            hipLaunchKernelGGL(k_sep_begin, dim3(gs), dim3(kBlock), 0, stream_, P_);
+            for (uint32_t m = 0; m < kSepIters; ++m)
+                hipLaunchKernelGGL(k_sep_round, dim3(gs), dim3(kBlock), 0, stream_, P_, m);
+            hipLaunchKernelGGL(k_sep_finish, dim3(gs), dim3(kBlock), 0, stream_, P_);
+            hipLaunchKernelGGL(k_wall_separate, dim3(P_.nscenes), dim3(kBlock), 0, stream_, P_);
+            M("wall_separate");
+        }
+        hipLaunchKernelGGL(k_wall_box, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        M("wall_box");
+    }
     hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
     M("box_reduce");
     if (nspans_) {

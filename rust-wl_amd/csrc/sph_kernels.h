@@ -57,6 +57,16 @@ struct Pool {
     RigidDev *rigid;
     uint32_t nrigid;
     const float4 *rlat;
+    // provider walls (walls = null: no scene of the pool has walls)
+    const WallSceneDev *walls;  // [nscenes]
+    uint32_t *went;   // [2][nblocks * kBlock] table entry of slot pbase + fluid index (kWallNone: none)
+    SepState *sep;    // [nscenes] coincident separation of this substep
+    uint64_t *wsep;   // [2 * slots] (fluid index << 32 | slot) of the pushed ones, from 2 * pbase
+    float4 *wn;       // [slots] contact normal sum of a pushed particle (grid order)
+    float4 *fpos;     // [2][slots] positions of the pushed ones (fixed-point rounds), from pbase
+    uint32_t *hhead;  // [3][2 * slots] spatial hash buckets of the pushed ones, from 2 * pbase
+    uint32_t *hnext;  // [3][slots] bucket chains, from pbase
+    uint32_t wslots;  // slots of the wall arrays (their per-copy stride)
 };
 
 __global__ void k_box(Pool P);
@@ -73,6 +83,25 @@ __global__ void k_density(Pool P);
 template <uint32_t WALK>
 __global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks
 __global__ void k_bforce(Pool P);
+// Provider walls (scenes with WallSceneDev::mode != kWallsOff):
+// k_wall_density after k_density (mirror ghosts, then Tait again),
+// k_wall_project after the force pass (projection of the integrated state),
+// k_wall_separate (one block per scene: the serial coincident separation
+// where the parallel rounds did not converge), k_wall_box (the blocks'
+// partial boxes again).
+__global__ void k_wall_density(Pool P);
+__global__ void k_wall_project(Pool P);
+__global__ void k_wall_separate(Pool P);
+__global__ void k_wall_box(Pool P);
+// Parallel coincident separation (between k_wall_project and k_wall_separate):
+// k_sep_begin, k_sep_round(m) for m < kSepIters, k_sep_finish.
+__global__ void k_sep_begin(Pool P);
+__global__ void k_sep_round(Pool P, uint32_t m);
+__global__ void k_sep_finish(Pool P);
+// Table entry index of scene s, side: every fluid index of the scene to
+// kWallNone, then went[pbase + ids[e]] = e.
+__global__ void k_wall_index_clear(Pool P, uint32_t s, uint32_t side);
+__global__ void k_wall_index_set(Pool P, uint32_t s, uint32_t side, const uint32_t *ids, uint32_t ne);
 __global__ void k_reactions(Pool P);
 // Rigid leg, once per tick: integrate the bodies (before the first substep),
 // rebuild their particles from the pose and the scenes' AABBs, keep the

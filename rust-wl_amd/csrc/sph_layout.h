@@ -88,6 +88,61 @@ constexpr uint32_t kReactDynamic = 2;  // only spans of dynamic rigid bodies (co
 
 constexpr uint32_t kReactMaxNeighbours = 96;  // fluid neighbours per boundary particle
 
+// Provider walls of a scene (crates/vxl-phys-core/src/interop.rs:166-224).
+//
+// kWallsHalfspace: providers made of axis-aligned half-space faces, queried
+// on the device every substep with the CPU engine's semantics: mirror ghosts
+// summed with each neighbour (fluid_density.rs:17-51, :76-121), projection
+// of the integrated position and coincident separation
+// (fluid_boundary.rs:45-131).
+// kWallsTable: sparse plane tables gathered by the host from any provider
+// once per tick and held until replaced, with the GPU pipeline's semantics
+// (wall_ghost.wgsl:61-178): ghosts summed apart and added after the density
+// pass, projection against the pushed position, no separation.
+constexpr uint32_t kWallsOff = 0;
+constexpr uint32_t kWallsHalfspace = 1;
+constexpr uint32_t kWallsTable = 2;
+constexpr uint32_t kWallMaxProviders = 8;
+constexpr uint32_t kWallMaxFaces = 16;
+constexpr uint32_t kWallMirrorPlanes = 8;  // planes kept per particle (fluid_density.rs:108)
+constexpr uint32_t kWallNone = 0xffffffffu;
+// Coincident separation of a half-space scene: rounds of the parallel
+// fixed-point iteration before the serial fallback.
+constexpr uint32_t kSepIters = 6;
+
+// Per-scene state of the coincident separation of one substep.
+struct SepState {
+    uint32_t k;                  // particles the projection pushed
+    uint32_t changed[kSepIters]; // round m moved some particle (0: converged)
+    uint32_t pad;
+};
+
+struct WallFaceDev {
+    uint32_t axis;   // 0, 1, 2
+    float plane;
+    float sign;      // outward normal sign * e_axis (the fluid side)
+    uint32_t pad;
+};
+
+struct WallProvDev {
+    float lo[3], hi[3];  // bounds() for the AABB prefilter
+    uint32_t has_bounds;
+    uint32_t face0, nface;  // faces of this provider, in contact order
+    uint32_t pad[3];
+};
+
+struct WallSceneDev {
+    uint32_t mode;       // kWallsOff | kWallsHalfspace | kWallsTable
+    uint32_t nprov;
+    uint32_t ne[2];      // table entries (0 mirror, 1 project)
+    WallProvDev prov[kWallMaxProviders];
+    WallFaceDev face[kWallMaxFaces];
+    // Plane tables of the scene (kWallsTable): start[ne + 1] plane ranges and
+    // planes as 8 floats each (point xyz, 0, outward normal xyz, 0).
+    const uint32_t *tstart[2];
+    const float *tplane[2];
+};
+
 // Device-written per-substep grid of one scene (grid_box rule, core/grid.rs:34-51).
 struct SceneGrid {
     float gmin[3];

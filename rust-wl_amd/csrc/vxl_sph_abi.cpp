@@ -28,12 +28,55 @@ struct VxlSphStepper {
     uint32_t n = 0, nf = 0;
     std::vector<Span> spans;
     std::vector<VxlSphReaction> reactions;
+    // Walls handed over before the first step (the scene is created by it).
+    bool pend_halfspace = false;
+    std::vector<WallProvider> pend_prov;
+    struct PendTable {
+        bool set = false;
+        std::vector<uint32_t> ids, start;
+        std::vector<float> planes;
+    } pend_tab[2];
     VxlSphStepper(int dev, const SceneConfig &c) : eng(dev), cfg(c) {}
+    void apply_pending_walls() {
+        if (pend_halfspace) eng.set_halfspace_walls(scene, pend_prov);
+        for (uint32_t side = 0; side < 2; ++side) {
+            const PendTable &t = pend_tab[side];
+            if (t.set)
+                eng.set_wall_tables(scene, side, t.ids.data(), (uint32_t)t.ids.size(), t.start.data(), t.planes.data(),
+                                    (uint32_t)(t.planes.size() / 6));
+        }
+        pend_halfspace = false;
+        pend_prov.clear();
+        pend_tab[0] = pend_tab[1] = PendTable{};
+    }
 };
 
 namespace {
 
 thread_local std::string g_last_error;
+
+// Provider array + faces (listed provider by provider) -> engine providers.
+std::vector<WallProvider> to_wall_providers(uint32_t nprov, const VxlSphWallProvider *provs, uint32_t nface,
+                                            const VxlSphWallFace *faces) {
+    std::vector<WallProvider> out(nprov);
+    uint32_t last = 0;
+    for (uint32_t k = 0; k < nprov; ++k) {
+        out[k].id = provs[k].id;
+        out[k].has_bounds = provs[k].has_bounds != 0;
+        for (int a = 0; a < 3; ++a) {
+            out[k].lo[a] = provs[k].lo[a];
+            out[k].hi[a] = provs[k].hi[a];
+        }
+    }
+    for (uint32_t f = 0; f < nface; ++f) {
+        const uint32_t k = faces[f].provider;
+        if (k >= nprov) throw std::invalid_argument("wall face names no provider");
+        if (k < last) throw std::invalid_argument("wall faces must be listed provider by provider");
+        last = k;
+        out[k].faces.push_back(WallProvider::Face{faces[f].axis, faces[f].plane, faces[f].sign});
+    }
+    return out;
+}
 
 VxlSphStatus fail(VxlSphStatus s, const std::string &msg) {
     g_last_error = msg;
@@ -305,6 +348,25 @@ VxlSphStatus vxl_sph_scene_upload_boundary(VxlSphEngine *e, uint32_t scene, uint
     return guard([&] { e->eng.upload_boundary(scene, nb, pos, vel); });
 }
 
+VxlSphStatus vxl_sph_scene_set_halfspace_walls(VxlSphEngine *e, uint32_t scene, uint32_t nprov,
+                                               const VxlSphWallProvider *provs, uint32_t nface,
+                                               const VxlSphWallFace *faces) {
+    if (!e || (nprov && !provs) || (nface && !faces)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] { e->eng.set_halfspace_walls(scene, to_wall_providers(nprov, provs, nface, faces)); });
+}
+
+VxlSphStatus vxl_sph_scene_set_wall_tables(VxlSphEngine *e, uint32_t scene, uint32_t side, uint32_t ne,
+                                           const uint32_t *ids, const uint32_t *start, uint32_t np,
+                                           const float *planes) {
+    if (!e || !start || (ne && !ids) || (np && !planes)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] { e->eng.set_wall_tables(scene, side, ids, ne, start, planes, np); });
+}
+
+VxlSphStatus vxl_sph_scene_clear_walls(VxlSphEngine *e, uint32_t scene) {
+    if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
+    return guard([&] { e->eng.clear_walls(scene); });
+}
+
 VxlSphStatus vxl_sph_scene_read_state(VxlSphEngine *e, uint32_t scene, uint32_t n, float *pos, float *vel) {
     if (!e) return fail(VXL_SPH_NULL_POINTER, "engine is null");
     return guard([&] {
@@ -410,6 +472,12 @@ VxlSphStatus vxl_sph_stepper_step(VxlSphStepper *s, float dt_tick, uint32_t n_fl
             s->cfg.dt_tick = dt_tick;
             s->spans = to_spans(spans, nspans);
             s->scene = s->eng.add_scene(s->cfg, n, n_fluid, pos, vel, pmass, s->spans.data(), nspans);
+            try {
+                s->apply_pending_walls();
+            } catch (...) {
+                s->eng.remove_scene(s->scene);
+                throw;
+            }
             s->n = n;
             s->nf = n_fluid;
             s->started = true;
@@ -452,6 +520,40 @@ VxlSphStatus vxl_sph_stepper_read_state(VxlSphStepper *s, uint32_t n, float *pos
     return guard([&] {
         if (n != s->n) throw std::length_error("n differs from the stepper's particle count");
         s->eng.read_state(s->scene, pos, vel);
+    });
+}
+
+VxlSphStatus vxl_sph_stepper_set_wall_tables(VxlSphStepper *s, uint32_t side, uint32_t ne, const uint32_t *ids,
+                                             const uint32_t *start, uint32_t np, const float *planes) {
+    if (!s || !start || (ne && !ids) || (np && !planes)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    if (side > 1) return fail(VXL_SPH_INVALID_ARGUMENT, "wall table side must be 0 (mirror) or 1 (project)");
+    return guard([&] {
+        if (s->started) {
+            s->eng.set_wall_tables(s->scene, side, ids, ne, start, planes, np);
+            return;
+        }
+        auto &t = s->pend_tab[side];
+        t.set = true;
+        t.ids.assign(ids, ids + ne);
+        t.start.assign(start, start + ne + 1);
+        t.planes.assign(planes, planes + 6ull * np);
+        s->pend_halfspace = false;
+        s->pend_prov.clear();
+    });
+}
+
+VxlSphStatus vxl_sph_stepper_set_halfspace_walls(VxlSphStepper *s, uint32_t nprov, const VxlSphWallProvider *provs,
+                                                 uint32_t nface, const VxlSphWallFace *faces) {
+    if (!s || (nprov && !provs) || (nface && !faces)) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        std::vector<WallProvider> pv = to_wall_providers(nprov, provs, nface, faces);
+        if (s->started) {
+            s->eng.set_halfspace_walls(s->scene, pv);
+            return;
+        }
+        s->pend_halfspace = true;
+        s->pend_prov = std::move(pv);
+        s->pend_tab[0] = s->pend_tab[1] = VxlSphStepper::PendTable{};
     });
 }
 

@@ -100,6 +100,14 @@ class RigidBox(C.Structure):
     _fields_ = [("box", BoxBody), ("density", C.c_float)]
 
 
+class WallProvider(C.Structure):
+    _fields_ = [("id", C.c_uint32), ("has_bounds", C.c_uint32), ("lo", C.c_float * 3), ("hi", C.c_float * 3)]
+
+
+class WallFace(C.Structure):
+    _fields_ = [("axis", C.c_uint32), ("plane", C.c_float), ("sign", C.c_float), ("provider", C.c_uint32)]
+
+
 class DamBreakSpec(C.Structure):
     _fields_ = [("hdr", Header), ("tank", C.c_float * 3), ("block", C.c_uint32 * 3), ("d", C.c_float),
                 ("jitter", C.c_float), ("seed", C.c_uint64), ("walls", C.c_uint32), ("reserved", C.c_uint32)]
@@ -153,6 +161,8 @@ EXPORTS = (
     "vxl_sph_request_bodies", "vxl_sph_scene_state_hash_all", "vxl_sph_vxld_open", "vxl_sph_vxld_frame",
     "vxl_sph_vxld_frame_raw", "vxl_sph_vxld_close",
     "vxl_sph_sample_batch", "vxl_sph_sample_requests", "vxl_sph_request_config", "vxl_sph_serve",
+    "vxl_sph_scene_set_halfspace_walls", "vxl_sph_scene_set_wall_tables", "vxl_sph_scene_clear_walls",
+    "vxl_sph_stepper_set_wall_tables", "vxl_sph_stepper_set_halfspace_walls",
 )
 
 
@@ -266,6 +276,18 @@ def lib():
     L.vxl_sph_serve.restype = st
     L.vxl_sph_serve.argtypes = [C.POINTER(ServeConfig), C.c_uint32, C.POINTER(Request), C.POINTER(ServeStats),
                                 C.POINTER(ServeResult)]
+    L.vxl_sph_scene_set_halfspace_walls.restype = st
+    L.vxl_sph_scene_set_halfspace_walls.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(WallProvider), C.c_uint32,
+                                                    C.POINTER(WallFace)]
+    L.vxl_sph_scene_set_wall_tables.restype = st
+    L.vxl_sph_scene_set_wall_tables.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32, up, up, C.c_uint32, fp]
+    L.vxl_sph_scene_clear_walls.restype = st
+    L.vxl_sph_scene_clear_walls.argtypes = [P, C.c_uint32]
+    L.vxl_sph_stepper_set_wall_tables.restype = st
+    L.vxl_sph_stepper_set_wall_tables.argtypes = [P, C.c_uint32, C.c_uint32, up, up, C.c_uint32, fp]
+    L.vxl_sph_stepper_set_halfspace_walls.restype = st
+    L.vxl_sph_stepper_set_halfspace_walls.argtypes = [P, C.c_uint32, C.POINTER(WallProvider), C.c_uint32,
+                                                      C.POINTER(WallFace)]
     L.vxl_sph_fluid_constants.restype = st
     L.vxl_sph_fluid_constants.argtypes = [C.POINTER(FluidConfig), fp]
     L.vxl_sph_dam_break.restype = st
@@ -609,6 +631,21 @@ class Engine:
         _check(lib().vxl_sph_scene_state_hash(self.h, sid, C.byref(v)))
         return v.value
 
+    def set_halfspace_walls(self, sid, providers):
+        """Half-space providers (HalfspaceProvider) queried on the device every
+        substep with the CPU engine's semantics; [] removes them."""
+        provs, np_, faces, nf = _wall_arrays(providers)
+        _check(lib().vxl_sph_scene_set_halfspace_walls(self.h, sid, np_, provs, nf, faces))
+
+    def set_wall_tables(self, sid, side, ids, start, planes):
+        """Plane table of side 0 (mirror) / 1 (project), as gather_wall_contacts
+        returns it; held until replaced."""
+        t = _Table(ids, start, planes)
+        _check(lib().vxl_sph_scene_set_wall_tables(self.h, sid, side, *t.args()))
+
+    def clear_walls(self, sid):
+        _check(lib().vxl_sph_scene_clear_walls(self.h, sid))
+
 
 class FluidStepper:
     """FluidStepper drop-in with GpuFluidStepper semantics (pipeline/stepper.rs:70-147)."""
@@ -632,9 +669,42 @@ class FluidStepper:
         pos = np.ascontiguousarray(pos, np.float32)
         vel = np.ascontiguousarray(vel, np.float32)
         pmass = np.ascontiguousarray(pmass, np.float32)
+        if getattr(self, "_gather", None) is not None:
+            # gather_walls before the first step: the positions this step starts from.
+            h, providers = self._gather
+            self._gather = None
+            self._upload_walls(h, providers, pos.reshape(-1, 3)[:n_fluid])
         self._n = len(pmass)
+        self._nf = n_fluid
         _check(lib().vxl_sph_stepper_step(self.h, C.c_float(dt_tick), n_fluid, len(pmass), _f(pos), _f(vel),
                                           _f(pmass), _spans_arr(spans), len(spans)))
+
+    def set_boundaries(self, ids):
+        """Provider ids whose walls gather_walls collects (GpuFluidStepper::new's
+        `boundaries`, pipeline/stepper.rs:34-56)."""
+        self._boundaries = [int(b) for b in ids]
+
+    def gather_walls(self, h, providers):
+        """FluidStepper::gather_walls (interop.rs:313-315; pipeline/stepper.rs:118-137):
+        both plane tables from the stepper's current fluid positions; before the
+        first step, from the positions that step starts from."""
+        if not getattr(self, "_boundaries", None):
+            return
+        if not hasattr(self, "_n"):
+            self._gather = (h, providers)
+            return
+        pos, _ = self.read_state()
+        self._upload_walls(h, providers, pos[:self._nf])
+
+    def _upload_walls(self, h, providers, ps):
+        for side, fn in ((0, gather_wall_contacts), (1, gather_wall_project_contacts)):
+            t = _Table(*fn(self._boundaries, h, ps, providers))
+            _check(lib().vxl_sph_stepper_set_wall_tables(self.h, side, *t.args()))
+
+    def set_halfspace_walls(self, providers):
+        """Half-space providers queried on the device every substep (CPU semantics)."""
+        provs, np_, faces, nf = _wall_arrays(providers)
+        _check(lib().vxl_sph_stepper_set_halfspace_walls(self.h, np_, provs, nf, faces))
 
     def reactions(self):
         cnt = C.c_uint32()
@@ -655,6 +725,189 @@ class FluidStepper:
         vel = np.zeros((self._n, 3), np.float32)
         _check(lib().vxl_sph_stepper_read_state(self.h, self._n, _f(pos), _f(vel)))
         return pos, vel
+
+
+# --- provider walls (host side) ----------------------------------------------
+
+class HalfspaceProvider:
+    """A wall provider made of axis-aligned half-space faces, with the
+    ProviderColliders point queries (interop.rs:166-224): face (axis, plane,
+    sign) has its fluid side at sign * (x[axis] - plane) > 0; a query point p
+    within 2 * probe of it gets the contact (p with p[axis] = plane, normal
+    sign * e_axis). The reference's test doubles FloorY and Tank
+    (fluid/src/tests.rs:11-110) are such providers; the device evaluates the
+    same queries itself when the provider is handed to set_halfspace_walls."""
+
+    def __init__(self, faces, bounds=None, id=0):
+        self.id = int(id)
+        self.faces = [(int(a), np.float32(p), np.float32(sg)) for (a, p, sg) in faces]
+        self._bounds = None if bounds is None else (np.asarray(bounds[0], np.float32),
+                                                    np.asarray(bounds[1], np.float32))
+
+    def bounds(self, id=None):
+        return self._bounds
+
+    def contacts_point(self, id, p, skin):
+        """[(point, normal)] for one query point (the reference's per-point query)."""
+        p = np.asarray(p, np.float32)
+        out = []
+        for (axis, plane, sign) in self.faces:
+            if (p[axis] - plane) * sign > np.float32(2.0) * np.float32(skin):
+                continue
+            pt = p.copy()
+            pt[axis] = plane
+            n = np.zeros(3, np.float32)
+            n[axis] = sign
+            out.append((pt, n))
+        return out
+
+    def contacts_point_boundary(self, id, p, skin):
+        return self.contacts_point(id, p, skin)
+
+    def contacts_points(self, id, ps, probe):
+        """The same query for many points at once: per face in order,
+        (mask, contact points[n, 3], normal)."""
+        for (axis, plane, sign) in self.faces:
+            mask = ~((ps[:, axis] - plane) * sign > np.float32(2.0) * np.float32(probe))
+            pt = ps.copy()
+            pt[:, axis] = plane
+            n = np.zeros(3, np.float32)
+            n[axis] = sign
+            yield mask, pt, n
+
+
+FLOOR_Y = HalfspaceProvider([(1, 0.0, 1.0)], bounds=((-50.0, -1.0, -50.0), (50.0, 0.0, 50.0)))
+TANK = HalfspaceProvider([(1, 0.0, 1.0), (0, -0.2, 1.0), (0, 0.2, -1.0), (2, -0.2, 1.0), (2, 0.2, -1.0)],
+                         bounds=((-50.0, -50.0, -50.0), (50.0, 50.0, 50.0)))
+
+
+class Providers:
+    """A ProviderColliders set: provider objects by id."""
+
+    def __init__(self, by_id):
+        self.by_id = dict(by_id)
+
+    def bounds(self, id):
+        pv = self.by_id.get(id)
+        return None if pv is None else pv.bounds(id)
+
+    def contacts_point(self, id, p, skin):
+        pv = self.by_id.get(id)
+        return None if pv is None else pv.contacts_point(id, p, skin)
+
+    def contacts_point_boundary(self, id, p, skin):
+        pv = self.by_id.get(id)
+        return None if pv is None else pv.contacts_point_boundary(id, p, skin)
+
+
+def _gather(boundaries, h, ps, providers, project):
+    # gather_in (pipeline/walls.rs:168-216): per particle, providers in order,
+    # contacts in order; keep 0 < sdf < h (mirror) or sdf < h (project), at
+    # most 8 per particle. Providers with a batched query (contacts_points)
+    # are evaluated for all particles at once in the same f32 operations.
+    ps = np.ascontiguousarray(ps, np.float32).reshape(-1, 3)
+    n = len(ps)
+    h = np.float32(h)
+    count = np.zeros(n, np.int64)
+    rows = []  # (particle, order, point, normal)
+    order = 0
+    for bid in boundaries:
+        bb = providers.bounds(bid)
+        live = np.ones(n, bool)
+        if bb is not None:
+            lo, hi = (np.asarray(b, np.float32) for b in bb)
+            for a in range(3):
+                live &= ~((ps[:, a] < lo[a] - h) | (ps[:, a] > hi[a] + h))
+        pv = providers.by_id.get(bid) if isinstance(providers, Providers) else None
+        batched = pv is not None and hasattr(pv, "contacts_points")
+        if batched:
+            contacts = pv.contacts_points(bid, ps, h)
+        else:
+            contacts = []
+            per = [None] * n
+            for i in np.nonzero(live)[0]:
+                q = (providers.contacts_point_boundary if project else providers.contacts_point)(bid, ps[i], h)
+                per[i] = q or []
+            width = max((len(q) for q in per if q), default=0)
+            for k in range(width):
+                mask = np.array([bool(q) and len(q) > k for q in per], bool)
+                pt = np.zeros((n, 3), np.float32)
+                nr = np.zeros((n, 3), np.float32)
+                for i in np.nonzero(mask)[0]:
+                    pt[i], nr[i] = per[i][k]
+                contacts.append((mask, pt, nr))
+        for mask, pt, nr in contacts:
+            nr2 = np.broadcast_to(np.asarray(nr, np.float32), (n, 3))
+            d = ps - pt
+            sdf = d[:, 0] * nr2[:, 0] + d[:, 1] * nr2[:, 1] + d[:, 2] * nr2[:, 2]
+            keep = live & mask & (sdf < h)
+            if not project:
+                keep &= sdf > np.float32(0.0)
+            keep &= count < 8
+            idx = np.nonzero(keep)[0]
+            count[idx] += 1
+            if len(idx):
+                rows.append((idx, np.full(len(idx), order), pt[idx], nr2[idx]))
+            order += 1
+    if not rows:
+        return np.zeros(0, np.uint32), np.zeros(1, np.uint32), np.zeros((0, 6), np.float32)
+    part = np.concatenate([r[0] for r in rows])
+    ordk = np.concatenate([r[1] for r in rows])
+    pts = np.concatenate([r[2] for r in rows])
+    nrs = np.concatenate([r[3] for r in rows])
+    srt = np.lexsort((ordk, part))
+    part, pts, nrs = part[srt], pts[srt], nrs[srt]
+    ids, counts = np.unique(part, return_counts=True)
+    start = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
+    planes = np.concatenate([pts, nrs], axis=1).astype(np.float32)
+    return ids.astype(np.uint32), start, planes
+
+
+def gather_wall_contacts(boundaries, h, ps, providers):
+    """Mirror table (ids, start, planes[np, 6]) of the fluid positions ps
+    (pipeline/walls.rs:145-153; contacts_point, 0 < sdf < h)."""
+    return _gather(boundaries, h, ps, providers, project=False)
+
+
+def gather_wall_project_contacts(boundaries, h, ps, providers):
+    """Project table (pipeline/walls.rs:155-166; contacts_point_boundary, sdf < h)."""
+    return _gather(boundaries, h, ps, providers, project=True)
+
+
+def _wall_arrays(providers):
+    provs = (WallProvider * max(len(providers), 1))()
+    flist = []
+    for k, pv in enumerate(providers):
+        provs[k].id = pv.id
+        b = pv.bounds(pv.id)
+        provs[k].has_bounds = 0 if b is None else 1
+        if b is not None:
+            provs[k].lo = (C.c_float * 3)(*[float(x) for x in b[0]])
+            provs[k].hi = (C.c_float * 3)(*[float(x) for x in b[1]])
+        for (axis, plane, sign) in pv.faces:
+            flist.append((axis, float(plane), float(sign), k))
+    faces = (WallFace * max(len(flist), 1))()
+    for q, (axis, plane, sign, k) in enumerate(flist):
+        faces[q].axis = axis
+        faces[q].plane = plane
+        faces[q].sign = sign
+        faces[q].provider = k
+    return provs, len(providers), faces, len(flist)
+
+
+class _Table:
+    """A plane table as C arguments (ne, ids, start, np, planes); holds the arrays."""
+
+    def __init__(self, ids, start, planes):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        self.start = np.ascontiguousarray(start, np.uint32)
+        planes = np.ascontiguousarray(planes, np.float32).reshape(-1, 6)
+        self.ne, self.np = len(ids), len(planes)
+        self.ids = ids if len(ids) else np.zeros(1, np.uint32)
+        self.planes = planes if len(planes) else np.zeros((1, 6), np.float32)
+
+    def args(self):
+        return self.ne, _u(self.ids), _u(self.start), self.np, _f(self.planes)
 
 
 def sample_batch(seed, count, nmin=8192, nmax=524288):
