@@ -181,7 +181,7 @@ def bench_walls(V, args):
     wall = time.perf_counter() - t0
     prof = eng.profile(args.profile_steps)
     kern = {k: round(v / args.profile_steps, 4) for k, v in prof.items()}
-    wall_ms = sum(v for k, v in kern.items() if k.startswith("wall_"))
+    wall_ms = sum(v for k, v in kern.items() if k.startswith(("wall_", "sep_")))
     eng.close()
     half = {"value": nf * args.walls_steps / wall, "unit": "particle-updates/s",
             "ms_per_step": 1e3 * wall / args.walls_steps, "wall_kernels_ms_per_step": round(wall_ms, 4),

@@ -232,7 +232,8 @@ private:
     uint32_t nspans_ = 0;
     WallSceneDev *d_walls_ = nullptr;
     uint32_t nwall_ = 0;           // scenes with walls (wall kernels launched when > 0)
-    bool wall_halfspace_ = false;  // some scene has half-space walls (separation launched)
+    bool wall_halfspace_ = false;  // some scene has half-space walls (ghosts in the density pass, separation)
+    bool wall_table_ = false;      // some scene has plane tables (ghost kernels after the density pass)
     std::vector<uint32_t> span_first_;  // first global span of each scene (by scene position)
     // spare state arrays: resident scenes move into them at a layout change
     float4 *spare_cA_ = nullptr, *spare_cV_ = nullptr;

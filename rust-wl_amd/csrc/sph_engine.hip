@@ -523,12 +523,14 @@ void Engine::layout() {
     free_walls();
     nwall_ = 0;
     wall_halfspace_ = false;
+    wall_table_ = false;
     std::vector<WallSceneDev> walls;
     for (auto &sc : scenes_) {
         for (uint32_t side = 0; side < 2 && sc.wall_mode == kWallsTable; ++side) upload_wall_table(sc, side);
         walls.push_back(wall_scene(sc));
         nwall_ += sc.wall_mode != kWallsOff ? 1u : 0u;
         wall_halfspace_ = wall_halfspace_ || sc.wall_mode == kWallsHalfspace;
+        wall_table_ = wall_table_ || sc.wall_mode == kWallsTable;
     }
     if (nwall_) {
         d_walls_ = dalloc<WallSceneDev>(walls.size());
@@ -537,8 +539,11 @@ void Engine::layout() {
         P_.went = dalloc<uint32_t>(2ull * slots_);
         P_.wsep = dalloc<uint64_t>(2ull * slots_);
         P_.wn = dalloc<float4>(slots_);
-        P_.sep = dalloc<SepState>(walls.size());
-        hip_check(hipMemset(P_.sep, 0, sizeof(SepState) * walls.size()), "separation state");
+        P_.wband = dalloc<uint32_t>(slots_);
+        P_.wband_n = dalloc<uint32_t>(1);
+        hip_check(hipMemset(P_.wband_n, 0, 4), "band counter");
+        P_.sep = dalloc<SepState>(walls.size() + 1);
+        hip_check(hipMemset(P_.sep, 0, sizeof(SepState) * (walls.size() + 1)), "separation state");
         P_.wslots = slots_;
         if (wall_halfspace_) {
             P_.fpos = dalloc<float4>(2ull * slots_);
@@ -602,6 +607,9 @@ void Engine::free_walls() {
     dfree(P_.wsep);
     dfree(P_.wn);
     dfree(P_.sep);
+    dfree(P_.wband);
+    dfree(P_.wband_n);
+    P_.wband = P_.wband_n = nullptr;
     dfree(P_.fpos);
     dfree(P_.hhead);
     dfree(P_.hnext);
@@ -726,6 +734,8 @@ void Engine::set_wall_tables(uint32_t id, uint32_t side, const uint32_t *ids, ui
         if (ids[e] >= sc.params.nf) throw std::invalid_argument("wall table id is not a fluid index");
         if (e && ids[e] <= ids[e - 1]) throw std::invalid_argument("wall table ids must be strictly ascending");
         if (start[e + 1] < start[e]) throw std::invalid_argument("wall table start must not decrease");
+        if (start[e + 1] - start[e] > kWallMirrorPlanes)
+            throw std::invalid_argument("at most 8 planes per wall table entry (gather_in keeps 8)");
     }
     for (size_t q = 0; q < 6ull * np; ++q)
         if (!std::isfinite(planes[q])) throw std::invalid_argument("wall planes must be finite");
@@ -791,10 +801,14 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     M("place");
     hipLaunchKernelGGL(k_canon_gather, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("canon_gather");
-    hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    if (wall_halfspace_)
+        hipLaunchKernelGGL(k_density_walls, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    else
+        hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("density");
-    if (nwall_) {
-        hipLaunchKernelGGL(k_wall_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    if (wall_table_) {
+        hipLaunchKernelGGL(k_wall_band, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(k_wall_density, dim3(std::min<uint32_t>(nb, 2048)), dim3(kBlock), 0, stream_, P_);
         M("wall_density");
     }
     if (tuning_.force_kernel == kForceScan)
@@ -806,11 +820,15 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
         hipLaunchKernelGGL(k_wall_project, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
         M("wall_project");
         if (wall_halfspace_) {
-            const uint32_t gs = std::min<uint32_t>(nb, 2048);
+            const uint32_t gs = std::min<uint32_t>(nb, 8192);
             hipLaunchKernelGGL(k_sep_begin, dim3(gs), dim3(kBlock), 0, stream_, P_);
-            for (uint32_t m = 0; m < kSepIters; ++m)
+            M("sep_begin");
+            for (uint32_t m = 0; m < kSepIters; ++m) {
                 hipLaunchKernelGGL(k_sep_round, dim3(gs), dim3(kBlock), 0, stream_, P_, m);
+                M(m == 0 ? "sep_round0" : "sep_rounds");
+            }
             hipLaunchKernelGGL(k_sep_finish, dim3(gs), dim3(kBlock), 0, stream_, P_);
+            M("sep_finish");
             hipLaunchKernelGGL(k_wall_separate, dim3(P_.nscenes), dim3(kBlock), 0, stream_, P_);
             M("wall_separate");
         }

@@ -60,13 +60,16 @@ struct Pool {
     // provider walls (walls = null: no scene of the pool has walls)
     const WallSceneDev *walls;  // [nscenes]
     uint32_t *went;   // [2][nblocks * kBlock] table entry of slot pbase + fluid index (kWallNone: none)
-    SepState *sep;    // [nscenes] coincident separation of this substep
+    SepState *sep;    // [nscenes + 1] coincident separation of this substep; [nscenes]: pool totals
+                      // (k = pushed particles of all scenes, changed[m] = some scene moved in round m)
     uint64_t *wsep;   // [2 * slots] (fluid index << 32 | slot) of the pushed ones, from 2 * pbase
     float4 *wn;       // [slots] contact normal sum of a pushed particle (grid order)
     float4 *fpos;     // [2][slots] positions of the pushed ones (fixed-point rounds), from pbase
     uint32_t *hhead;  // [3][2 * slots] spatial hash buckets of the pushed ones, from 2 * pbase
     uint32_t *hnext;  // [3][slots] bucket chains, from pbase
     uint32_t wslots;  // slots of the wall arrays (their per-copy stride)
+    uint32_t *wband;  // [slots] fluid slots with mirror ghosts this substep
+    uint32_t *wband_n;
 };
 
 __global__ void k_box(Pool P);
@@ -80,15 +83,18 @@ __global__ void k_scan_apply(Pool P, const uint32_t *tile_sums);
 __global__ void k_place(Pool P);
 __global__ void k_canon_gather(Pool P);
 __global__ void k_density(Pool P);
+__global__ void k_density_walls(Pool P);  // pools with half-space walls: mirror ghosts in the sums
 template <uint32_t WALK>
 __global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks
 __global__ void k_bforce(Pool P);
 // Provider walls (scenes with WallSceneDev::mode != kWallsOff):
-// k_wall_density after k_density (mirror ghosts, then Tait again),
+// k_wall_band + k_wall_density after the density pass (table scenes: the
+// particles with mirror ghosts listed, then their density and Tait again),
 // k_wall_project after the force pass (projection of the integrated state),
 // k_wall_separate (one block per scene: the serial coincident separation
 // where the parallel rounds did not converge), k_wall_box (the blocks'
 // partial boxes again).
+__global__ void k_wall_band(Pool P);
 __global__ void k_wall_density(Pool P);
 __global__ void k_wall_project(Pool P);
 __global__ void k_wall_separate(Pool P);

@@ -16,6 +16,7 @@
 
 #include "sph_device.h"
 #include "sph_kernels.h"
+#include "sph_walls_dev.h"
 
 namespace vxl {
 
@@ -111,9 +112,9 @@ __device__ __forceinline__ int nb_step(NbWalk &w, const Pool &P, const SceneGrid
 
 }  // namespace
 
-// AABB of the cur positions into box[parity(step)] (used at upload).
+// Partial boxes of the cur positions (at upload; k_box_reduce folds them into
+// the scenes' boxes).
 __global__ __launch_bounds__(kBlock) void k_box(Pool P) {
-    const uint32_t par = P.ctl->step & 1u;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
@@ -465,6 +466,38 @@ struct DensAcc {
             sum_b = (acc && !fj && fluid_i) ? sum_b + mw : sum_b;
         }
     }
+    __device__ __forceinline__ void ghost(bool, float, float, float) {}
+};
+
+// Density sums of a wave holding fluid particles in the band of half-space
+// walls: DensAcc<false> plus, after each accepted neighbour of a band
+// particle, the neighbour's mirror images in the particle's band faces added
+// to `sum` (fluid_density.rs:24-51, the reference's summation order).
+struct DensAccWalls {
+    float sum, sum_b;
+    bool fluid_i;
+    wall::V3 pi;
+    uint32_t fm;  // the lane's band faces (bit k = face k of the scene)
+    uint32_t wm;  // their union over the wave
+    const WallFaceDev *face;
+    float h2, k6;
+    __device__ __forceinline__ void add(bool acc, float r2, float aw, float h2_, float k6_) {
+        const float t = h2_ - r2;
+        const float w = ((k6_ * t) * t) * t;
+        const bool fj = __float_as_int(aw) >= 0;
+        const float mw = fabsf(aw) * w;
+        const float add_sum = fluid_i ? w : mw;
+        sum = (acc && fj) ? sum + add_sum : sum;
+        sum_b = (acc && !fj && fluid_i) ? sum_b + mw : sum_b;
+    }
+    __device__ __forceinline__ void ghost(bool acc, float ddx, float ddy, float ddz) {
+        for (uint32_t m = wm; m; m &= m - 1u) {
+            const uint32_t k = (uint32_t)__builtin_ctz(m);
+            float w;
+            if (acc && ((fm >> k) & 1u) && wall::face_ghost(face[k], pi, wall::V3{ddx, ddy, ddz}, h2, k6, w))
+                sum = sum + w;
+        }
+    }
 };
 
 // Four consecutive candidates jb .. jb + 3 of a row [j0, j1) (loaded without
@@ -494,6 +527,7 @@ __device__ __forceinline__ void dens_group(const float4 *__restrict__ sA, const 
         else
             cnt += acc ? 1u : 0u;
         da.add(acc, r2, aj[u].w, h2, k6);
+        da.ghost(acc, ddx, ddy, ddz);
     }
     if (MASK) mk |= gb << (jb - j0);
 }
@@ -516,12 +550,15 @@ __device__ __forceinline__ uint32_t dens_row(const float4 *__restrict__ sA, cons
     return mk;
 }
 
-// Density (fluid_density.rs:17-70, no providers) + Tait (fluid_force.rs:11-29,
-// gamma = 7) + the per-particle pressure term p / (rho * rho) used by the
-// force. The sums are taken while the candidates are tested (selects keep the
+// Density (fluid_density.rs:17-70) + Tait (fluid_force.rs:11-29, gamma = 7)
+// + the per-particle pressure term p / (rho * rho) used by the force. The
+// sums are taken while the candidates are tested (selects keep the
 // reference's accumulation order and bits). Records every fluid particle's
-// accepted neighbours (nbm, nbs) for the force pass.
-__global__ __launch_bounds__(kBlock) void k_density(Pool P) {
+// accepted neighbours (nbm, nbs) for the force pass. WALLS: the pool has
+// scenes with half-space walls, whose band particles add the mirror ghosts of
+// their neighbours (fluid_density.rs:24-51, :76-121).
+template <bool WALLS>
+__device__ __forceinline__ void density_pass(const Pool &P) {
     if (P.ctl->err) return;
     const uint32_t stride = P.nblocks * kBlock;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
@@ -559,7 +596,20 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
             sum = da.sum;
             sum_b = da.sum_b;
         };
-        if (__all(fluid_i)) {
+        uint32_t fm = 0;
+        const WallFaceDev *faces = nullptr;
+        if constexpr (WALLS) {
+            const WallSceneDev &W = P.walls[s];
+            faces = W.face;
+            if (W.mode == kWallsHalfspace && fluid_i) fm = wall::band_faces(W, wall::V3{ai.x, ai.y, ai.z}, sp.h);
+        }
+        if (WALLS && __any(fm != 0u)) {
+            uint32_t wm = 0;
+            for (uint32_t k = 0; k < kWallMaxFaces; ++k)
+                if (__any((fm >> k) & 1u)) wm |= 1u << k;
+            DensAccWalls da{sum, sum_b, fluid_i, wall::V3{ai.x, ai.y, ai.z}, fm, wm, faces, h2, k6};
+            full_scan(da);
+        } else if (__all(fluid_i)) {
             DensAcc<true> da{sum, sum_b, true};
             full_scan(da);
         } else if (__any(fluid_i) || P.ncount) {
@@ -596,6 +646,9 @@ __global__ __launch_bounds__(kBlock) void k_density(Pool P) {
         if (P.ncount) P.ncount[i] = cnt;
     }
 }
+
+__global__ __launch_bounds__(kBlock) void k_density(Pool P) { density_pass<false>(P); }
+__global__ __launch_bounds__(kBlock) void k_density_walls(Pool P) { density_pass<true>(P); }
 
 
 

@@ -14,67 +14,19 @@
 
 #include "sph_device.h"
 #include "sph_kernels.h"
+#include "sph_walls_dev.h"
 
 namespace vxl {
 
 namespace {
 
+using dev::at;
 using dev::block_box;
 using dev::Cursor;
 using dev::for_rows;
 using dev::make_cursor;
 using dev::tait;
-
-struct V3 {
-    float x, y, z;
-};
-__device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ V3 mul(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 xyz(float4 a) { return V3{a.x, a.y, a.z}; }
-// Component `axis` of v (selects, so v stays in registers).
-__device__ __forceinline__ float get_axis(V3 v, uint32_t axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); }
-__device__ __forceinline__ V3 with_axis(V3 v, uint32_t axis, float c) {
-    return V3{axis == 0 ? c : v.x, axis == 1 ? c : v.y, axis == 2 ? c : v.z};
-}
-
-// Point query of one half-space provider (the reference's test doubles FloorY
-// and Tank, crates/vxl-phys-fluid/src/tests.rs:11-110): face (axis, plane,
-// sign) within 2 * probe of p yields the contact point p with p[axis] = plane
-// and the outward normal sign * e_axis, faces in order.
-template <class F>
-__device__ __forceinline__ void prov_contacts(const WallSceneDev &W, const WallProvDev &pv, V3 p, float probe,
-                                              F &&emit) {
-    for (uint32_t k = 0; k < pv.nface; ++k) {
-        const WallFaceDev f = W.face[pv.face0 + k];
-        const float sdf = (get_axis(p, f.axis) - f.plane) * f.sign;
-        if (sdf > 2.0f * probe) continue;
-        emit(with_axis(p, f.axis, f.plane), with_axis(V3{0.0f, 0.0f, 0.0f}, f.axis, f.sign));
-    }
-}
-
-// AABB prefilter of every provider query with margin m (fluid_density.rs:88-99,
-// fluid_boundary.rs:62-73): true = the provider is skipped.
-__device__ __forceinline__ bool outside(const WallProvDev &pv, V3 p, float m) {
-    if (!pv.has_bounds) return false;
-    return p.x < pv.lo[0] - m || p.x > pv.hi[0] + m || p.y < pv.lo[1] - m || p.y > pv.hi[1] + m ||
-           p.z < pv.lo[2] - m || p.z > pv.hi[2] + m;
-}
-
-// Mirror image of neighbour j (at pi - d) in plane (pt, n), and its kernel
-// weight at pi when within h (fluid_density.rs:36-45).
-__device__ __forceinline__ bool ghost_w(V3 pi, V3 d, V3 pt, V3 n, float h2, float k6, float &w) {
-    const V3 pj = sub(pi, d);
-    const float dn = dot(sub(pj, pt), n);
-    const V3 g = sub(pj, mul(n, 2.0f * dn));
-    const V3 dr = sub(pi, g);
-    const float rg2 = dot(dr, dr);
-    if (!(rg2 <= h2)) return false;
-    const float t = h2 - rg2;
-    w = ((k6 * t) * t) * t;
-    return true;
-}
+using namespace wall;
 
 __device__ __forceinline__ uint32_t entry_of(const Pool &P, uint32_t side, uint32_t pbase, uint32_t gid) {
     return P.went[(size_t)side * P.nblocks * kBlock + pbase + gid];
@@ -82,102 +34,72 @@ __device__ __forceinline__ uint32_t entry_of(const Pool &P, uint32_t side, uint3
 
 }  // namespace
 
-// Mirror ghosts of the fluid particles in a wall band; their density, Tait
-// pressure and pressure term are written again.
-//
-// Half-space walls: the planes of pi within its h band (0 < sdf < h, at most
-// 8; wall_planes_in, fluid_density.rs:76-121); when there are any, the whole
-// density sum is taken again with each neighbour's ghosts added to `sum`
-// right after the neighbour itself (fluid_density.rs:24-51), which is the
-// reference's summation order.
-// Plane tables: the entry's planes whose band holds pi now; the ghosts of all
-// neighbours are summed apart and dens += mass * add (wall_ghost.wgsl:61-136).
-__global__ __launch_bounds__(kBlock) void k_wall_density(Pool P) {
+// The fluid particles of table scenes with a non-empty mirror entry, listed
+// (wave-aggregated appends) for k_wall_density. (Half-space scenes take their
+// ghosts inside the density pass.)
+__global__ __launch_bounds__(kBlock) void k_wall_band(Pool P) {
     if (P.ctl->err) return;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const uint32_t s = P.block_scene[b];
         const WallSceneDev &W = P.walls[s];
-        if (W.mode == kWallsOff) continue;
+        if (W.mode != kWallsTable) continue;
         const SceneParams &sp = P.params[s];
         const uint32_t i = b * kBlock + threadIdx.x;
-        if (i >= sp.pbase + sp.n) continue;
+        bool band = false;
+        if (i < sp.pbase + sp.n && P.sA[i].w > 0.0f) {
+            const uint32_t e = entry_of(P, 0, sp.pbase, P.sg[i]);
+            band = e != kWallNone && W.tstart[0][e] < W.tstart[0][e + 1];
+        }
+        const uint64_t m = __ballot(band);
+        if (m == 0) continue;
+        const uint32_t lane = threadIdx.x & 63u;
+        const uint32_t lead = (uint32_t)__builtin_ctzll(m);
+        uint32_t base = 0;
+        if (lane == lead) base = atomicAdd(P.wband_n, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, (int)lead, 64);
+        if (band) P.wband[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+}
+
+// Mirror ghosts of the listed table-scene particles (wall_ghost.wgsl:61-136):
+// the entry's planes whose band holds pi now (0 < sdf < h); the ghosts of all
+// neighbours summed apart in neighbour order, then dens += mass * add and the
+// Tait pressure and pressure term again.
+__global__ __launch_bounds__(kBlock) void k_wall_density(Pool P) {
+    if (P.ctl->err) return;
+    const uint32_t count = *P.wband_n;
+    for (uint32_t u = blockIdx.x * kBlock + threadIdx.x; u < count; u += gridDim.x * kBlock) {
+        const uint32_t i = P.wband[u];
+        const uint32_t s = P.block_scene[i / kBlock];
+        const WallSceneDev &W = P.walls[s];
+        const SceneParams &sp = P.params[s];
         const float4 ai = P.sA[i];
-        if (!(ai.w > 0.0f)) continue;  // boundary particles have no ghosts
         const V3 pi = xyz(ai);
         const SceneGrid &g = P.grid[s];
         const float h = sp.h, h2 = sp.h2, k6 = sp.k6;
-        const float4 *__restrict__ sA = P.sA;
+        const uint32_t e = entry_of(P, 0, sp.pbase, P.sg[i]);
+        const uint32_t st = W.tstart[0][e], en = W.tstart[0][e + 1];
+        const float4 *__restrict__ tp = (const float4 *)W.tplane[0];
         const Cursor cur = make_cursor(g, h, ai.x, ai.y, ai.z);
-        float rho;
-        if (W.mode == kWallsHalfspace) {
-            V3 pt[kWallMirrorPlanes], nr[kWallMirrorPlanes];
-            uint32_t np = 0;
-            for (uint32_t k = 0; k < W.nprov; ++k) {
-                const WallProvDev &pv = W.prov[k];
-                if (outside(pv, pi, h)) continue;
-                prov_contacts(W, pv, pi, h, [&](V3 point, V3 n) {
-                    const float sdf = dot(sub(pi, point), n);
-                    if (sdf > 0.0f && sdf < h && np < kWallMirrorPlanes) {
-#pragma unroll
-                        for (uint32_t q = 0; q < kWallMirrorPlanes; ++q) {
-                            if (q == np) {
-                                pt[q] = point;
-                                nr[q] = n;
-                            }
-                        }
-                        ++np;
-                    }
-                });
+        float acc = 0.0f;
+        for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t j0, uint32_t j1) {
+            for (uint32_t j = j0; j < j1; ++j) {
+                if (j == i) continue;
+                const float4 aj = *at(P.sA, j);
+                const V3 d{ai.x - aj.x, ai.y - aj.y, ai.z - aj.z};
+                const float r2 = dot(d, d);
+                if (!(r2 <= h2)) continue;
+                for (uint32_t q = st; q < en; ++q) {
+                    const V3 pt = xyz(tp[2 * q]), n = xyz(tp[2 * q + 1]);
+                    const float sdf_i = dot(sub(pi, pt), n);
+                    if (sdf_i <= 0.0f || sdf_i >= h) continue;
+                    float wg;
+                    if (ghost_w(pi, d, pt, n, h2, k6, wg)) acc = acc + wg;
+                }
             }
-            if (np == 0) continue;
-            float sum = sp.w0, sum_b = 0.0f;
-            for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t j0, uint32_t j1) {
-                for (uint32_t j = j0; j < j1; ++j) {
-                    if (j == i) continue;
-                    const float4 aj = sA[j];
-                    const V3 d{ai.x - aj.x, ai.y - aj.y, ai.z - aj.z};
-                    const float r2 = dot(d, d);
-                    if (!(r2 <= h2)) continue;
-                    const float t = h2 - r2;
-                    const float w = ((k6 * t) * t) * t;
-                    if (__float_as_int(aj.w) >= 0)
-                        sum = sum + w;
-                    else
-                        sum_b = sum_b + fabsf(aj.w) * w;
-#pragma unroll
-                    for (uint32_t q = 0; q < kWallMirrorPlanes; ++q) {
-                        float wg;
-                        if (q < np && ghost_w(pi, d, pt[q], nr[q], h2, k6, wg)) sum = sum + wg;
-                    }
-                }
-            });
-            rho = sp.mass * sum + sum_b;
-        } else {
-            const uint32_t e = entry_of(P, 0, sp.pbase, P.sg[i]);
-            if (e == kWallNone) continue;
-            const uint32_t st = W.tstart[0][e], en = W.tstart[0][e + 1];
-            if (st >= en) continue;
-            const float4 *__restrict__ pl = (const float4 *)W.tplane[0];
-            float acc = 0.0f;
-            for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t j0, uint32_t j1) {
-                for (uint32_t j = j0; j < j1; ++j) {
-                    if (j == i) continue;
-                    const float4 aj = sA[j];
-                    const V3 d{ai.x - aj.x, ai.y - aj.y, ai.z - aj.z};
-                    const float r2 = dot(d, d);
-                    if (!(r2 <= h2)) continue;
-                    for (uint32_t q = st; q < en; ++q) {
-                        const V3 pt = xyz(pl[2 * q]), n = xyz(pl[2 * q + 1]);
-                        const float sdf_i = dot(sub(pi, pt), n);
-                        if (sdf_i <= 0.0f || sdf_i >= h) continue;
-                        float wg;
-                        if (ghost_w(pi, d, pt, n, h2, k6, wg)) acc = acc + wg;
-                    }
-                }
-            });
-            if (acc == 0.0f) continue;
-            rho = P.sV[i].w + sp.mass * acc;
-        }
+        });
+        if (acc == 0.0f) continue;
+        const float rho = P.sV[i].w + sp.mass * acc;
         float p, c2;
         tait(sp, rho, p, c2);
         P.sV[i].w = rho;
@@ -217,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_project(Pool P) {
             for (uint32_t k = 0; k < W.nprov; ++k) {
                 const WallProvDev &pv = W.prov[k];
                 if (outside(pv, pi, h)) continue;
-                prov_contacts(W, pv, pi, skin, [&](V3 point, V3 n) {
+                prov_contacts(W, pv, pi, skin, [&](uint32_t, V3 point, V3 n) {
                     const float sdf = dot(sub(pi, point), n);
                     const float pen = -sdf;
                     if (pen <= 0.0f) return;
@@ -228,10 +150,21 @@ __global__ __launch_bounds__(kBlock) void k_wall_project(Pool P) {
                     if (vn < 0.0f) v = sub(v, mul(n, vn));
                 });
             }
-            if (hit) {
-                P.wn[i] = make_float4(n_sum.x, n_sum.y, n_sum.z, 0.0f);
-                const uint32_t k = atomicAdd(&P.sep[s].k, 1u);
-                P.wsep[2ull * sp.pbase + k] = ((uint64_t)P.cg[i] << 32) | i;
+            // Append the pushed ones to the scene's list (one atomic per wave).
+            const uint64_t mk = __ballot(hit);
+            if (mk) {
+                const uint32_t lane = threadIdx.x & 63u, lead = (uint32_t)__builtin_ctzll(mk);
+                uint32_t base = 0;
+                if (lane == lead) {
+                    base = atomicAdd(&P.sep[s].k, (uint32_t)__popcll(mk));
+                    atomicAdd(&P.sep[P.nscenes].k, (uint32_t)__popcll(mk));
+                }
+                base = (uint32_t)__shfl((int)base, (int)lead, 64);
+                if (hit) {
+                    P.wn[i] = make_float4(n_sum.x, n_sum.y, n_sum.z, 0.0f);
+                    P.wsep[2ull * sp.pbase + base + (uint32_t)__popcll(mk & ((1ull << lane) - 1ull))] =
+                        ((uint64_t)P.cg[i] << 32) | i;
+                }
             }
         } else {
             const uint32_t e = entry_of(P, 1, sp.pbase, P.cg[i]);
@@ -292,16 +225,27 @@ __device__ __forceinline__ uint32_t sep_buckets(uint32_t k, uint32_t alloc) {
     return min(hb, cap);
 }
 
+// Hash cells of side 2.0625 r_min, computed in f64 (exact enough at any
+// coordinate): every point closer than r_min to p lies in the 2 x 2 x 2
+// block of cells nearest p.
+constexpr double kSepCell = 2.0625;
+__device__ __forceinline__ int sep_coord(double t) {
+    return (int)fmin(fmax(floor(t), -1073741824.0), 1073741824.0);  // NaN -> lower bound
+}
 struct SepCell {
     int x, y, z;
 };
-// Cell of side 1.0625 r_min: points closer than r_min are in adjacent cells.
-__device__ __forceinline__ SepCell sep_cell(float4 p, float inv) {
-    auto c = [&](float v) {
-        const float f = floorf(v * inv);
-        return (int)fminf(fmaxf(f, -1073741824.0f), 1073741824.0f);  // NaN -> lower bound
+__device__ __forceinline__ SepCell sep_cell(float4 p, double inv) {
+    return SepCell{sep_coord((double)p.x * inv), sep_coord((double)p.y * inv), sep_coord((double)p.z * inv)};
+}
+// Lower corner of the 2 x 2 x 2 block around p.
+__device__ __forceinline__ SepCell sep_block(V3 p, double inv) {
+    auto lo = [&](float v) {
+        const double t = (double)v * inv;
+        const int c = sep_coord(t);
+        return (t - floor(t)) < 0.5 ? c - 1 : c;
     };
-    return SepCell{c(p.x), c(p.y), c(p.z)};
+    return SepCell{lo(p.x), lo(p.y), lo(p.z)};
 }
 __device__ __forceinline__ uint32_t sep_bucket(int x, int y, int z, uint32_t nb) {
     return (((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u)) & (nb - 1u);
@@ -328,7 +272,7 @@ __device__ __forceinline__ SepEntry sep_entry(const Pool &P, uint32_t b) {
 }
 
 __device__ __forceinline__ void sep_insert(const Pool &P, uint32_t copy, const SceneParams &sp, uint32_t nb, uint32_t e,
-                                           float4 p, float inv) {
+                                           float4 p, double inv) {
     const SepCell c = sep_cell(p, inv);
     uint32_t *head = P.hhead + (size_t)copy * 2u * P.wslots + 2ull * sp.pbase;
     P.hnext[(size_t)copy * P.wslots + sp.pbase + e] = atomicExch(&head[sep_bucket(c.x, c.y, c.z, nb)], e);
@@ -338,14 +282,14 @@ __device__ __forceinline__ void sep_insert(const Pool &P, uint32_t copy, const S
 
 // Round 0's positions (the projected ones) and their hash (copy 0).
 __global__ __launch_bounds__(kBlock) void k_sep_begin(Pool P) {
-    if (P.ctl->err) return;
+    if (P.ctl->err || P.sep[P.nscenes].k == 0) return;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const SepEntry t = sep_entry(P, b);
         if (!t.live) continue;
         const uint32_t slot = (uint32_t)P.wsep[2ull * t.sp->pbase + t.e];
         const float4 q = P.cA[slot];
         P.fpos[t.sp->pbase + t.e] = q;
-        sep_insert(P, 0, *t.sp, t.nb, t.e, q, 1.0f / (1.0625f * (0.2f * t.sp->h)));
+        sep_insert(P, 0, *t.sp, t.nb, t.e, q, 1.0 / (kSepCell * (double)(0.2f * t.sp->h)));
     }
 }
 
@@ -353,7 +297,8 @@ __global__ __launch_bounds__(kBlock) void k_sep_begin(Pool P) {
 // the others (hash copy m % 3); its result goes to position copy (m + 1) % 2
 // and hash copy (m + 1) % 3; hash copy (m + 2) % 3 is cleared for round m + 1.
 __global__ __launch_bounds__(kBlock) void k_sep_round(Pool P, uint32_t m) {
-    if (P.ctl->err) return;
+    if (P.ctl->err || P.sep[P.nscenes].k == 0) return;
+    if (m > 0 && P.sep[P.nscenes].changed[m - 1] == 0u) return;  // every scene converged
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const SepEntry t = sep_entry(P, b);
         if (t.k == 0 || !t.sp) continue;
@@ -372,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_sep_round(Pool P, uint32_t m) {
         }
         if (!t.live) continue;
         const float r_min = 0.2f * sp.h;
-        const float inv = 1.0f / (1.0625f * r_min);
+        const double inv = 1.0 / (kSepCell * (double)r_min);
         const float4 *cur = P.fpos + (size_t)(m % 2u) * P.wslots + sp.pbase;
         float4 *nxt = P.fpos + (size_t)((m + 1u) % 2u) * P.wslots + sp.pbase;
         const uint32_t copy = m % 3u;
@@ -391,25 +336,26 @@ __global__ __launch_bounds__(kBlock) void k_sep_round(Pool P, uint32_t m) {
             uint32_t last = 0;  // indices at or below `last` are done (first pass: from 0)
             bool first = true;
             for (;;) {
-                const float4 pa4 = make_float4(pa.x, pa.y, pa.z, 0.0f);
-                const SepCell c = sep_cell(pa4, inv);
+                const SepCell c = sep_block(pa, inv);
+                uint32_t hd[8];
+#pragma unroll
+                for (uint32_t k = 0; k < 8u; ++k)
+                    hd[k] = head[sep_bucket(c.x + (int)(k & 1u), c.y + (int)((k >> 1) & 1u), c.z + (int)(k >> 2), t.nb)];
                 uint32_t best = 0xffffffffu;
                 float best_r2 = 0.0f;
-                for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            for (uint32_t f = head[sep_bucket(c.x + dx, c.y + dy, c.z + dz, t.nb)]; f != kWallNone;
-                                 f = next[f]) {
-                                const uint32_t gid_b = (uint32_t)(list[f] >> 32);
-                                if (gid_b >= gid_a || (!first && gid_b <= last) || gid_b >= best) continue;
-                                const V3 d = sub(pa, xyz(cur[f]));
-                                const float r2 = dot(d, d);
-                                if (r2 < r_min * r_min) {
-                                    best = gid_b;
-                                    best_r2 = r2;
-                                }
-                            }
+#pragma unroll
+                for (uint32_t k = 0; k < 8u; ++k) {
+                    for (uint32_t f = hd[k]; f != kWallNone; f = next[f]) {
+                        const uint32_t gid_b = (uint32_t)(list[f] >> 32);
+                        if (gid_b >= gid_a || (!first && gid_b <= last) || gid_b >= best) continue;
+                        const V3 d = sub(pa, xyz(cur[f]));
+                        const float r2 = dot(d, d);
+                        if (r2 < r_min * r_min) {
+                            best = gid_b;
+                            best_r2 = r2;
                         }
+                    }
+                }
                 if (best == 0xffffffffu) break;
                 const float r = sqrtf(best_r2);
                 pa = add(pa, mul(dir, r_min - r));
@@ -422,7 +368,10 @@ __global__ __launch_bounds__(kBlock) void k_sep_round(Pool P, uint32_t m) {
         const float4 was = cur[t.e];
         if (__float_as_uint(out.x) != __float_as_uint(was.x) || __float_as_uint(out.y) != __float_as_uint(was.y) ||
             __float_as_uint(out.z) != __float_as_uint(was.z))
+        {
             P.sep[t.s].changed[m] = 1u;
+            P.sep[P.nscenes].changed[m] = 1u;
+        }
         sep_insert(P, (m + 1u) % 3u, sp, t.nb, t.e, out, inv);
     }
 }
@@ -430,7 +379,7 @@ __global__ __launch_bounds__(kBlock) void k_sep_round(Pool P, uint32_t m) {
 // Converged scenes: the separated positions back into the state; every
 // scene's buckets cleared for the next substep.
 __global__ __launch_bounds__(kBlock) void k_sep_finish(Pool P) {
-    if (P.ctl->err) return;
+    if (P.ctl->err || P.sep[P.nscenes].k == 0) return;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const SepEntry t = sep_entry(P, b);
         if (t.k == 0 || !t.sp) continue;
@@ -527,7 +476,7 @@ __device__ void separate_wave(const Pool &P, const uint64_t *key, uint32_t k, fl
 __global__ __launch_bounds__(kBlock) void k_wall_separate(Pool P) {
     __shared__ uint64_t skey[kSepLds];
     __shared__ float4 spos[kSepLds];
-    if (P.ctl->err) return;
+    if (P.ctl->err || P.sep[P.nscenes].k == 0) return;
     const uint32_t s = blockIdx.x;
     if (s >= P.nscenes || P.walls[s].mode != kWallsHalfspace) return;
     SepState &st = P.sep[s];
@@ -575,6 +524,14 @@ __global__ __launch_bounds__(kBlock) void k_wall_separate(Pool P) {
 // Partial boxes of the wall scenes' blocks over the projected positions
 // (replacing the ones the force pass wrote).
 __global__ __launch_bounds__(kBlock) void k_wall_box(Pool P) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *P.wband_n = 0;  // band list of the next substep
+    if (blockIdx.x == 0 && threadIdx.x <= kSepIters && P.sep) {
+        // pool totals of the separation, for the next substep
+        if (threadIdx.x == kSepIters)
+            P.sep[P.nscenes].k = 0;
+        else
+            P.sep[P.nscenes].changed[threadIdx.x] = 0u;
+    }
     if (P.ctl->err) return;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const uint32_t s = P.block_scene[b];
