@@ -172,7 +172,11 @@ def bench_walls(V, args):
     eng = V.Engine()
     sid = eng.add(cfg, data, reactions=False)
     eng.set_halfspace_walls(sid, [walls])
-    tuned = eng.autotune(2)
+    if args.force_kernel is not None:
+        eng.set_tuning(force_kernel=args.force_kernel)
+        tuned = eng.get_tuning()
+    else:
+        tuned = eng.autotune(2)
     eng.step(args.warmup)
     eng.sync()
     t0 = time.perf_counter()
