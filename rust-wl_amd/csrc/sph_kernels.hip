@@ -466,6 +466,13 @@ struct DensAcc {
             sum_b = (acc && !fj && fluid_i) ? sum_b + mw : sum_b;
         }
     }
+    // All of this candidate's group are fluid particles: the class selects drop.
+    __device__ __forceinline__ void add_fluid(bool acc, float r2, float h2, float k6) {
+        const float t = h2 - r2;
+        const float w = ((k6 * t) * t) * t;
+        sum = acc ? sum + w : sum;
+    }
+    static constexpr bool kFluidGroups = ALL_FLUID;
     __device__ __forceinline__ void ghost(bool, float, float, float) {}
 };
 
@@ -474,6 +481,8 @@ struct DensAcc {
 // particle, the neighbour's mirror images in the particle's band faces added
 // to `sum` (fluid_density.rs:24-51, the reference's summation order).
 struct DensAccWalls {
+    static constexpr bool kFluidGroups = false;
+    __device__ __forceinline__ void add_fluid(bool, float, float, float) {}
     float sum, sum_b;
     bool fluid_i;
     wall::V3 pi;
@@ -514,21 +523,37 @@ __device__ __forceinline__ void dens_group(const float4 *__restrict__ sA, const 
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) aj[u] = p[u];
     uint32_t gb = 0;
+    auto each = [&](auto &&sum_one) {
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u) {
-        const uint32_t j = jb + u;
-        const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
-        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        bool acc = r2 <= h2;
-        if (SELF) acc = acc && j != i;
-        if (PART) acc = acc && j < j1;
-        if (MASK)
-            gb |= acc ? 1u << u : 0u;
-        else
-            cnt += acc ? 1u : 0u;
-        da.add(acc, r2, aj[u].w, h2, k6);
-        da.ghost(acc, ddx, ddy, ddz);
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint32_t j = jb + u;
+            const float ddx = ai.x - aj[u].x, ddy = ai.y - aj[u].y, ddz = ai.z - aj[u].z;
+            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            bool acc = r2 <= h2;
+            if (SELF) acc = acc && j != i;
+            if (PART) acc = acc && j < j1;
+            if (MASK)
+                gb |= acc ? 1u << u : 0u;
+            else
+                cnt += acc ? 1u : 0u;
+            sum_one(acc, r2, aj[u].w, ddx, ddy, ddz);
+        }
+    };
+    // Waves of fluid particles: a group whose four candidates are all fluid
+    // (the fluid interior) takes the sum without the class selects.
+    bool fluid_group = false;
+    if constexpr (Acc::kFluidGroups) {
+        const int cls = __float_as_int(aj[0].w) | __float_as_int(aj[1].w) | __float_as_int(aj[2].w) |
+                        __float_as_int(aj[3].w);
+        fluid_group = !__any(cls < 0);
     }
+    if (fluid_group)
+        each([&](bool acc, float r2, float, float, float, float) { da.add_fluid(acc, r2, h2, k6); });
+    else
+        each([&](bool acc, float r2, float aw, float ddx, float ddy, float ddz) {
+            da.add(acc, r2, aw, h2, k6);
+            da.ghost(acc, ddx, ddy, ddz);
+        });
     if (MASK) mk |= gb << (jb - j0);
 }
 

@@ -335,7 +335,9 @@ def test_serve_results_equal_alone():
 
 
 def test_serve_residency_limit_and_fifo_admission():
-    reqs = V.sample_requests(6, 6, rate=1000.0, nmin=1500, nmax=4000, smin=4, smax=6)
+    # Requests long enough (150-200 steps) that a second one is built and
+    # waiting while the first still runs, whatever the host's speed.
+    reqs = V.sample_requests(6, 6, rate=1000.0, nmin=1500, nmax=4000, smin=150, smax=200)
     stats, results = V.serve(reqs, max_resident=2, builders=2)
     assert stats["completed"] == len(reqs)
     assert stats["peak_resident"] == 2
