@@ -145,6 +145,46 @@ def test_coupled_scene_argument_errors():
     assert L.vxl_sph_request_bodies(None, 0, None, C.byref(cnt), None) == 1
 
 
+def test_wall_entry_points_reject_null_arguments():
+    L = V.lib()
+    prov = (V.WallProvider * 1)()
+    face = (V.WallFace * 1)()
+    assert L.vxl_sph_scene_set_halfspace_walls(None, 0, 1, prov, 1, face) == 1
+    assert L.vxl_sph_scene_set_wall_tables(None, 0, 0, 0, None, None, 0, None) == 1
+    assert L.vxl_sph_scene_clear_walls(None, 0) == 1
+    assert L.vxl_sph_stepper_set_wall_tables(None, 0, 0, None, None, 0, None) == 1
+    assert L.vxl_sph_stepper_set_halfspace_walls(None, 0, None, 0, None) == 1
+
+
+def test_host_gather_of_the_reference_floor_and_tank():
+    """vxl_sph.gather_wall_contacts (gather_in, pipeline/walls.rs:168-216) on
+    hand-placed points: the band filters of both sides, face order, the cap of
+    8 planes and the AABB prefilter."""
+    F = np.float32
+    ps = np.array([(0.0, 0.05, 0.0),      # floor band only
+                   (0.15, 0.02, 0.15),    # floor, right wall, front wall
+                   (0.0, -0.01, 0.0),     # below the floor: project side only
+                   (0.0, 0.5, 0.0),       # far from every wall
+                   (0.0, 0.05, 60.0)], F)  # outside the Tank's bounds: prefiltered
+    provs = V.Providers({0: V.TANK})
+    h = F(0.1)
+    ids, start, planes = V.gather_wall_contacts([0], h, ps, provs)
+    assert list(ids) == [0, 1] and list(start) == [0, 1, 4]
+    np.testing.assert_array_equal(planes[1:, 3:], np.array([(0, 1, 0), (-1, 0, 0), (0, 0, -1)], F))
+    np.testing.assert_array_equal(planes[0], np.array([0.0, 0.0, 0.0, 0.0, 1.0, 0.0], F))
+    ids, start, planes = V.gather_wall_project_contacts([0], h, ps, provs)
+    assert list(ids) == [0, 1, 2] and list(start) == [0, 1, 4, 5]
+    # the FloorY double reports any point within 2h of the floor, so the
+    # point far above it keeps nothing; two floor providers give two planes
+    both = V.Providers({0: V.FLOOR_Y, 1: V.FLOOR_Y})
+    ids, start, _ = V.gather_wall_contacts([0, 1], h, ps[:2], both)
+    assert list(start) == [0, 2, 4]
+    # at most 8 planes per particle
+    many = V.Providers({k: V.FLOOR_Y for k in range(10)})
+    ids, start, _ = V.gather_wall_contacts(list(range(10)), h, ps[:1], many)
+    assert list(start) == [0, 8]
+
+
 def test_request_bodies_are_tank_then_boxes():
     """A served request with boxes runs with the tank's five static Box bodies
     (the same particles as the request's own tank) followed by its dynamic

@@ -234,3 +234,32 @@ def test_walls_in_a_mixed_pool_equal_alone():
     assert m.state_hash(b) == alone["table"]
     m.clear_walls(a)
     m.step(1)  # walls removed: the pool steps on without them
+
+
+def test_coupled_boxes_in_a_provider_tank_match_oracle():
+    """Dynamic boxes (rigid leg on the device, both-way coupling) in fluid held
+    by half-space provider walls instead of static Akinci boxes: the boxes'
+    particles take no ghosts and are not projected (fluid_density.rs:53-69,
+    fluid_boundary.rs:37-39); fluid, bodies and hashes equal the oracle's."""
+    from test_gpu_coupled import OracleCoupled, assert_bodies_equal, coupled_scene
+
+    cfg, fluid, _statics, boxes, tank, d = coupled_scene(seed=21)
+    nf = fluid.n_fluid
+    walls = V.HalfspaceProvider([(1, 0.0, 1.0), (0, 0.0, 1.0), (0, tank[0], -1.0), (2, 0.0, 1.0),
+                                 (2, tank[2], -1.0)])
+    eng = V.Engine()
+    sid = eng.add_coupled(cfg, V.rigid_config(tank=tank), fluid.pos[:nf], boxes)
+    eng.set_halfspace_walls(sid, [walls])
+    orc = OracleCoupled(cfg, fluid, [], boxes, tank)
+    orc.sys.set_halfspace_walls([dict(id=0, bounds=None, faces=walls.faces)])
+    for chunk in (1, 39, 60):
+        eng.step(chunk)
+        for _ in range(chunk):
+            orc.tick()
+        pos, vel = eng.read_state(sid)
+        np.testing.assert_array_equal(bits(pos), bits(orc.sys.pos()), err_msg="positions")
+        np.testing.assert_array_equal(bits(vel), bits(orc.sys.vel()), err_msg="velocities")
+        assert eng.state_hash(sid) == orc.sys.state_hash()
+        assert_bodies_equal(eng.read_bodies(sid), orc)
+    p = pos[:nf]
+    assert p[:, 1].min() > -0.01 and p[:, 0].min() > -0.01 and p[:, 0].max() < tank[0] + 0.01
