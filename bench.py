@@ -230,6 +230,8 @@ def main():
     ap.add_argument("--budget-log2", type=int, default=22,
                     help="grid budget GRID_MAX_BINS = 2^k (22 keeps bin = h at 10M; the reference rule is 21)")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-single-steps", type=int, default=1,
+                    help="steps of the single-thread CPU baseline (the reference's default threads = 1); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--force-kernel", type=int, default=None, choices=(1, 2),
@@ -335,6 +337,12 @@ def main():
         cpu = {"value": cv, "unit": "particle-updates/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} steps of the same {NF}-fluid scene ({cdt:.1f} s), strict-f32 C++ oracle, "
                          f"phase-parallel over {threads} threads"}
+        if args.cpu_single_steps > 0:
+            print(f"# cpu baseline: oracle, 1 thread, {args.cpu_single_steps} step(s)", file=sys.stderr, flush=True)
+            c1, c1dt = cpu_baseline(cfg, data, args.cpu_single_steps, 1)
+            cpu["single_thread"] = {"value": c1, "unit": "particle-updates/s", "cores": 1,
+                                    "sample": f"{args.cpu_single_steps} step(s) of the same scene ({c1dt:.1f} s), "
+                                              "threads = 1 (the reference's default)"}
 
     out = {
         "metric": METRIC,
