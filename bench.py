@@ -234,8 +234,8 @@ def main():
                     help="steps of the single-thread CPU baseline (the reference's default threads = 1); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--force-kernel", type=int, default=None, choices=(1, 2),
-                    help="force kernel instead of the autotuner's choice: 1 candidate scan, 2 neighbour walk")
+    ap.add_argument("--force-kernel", type=int, default=None, choices=(1, 2, 3),
+                    help="force kernel instead of the autotuner's choice: 1 candidate scan, 2 neighbour walk, 3 walk with loads one pair ahead")
     ap.add_argument("--sweep", action="store_true", help="print per-kernel times for every tuning variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
@@ -287,7 +287,7 @@ def main():
     if args.sweep:
         eng.step(2)
         saved = eng.get_tuning()
-        for fk in (V.FORCE_SCAN, V.FORCE_WALK):
+        for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD):
             for pb in (0, 2048):
                 eng.set_tuning(force_kernel=fk, persist_blocks=pb)
                 pr = eng.profile(3)

@@ -124,6 +124,7 @@ typedef struct VxlSphTuning {
 #define VXL_SPH_FORCE_DEFAULT 0u
 #define VXL_SPH_FORCE_SCAN 1u
 #define VXL_SPH_FORCE_WALK 2u
+#define VXL_SPH_FORCE_WALK_AHEAD 3u  /* the walk with each pair's loads issued one pair ahead */
 VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t);
 VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out);
 /* Autotuner: times every variant for `substeps` real substeps each (results

@@ -153,10 +153,10 @@ public:
     uint64_t substeps_done() const { return substeps_done_; }
 
     // Launch variants (autotuned; every variant gives identical results).
-    enum : uint32_t { kForceScan = 1, kForceWalk = 2 };
+    enum : uint32_t { kForceScan = 1, kForceWalk = 2, kForceWalkAhead = 3 };
     struct Tuning {
         uint32_t persist_blocks = 0;         // grid-stride grid size cap for slot kernels (0 = none)
-        uint32_t force_kernel = kForceWalk;  // kForceScan | kForceWalk
+        uint32_t force_kernel = kForceWalk;  // kForceScan | kForceWalk | kForceWalkAhead
     };
     // 0 in a kernel field selects its default.
     void set_tuning(const Tuning &t);

@@ -813,6 +813,8 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     }
     if (tuning_.force_kernel == kForceScan)
         hipLaunchKernelGGL(k_force_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    else if (tuning_.force_kernel == kForceWalkAhead)
+        hipLaunchKernelGGL(k_force_t<2>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     else
         hipLaunchKernelGGL(k_force_t<1>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("force");
@@ -931,7 +933,7 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
     }
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
     std::vector<Tuning> space;
-    for (uint32_t fk : {(uint32_t)kForceScan, (uint32_t)kForceWalk})
+    for (uint32_t fk : {(uint32_t)kForceScan, (uint32_t)kForceWalk, (uint32_t)kForceWalkAhead})
         for (uint32_t pb : {0u, 8u * cus}) {
             Tuning t;
             t.force_kernel = fk;
@@ -981,7 +983,7 @@ uint32_t Engine::box_reduce_grid() const {
 void Engine::set_tuning(const Tuning &t) {
     tuning_ = t;
     if (tuning_.force_kernel == 0) tuning_.force_kernel = Tuning{}.force_kernel;
-    if (tuning_.force_kernel > kForceWalk) throw std::invalid_argument("unknown kernel variant");
+    if (tuning_.force_kernel > kForceWalkAhead) throw std::invalid_argument("unknown kernel variant");
     invalidate_graph();
 }
 

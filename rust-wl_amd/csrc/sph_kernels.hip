@@ -513,12 +513,24 @@ struct DensAccWalls {
 // clamping: the arrays are padded). SELF: the range may hold i; PART: the
 // group may run past j1; MASK: set the mask bits (candidate k = j - j0 < 32),
 // else count the accepted ones.
-template <bool SELF, bool PART, bool MASK, class Acc>
-__device__ __forceinline__ void dens_group(const float4 *__restrict__ sA, const float4 ai, uint32_t i, uint32_t jb,
-                                           uint32_t j0, uint32_t j1, float h2, float k6, Acc &da, uint32_t &mk,
-                                           uint32_t &cnt) {
+// Where a row's candidate records come from: the grid-ordered array in global
+// memory (padded: groups of four load past a range end), or the wave's LDS
+// copy of the rows' union (staged from slot `base` on, padded the same way).
+struct GlobalSrc {
+    const float4 *__restrict__ sA;
+    __device__ __forceinline__ const float4 *group(uint32_t jb) const { return at(sA, jb); }
+};
+struct LdsSrc {
+    const float4 *lds;
+    uint32_t base;
+    __device__ __forceinline__ const float4 *group(uint32_t jb) const { return lds + (jb - base); }
+};
+
+template <bool SELF, bool PART, bool MASK, class Acc, class Src>
+__device__ __forceinline__ void dens_group(const Src &src, const float4 ai, uint32_t i, uint32_t jb, uint32_t j0,
+                                           uint32_t j1, float h2, float k6, Acc &da, uint32_t &mk, uint32_t &cnt) {
     constexpr uint32_t U = 4;
-    const float4 *p = at(sA, jb);
+    const float4 *p = src.group(jb);
     float4 aj[U];
 #pragma unroll
     for (uint32_t u = 0; u < U; ++u) aj[u] = p[u];
@@ -559,21 +571,25 @@ __device__ __forceinline__ void dens_group(const float4 *__restrict__ sA, const 
 
 // One row of the fused density loop; returns the mask of its first 32
 // candidates (those past them are counted into cnt).
-template <bool SELF, class Acc>
-__device__ __forceinline__ uint32_t dens_row(const float4 *__restrict__ sA, const float4 ai, uint32_t i, uint32_t j0,
-                                             uint32_t j1, float h2, float k6, Acc &da, uint32_t &cnt) {
+template <bool SELF, class Acc, class Src>
+__device__ __forceinline__ uint32_t dens_row(const Src &src, const float4 ai, uint32_t i, uint32_t j0, uint32_t j1,
+                                             float h2, float k6, Acc &da, uint32_t &cnt) {
     uint32_t mk = 0;
     const uint32_t jm = min(j1, j0 + 32u);
     uint32_t jb = j0;
-    for (; jb + 4u <= jm; jb += 4u) dens_group<SELF, false, true>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+    for (; jb + 4u <= jm; jb += 4u) dens_group<SELF, false, true>(src, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
     if (jb < jm) {
-        dens_group<SELF, true, true>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+        dens_group<SELF, true, true>(src, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
         jb += 4u;
     }
-    for (; jb + 4u <= j1; jb += 4u) dens_group<SELF, false, false>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
-    if (jb < j1) dens_group<SELF, true, false>(sA, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+    for (; jb + 4u <= j1; jb += 4u) dens_group<SELF, false, false>(src, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
+    if (jb < j1) dens_group<SELF, true, false>(src, ai, i, jb, j0, j1, h2, k6, da, mk, cnt);
     return mk;
 }
+
+// Records one wave may stage per neighbour row (the union of its lanes' row
+// windows); larger unions read global memory.
+constexpr uint32_t kStageRecords = 256;
 
 // Density (fluid_density.rs:17-70) + Tait (fluid_force.rs:11-29, gamma = 7)
 // + the per-particle pressure term p / (rho * rho) used by the force. The
@@ -584,6 +600,7 @@ __device__ __forceinline__ uint32_t dens_row(const float4 *__restrict__ sA, cons
 // their neighbours (fluid_density.rs:24-51, :76-121).
 template <bool WALLS>
 __device__ __forceinline__ void density_pass(const Pool &P) {
+    __shared__ float4 dens_stage[kBlock / 64][kStageRecords];
     if (P.ctl->err) return;
     const uint32_t stride = P.nblocks * kBlock;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
@@ -609,15 +626,68 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
                 summ |= row_summary(r, left, tail);
             }
         };
+        const GlobalSrc gsrc{sA};
+        // One row of the full scan from src; only the particle's own row (r = 4,
+        // uniform across the wave) can hold i itself.
+        auto scan_row = [&](auto &da, const auto &src, uint32_t r, bool left, uint32_t j0, uint32_t j1) {
+            const uint32_t mk = r == 4u ? dens_row<true>(src, ai, i, j0, j1, h2, k6, da, cnt)
+                                        : dens_row<false>(src, ai, i, j0, j1, h2, k6, da, cnt);
+            close_row(r, left, j0, j1, mk);
+        };
+        // A full wave whose lanes share one (y, z) cell row has the same nine
+        // neighbour rows in every lane; only the x windows differ. Each row's
+        // union of windows is one contiguous slot range, loaded into LDS once
+        // for the wave; every lane then scans its own window from there in
+        // the same order. (Vector memory data return bounds this kernel.)
+        const uint32_t lane = threadIdx.x & 63u;
+        const bool full_wave = __builtin_amdgcn_read_exec() == ~0ull;
+        const int cy0 = __builtin_amdgcn_readfirstlane(cur.cy), cz0 = __builtin_amdgcn_readfirstlane(cur.cz);
+        const bool staged = full_wave && __all(cur.cy == cy0 && cur.cz == cz0);
+        float4 *stage = dens_stage[threadIdx.x >> 6];
         // The full scan: every candidate tested, sums taken through selects.
         auto full_scan = [&](auto &da) {
-            for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
-                // Only the particle's own row (r = 4, uniform across the wave)
-                // can hold i itself.
-                const uint32_t mk = r == 4u ? dens_row<true>(sA, ai, i, j0, j1, h2, k6, da, cnt)
-                                            : dens_row<false>(sA, ai, i, j0, j1, h2, k6, da, cnt);
-                close_row(r, left, j0, j1, mk);
-            });
+            if (staged) {
+                const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
+                const float lx = cur.fx * cur.fx, rx = (1.0f - cur.fx) * (1.0f - cur.fx);
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int z = cz0 + dz;
+                    if (z < 0 || z >= nz) continue;
+                    const float ez = dz < 0 ? cur.fz : (dz > 0 ? 1.0f - cur.fz : 0.0f);
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        const int y = cy0 + dy;
+                        if (y < 0 || y >= ny) continue;
+                        const uint32_t r = (uint32_t)(3 * (dz + 1) + dy + 1);
+                        // the lane's window, as for_row_windows forms it
+                        const float ey = dy < 0 ? cur.fy : (dy > 0 ? 1.0f - cur.fy : 0.0f);
+                        const float rd = ez * ez + ey * ey;
+                        const bool has = !(rd > cur.cut);  // the skip test of for_row_windows
+                        const int xl = (cur.cx > 0 && rd + lx <= cur.cut) ? cur.cx - 1 : cur.cx;
+                        const int xr = (cur.cx + 1 < nx && rd + rx <= cur.cut) ? cur.cx + 1 : cur.cx;
+                        const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+                        const uint32_t cf = row + (uint32_t)xl, cl = row + (uint32_t)xr;
+                        if (!__any(has)) continue;
+                        const uint32_t ulo = wave_min_u32(has ? cf : 0xffffffffu);
+                        const uint32_t uhi = wave_max_u32(has ? cl : 0u);
+                        const uint32_t base = P.start[ulo], n = P.start[uhi + 1u] - base;
+                        uint32_t j0 = 0, j1 = 0;
+                        if (has) {
+                            j0 = P.start[cf];
+                            j1 = P.start[cl + 1u];
+                        }
+                        if (n + 4u > kStageRecords) {
+                            if (has) scan_row(da, gsrc, r, xl < cur.cx, j0, j1);
+                            continue;
+                        }
+                        for (uint32_t k = lane; k < n + 4u; k += 64u) stage[k] = *at(sA, base + k);
+                        const LdsSrc lsrc{stage, base};
+                        if (has) scan_row(da, lsrc, r, xl < cur.cx, j0, j1);
+                    }
+                }
+            } else {
+                for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
+                    scan_row(da, gsrc, r, left, j0, j1);
+                });
+            }
             sum = da.sum;
             sum_b = da.sum_b;
         };
@@ -655,7 +725,7 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
                     const uint32_t jn = lo + min(P.fcount[c], hi - lo);
                     uint32_t unused = 0;
                     for (uint32_t jb = lo; jb < jn; jb += 4u)
-                        dens_group<false, true, false>(sA, ai, i, jb, lo, jn, h2, k6, da, unused, cnt);
+                        dens_group<false, true, false>(gsrc, ai, i, jb, lo, jn, h2, k6, da, unused, cnt);
                     lo = hi;
                 }
             });
@@ -672,8 +742,15 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_density(Pool P) { density_pass<false>(P); }
-__global__ __launch_bounds__(kBlock) void k_density_walls(Pool P) { density_pass<true>(P); }
+// Six waves per SIMD (at most 80 VGPRs): the kernel is bound by vector
+// memory and LDS traffic, so occupancy pays more than the registers the
+// compiler would otherwise spend on scheduling (it fits without spills).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_density(Pool P) {
+    density_pass<false>(P);
+}
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_density_walls(Pool P) {
+    density_pass<true>(P);
+}
 
 
 
@@ -732,6 +809,7 @@ __device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const f
 // WALK = 1: every lane walks the neighbours the density pass recorded (same
 // positions, same test), so the candidate scan is not repeated; the wave
 // evaluates one pair per lane per iteration.
+// WALK = 2: the same walk with the next pair's loads issued one pair ahead.
 template <uint32_t WALK>
 __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
     if (P.ctl->err) return;
@@ -771,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
                     }
                 });
             }
-        } else {
+        } else if constexpr (WALK == 1) {
             Cursor cur{};
             if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
             NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur);
@@ -781,6 +859,37 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
                 const int st = nb_step(w, P, g, cur, i, ai, h2, stride, j);
                 if (!__any(st != kWalkDone)) break;
                 if (st == kWalkPair) force_pair(f, ai, vi, ci2, *at(sA, j), *at(sV, j), *at(c2, j), fc);
+            }
+        } else {
+            // The walk of WALK = 1 with the next pair's records loaded before
+            // the current pair is evaluated (same pairs, same order).
+            Cursor cur{};
+            if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+            NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur);
+            const ForceConst fc = force_const(sp);
+            uint32_t jn = 0;
+            int stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
+            float4 an = ai, vn = vi;
+            float cn = 0.0f;
+            if (stn == kWalkPair) {
+                an = *at(sA, jn);
+                vn = *at(sV, jn);
+                cn = *at(c2, jn);
+            }
+            for (;;) {
+                const int st = stn;
+                if (!__any(st != kWalkDone)) break;
+                const float4 aj = an, vj = vn;
+                const float cj = cn;
+                if (st != kWalkDone) {
+                    stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
+                    if (stn == kWalkPair) {
+                        an = *at(sA, jn);
+                        vn = *at(sV, jn);
+                        cn = *at(c2, jn);
+                    }
+                }
+                if (st == kWalkPair) force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
             }
         }
         float ox = ai.x, oy = ai.y, oz = ai.z;
@@ -816,6 +925,7 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
 
 template __global__ void k_force_t<0>(Pool P);
 template __global__ void k_force_t<1>(Pool P);
+template __global__ void k_force_t<2>(Pool P);
 
 // Reaction on every boundary particle of a scene with reactions:
 // bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending

@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvxl_sph.so")
 ABI_VERSION = 1
 # Force kernel variants of VxlSphTuning (vxl_sph.h VXL_SPH_FORCE_*).
-FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK = 0, 1, 2
+FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK, FORCE_WALK_AHEAD = 0, 1, 2, 3
 
 STATUS = {
     0: "OK", 1: "NULL_POINTER", 2: "VERSION_MISMATCH", 3: "SIZE_MISMATCH", 4: "NOT_READY",
@@ -509,7 +509,7 @@ class Engine:
         return v.value
 
     def set_tuning(self, persist_blocks=0, force_kernel=FORCE_DEFAULT):
-        """force_kernel: FORCE_SCAN | FORCE_WALK (0 = default); persist_blocks:
+        """force_kernel: FORCE_SCAN | FORCE_WALK | FORCE_WALK_AHEAD (0 = default); persist_blocks:
         grid size cap of the per-slot kernels (0 = one block per 256 slots).
         Every choice gives identical results."""
         t = Tuning()

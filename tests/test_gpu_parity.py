@@ -279,7 +279,7 @@ def test_reference_akinci_canary_on_gpu():
     np.testing.assert_array_equal(bits(gft), bits(ft))
 
 
-@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK])
+@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD])
 @pytest.mark.parametrize("persist", [0, 7])
 def test_kernel_variants_bit_exact(force_kernel, persist):
     """Every autotuner choice (force: candidate scan or walk of the recorded
