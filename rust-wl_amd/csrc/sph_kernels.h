@@ -85,7 +85,8 @@ __global__ void k_canon_gather(Pool P);
 __global__ void k_density(Pool P);
 __global__ void k_density_walls(Pool P);  // pools with half-space walls: mirror ghosts in the sums
 template <uint32_t WALK>
-__global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks
+__global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks;
+                                      // 2: the walk, loads one pair ahead
 __global__ void k_bforce(Pool P);
 // Provider walls (scenes with WallSceneDev::mode != kWallsOff):
 // k_wall_band + k_wall_density after the density pass (table scenes: the

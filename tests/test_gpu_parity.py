@@ -281,11 +281,15 @@ def test_reference_akinci_canary_on_gpu():
 
 @pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD])
 @pytest.mark.parametrize("persist", [0, 7])
-def test_kernel_variants_bit_exact(force_kernel, persist):
+@pytest.mark.parametrize("budget", [1 << 21, 1 << 9])
+def test_kernel_variants_bit_exact(force_kernel, persist, budget):
     """Every autotuner choice (force: candidate scan or walk of the recorded
-    neighbours; one block per 256 slots or a grid-stride loop over a few
-    blocks) gives the oracle's state bit for bit."""
-    cfg, data, tank, d = small_scene(seed=13, block=(16, 12, 10), tank=(0.6, 0.5, 0.4))
+    neighbours, directly or with loads one pair ahead; one block per
+    256 slots or a grid-stride loop over a few blocks) gives the oracle's state
+    bit for bit. The 2^9 budget doubles the bin: cells of ~64 particles, rows
+    longer than 32 candidates (tail tests in the walk) and row unions longer than
+    the density pass's LDS stage."""
+    cfg, data, tank, d = small_scene(seed=13, block=(16, 12, 10), tank=(0.6, 0.5, 0.4), max_bins=budget)
     orc = oracle_dam_break(cfg, data, tank, d)
     eng = V.Engine(diagnostics=True)
     sid = eng.add(cfg, data, reactions=False)
