@@ -29,9 +29,9 @@ uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
 }  // namespace
 
 uint64_t Engine::footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins) {
-    // cur + spare (36 B each), grid-ordered copy 36, c2/press 8, sort 16,
-    // neighbour record 40 B per slot; counts + start 8 B per reserved cell.
-    const uint64_t slot_bytes = 36 + 36 + 36 + 8 + 16 + 40;
+    // cur + spare (36 B each), grid-ordered copy 36, density record 32, sort
+    // 16, neighbour record 40 B per slot; counts + start 8 B per reserved cell.
+    const uint64_t slot_bytes = 36 + 36 + 36 + 32 + 16 + 40;
     return slot_bytes * round_up(n, kBlock) + 8ull * cell_reserve(cells_hint, max_bins);
 }
 
@@ -143,6 +143,9 @@ uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, cons
         sc.pmass.assign(n, sc.params.mass);
     for (float m : sc.pmass)
         if (!(m > 0.0f) || !std::isfinite(m)) throw std::invalid_argument("particle masses must be positive and finite");
+    sc.params.fmass_uniform = 1u;
+    for (uint32_t i = 0; i < nf; ++i)
+        if (sc.pmass[i] != sc.params.mass) sc.params.fmass_uniform = 0u;
     for (uint32_t k = 0; k < nspans; ++k) {
         if (spans[k].start < nf || spans[k].end > n || spans[k].start > spans[k].end)
             throw std::invalid_argument("span outside the boundary segment");
@@ -267,10 +270,6 @@ const SceneSlot &Engine::scene(uint32_t id) const {
 
 void Engine::alloc_pool(uint32_t slots) {
     slots_ = slots;
-    auto F = [&](float *&p) {
-        p = dalloc<float>(slots);
-        pool_allocs_.push_back(p);
-    };
     auto U = [&](uint32_t *&p) {
         p = dalloc<uint32_t>(slots);
         pool_allocs_.push_back(p);
@@ -286,7 +285,8 @@ void Engine::alloc_pool(uint32_t slots) {
     F4(P_.sV); U(P_.sg);
     P_.part = dalloc<SceneBox>(slots / kBlock + 1);
     pool_allocs_.push_back(P_.part);
-    F(P_.c2); F(P_.press);
+    P_.fr = dalloc<float4>(2ull * slots);
+    pool_allocs_.push_back(P_.fr);
     U(P_.key); U(P_.slot); U(P_.items_m); U(P_.items_g);
     P_.nbm = dalloc<uint32_t>(9ull * slots);
     pool_allocs_.push_back(P_.nbm);

@@ -22,8 +22,11 @@ struct Pool {
     // state gathered into this substep's grid order
     float4 *sA, *sV;
     uint32_t *sg;
-    float *c2;     // p / (rho * rho)
-    float *press;  // p
+    // [2 * slots] the density pass's record of each particle, one 32-byte
+    // pair (x, y, z, c2), (v, rho) with c2 = p / (rho * rho) and rho negated
+    // for boundary particles (the sign is the class): the force pass loads a
+    // neighbour in one line. The only store of rho and c2.
+    float4 *fr;
     // accepted neighbours, written by the density pass, read by the force pass:
     //   nbm[r * nblocks * kBlock + i]  bit k = candidate j0 + k of (dz, dy) row r
     //                                  (r = 3 (dz + 1) + dy + 1) is a neighbour

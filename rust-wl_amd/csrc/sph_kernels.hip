@@ -53,6 +53,21 @@ __device__ __forceinline__ uint32_t row_summary(uint32_t r, bool left, bool tail
     return (1u << r) | (left ? 1u << (9u + r) : 0u) | (tail ? 1u << (18u + r) : 0u);
 }
 
+// The nine neighbour rows of the scene a block works on, kept in LDS: cell =
+// cell offset of row r from the cell (cx, cy - 1, cz - 1), mask = r * stride
+// (the row's offset in the mask array). Opening a row then costs one LDS read
+// instead of the row index arithmetic.
+struct RowTab {
+    uint32_t cell, mask;
+};
+
+__device__ __forceinline__ void row_table(RowTab *t, const SceneGrid &g, uint32_t stride) {
+    if (threadIdx.x < 9u) {
+        const uint32_t r = threadIdx.x, q = r / 3u;
+        t[r] = RowTab{q * g.dims[0] * g.dims[1] + (r - 3u * q) * g.dims[0], r * stride};
+    }
+}
+
 // Resumable walk over the neighbours the density pass accepted, in the
 // reference enumeration order: rows in ascending r, inside a row ascending
 // slot (= cell by cell, ascending original index).
@@ -61,17 +76,16 @@ struct NbWalk {
     uint32_t mk, j0;      // accepted candidates j0 + k (bit k) of the open row
     uint32_t jt, je;      // tail candidates [jt, je) of the open row, tested here
     uint32_t rowb;        // internal cell of (cx, cy - 1, cz - 1) (mod 2^32)
-    uint32_t nx, nxny;    // row strides of the scene grid
+    const RowTab *tab;
 };
 
-__device__ __forceinline__ NbWalk nb_begin(uint32_t summ, const SceneGrid &g, const Cursor &c) {
+__device__ __forceinline__ NbWalk nb_begin(uint32_t summ, const SceneGrid &g, const Cursor &c, const RowTab *tab) {
     NbWalk w;
     w.summ = summ;
     w.pend = summ & kRowsPresent;
     w.mk = w.j0 = w.jt = w.je = 0u;
-    w.nx = g.dims[0];
-    w.nxny = g.dims[0] * g.dims[1];
     w.rowb = g.cbase + ((uint32_t)(c.cz - 1) * g.dims[1] + (uint32_t)(c.cy - 1)) * g.dims[0] + (uint32_t)c.cx;
+    w.tab = tab;
     return w;
 }
 
@@ -94,11 +108,11 @@ __device__ __forceinline__ int nb_step(NbWalk &w, const Pool &P, const SceneGrid
         if (w.pend == 0u) return kWalkDone;
         const uint32_t r = (uint32_t)__builtin_ctz(w.pend);
         w.pend &= w.pend - 1u;
-        const uint32_t q = (r * 11u) >> 5;  // r / 3 for r < 9
-        const uint32_t row = w.rowb + q * w.nxny + (r - 3u * q) * w.nx;
+        const RowTab t = w.tab[r];
+        const uint32_t row = w.rowb + t.cell;
         const uint32_t first = row - ((w.summ >> (9u + r)) & 1u);
-        w.j0 = P.start[first];
-        w.mk = P.nbm[(size_t)r * stride + i];
+        w.j0 = *at(P.start, first);
+        w.mk = *at(P.nbm, t.mask + i);
         if ((w.summ >> (18u + r)) & 1u) {
             w.jt = w.j0 + 32u;
             w.je = P.start[row - (uint32_t)c.cx + row_last_cell(g, c, r) + 1u];
@@ -592,8 +606,9 @@ __device__ __forceinline__ uint32_t dens_row(const Src &src, const float4 ai, ui
 constexpr uint32_t kStageRecords = 256;
 
 // Density (fluid_density.rs:17-70) + Tait (fluid_force.rs:11-29, gamma = 7)
-// + the per-particle pressure term p / (rho * rho) used by the force. The
-// sums are taken while the candidates are tested (selects keep the
+// + the per-particle pressure term c2 = p / (rho * rho) used by the force,
+// written with the position and velocity as the particle's density record
+// (fr). The sums are taken while the candidates are tested (selects keep the
 // reference's accumulation order and bits). Records every fluid particle's
 // accepted neighbours (nbm, nbs) for the force pass. WALLS: the pool has
 // scenes with half-space walls, whose band particles add the mirror ghosts of
@@ -735,9 +750,9 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
         const float rho = fluid_i ? sp.mass * sum + sum_b : (-ai.w) * sp.w0 + sum;
         float p, c2;
         tait(sp, rho, p, c2);
-        P.sV[i].w = rho;
-        P.press[i] = p;
-        P.c2[i] = c2;
+        const float4 vi = P.sV[i];
+        P.fr[2u * i] = make_float4(ai.x, ai.y, ai.z, c2);
+        P.fr[2u * i + 1u] = make_float4(vi.x, vi.y, vi.z, fluid_i ? rho : -rho);
         if (P.ncount) P.ncount[i] = cnt;
     }
 }
@@ -798,6 +813,18 @@ __device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const f
     f.xz = f.xz + (vj.z - vi.z) * fx;
 }
 
+// Neighbour j of the force sum from its density record (r0, r1): A with the
+// mass (the scene's for a fluid neighbour when every fluid mass is the
+// scene's, else read from sA), V with rho, and c2.
+__device__ __forceinline__ void nb_record(const float4 r0, const float4 r1, uint32_t j, const float4 *__restrict__ sA,
+                                          float smass, bool own_mass, float4 &aj, float4 &vj, float &cj) {
+    float mj = smass;
+    if (own_mass || r1.w < 0.0f) mj = fabsf(at(sA, j)->w);
+    aj = make_float4(r0.x, r0.y, r0.z, mj);
+    vj = make_float4(r1.x, r1.y, r1.z, fabsf(r1.w));
+    cj = r0.w;
+}
+
 // Force (fluid_force.rs:165-217: gravity + spiky pressure + Monaghan AV, XSPH)
 // fused with the semi-implicit Euler + CFL clamp of fluid_step.rs:110-123.
 // Writes the next state into the cur arrays (this substep's grid order) and
@@ -812,27 +839,37 @@ __device__ __forceinline__ void force_pair(ForceAcc &f, const float4 ai, const f
 // WALK = 2: the same walk with the next pair's loads issued one pair ahead.
 template <uint32_t WALK>
 __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
+    __shared__ RowTab rowtab[9];
     if (P.ctl->err) return;
     const uint32_t stride = P.nblocks * kBlock;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
         const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
+        if constexpr (WALK != 0) {
+            __syncthreads();  // the previous block's walks are done with the table
+            row_table(rowtab, P.grid[s], stride);
+            __syncthreads();
+        }
         const uint32_t i = b * kBlock + threadIdx.x;
         const bool live = i < sp.pbase + sp.n;
+        // Own state: A (position, signed mass) and the density record
+        // (velocity, rho, c2).
         float4 ai = make_float4(0.0f, 0.0f, 0.0f, 0.0f), vi = ai;
+        float ci2 = 0.0f;
         if (live) {
             ai = P.sA[i];
-            vi = P.sV[i];
+            vi = P.fr[2u * i + 1u];
+            ci2 = P.fr[2u * i].w;
         }
         const bool fluid = live && ai.w > 0.0f;
         const SceneGrid &g = P.grid[s];
         const float4 *__restrict__ sA = P.sA;
-        const float4 *__restrict__ sV = P.sV;
-        const float *__restrict__ c2 = P.c2;
+        const float4 *__restrict__ fr = P.fr;
+        // A boundary neighbour, or any neighbour in a scene of unequal fluid
+        // masses, reads its mass from sA.
+        const bool own_mass = sp.fmass_uniform == 0u;
         ForceAcc f{sp.gx, sp.gy, sp.gz, 0.0f, 0.0f, 0.0f};
-        float ci2 = 0.0f;
-        if (fluid) ci2 = c2[i];
         const float h2 = sp.h2;
         if constexpr (WALK == 0) {
             if (fluid) {
@@ -841,55 +878,65 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
                 for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t j0, uint32_t j1) {
                     for (uint32_t j = j0; j < j1; ++j) {
                         if (j == i) continue;
-                        const float4 aj = sA[j];
-                        const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                        const float4 r0 = fr[2u * j];
+                        const float ddx = ai.x - r0.x, ddy = ai.y - r0.y, ddz = ai.z - r0.z;
                         const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                         if (!(r2 <= h2)) continue;
-                        force_pair(f, ai, vi, ci2, aj, sV[j], c2[j], fc);
+                        float4 aj, vj;
+                        float cj;
+                        nb_record(r0, fr[2u * j + 1u], j, sA, sp.mass, own_mass, aj, vj, cj);
+                        force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
                     }
                 });
             }
         } else if constexpr (WALK == 1) {
             Cursor cur{};
             if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
-            NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur);
+            NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur, rowtab);
             const ForceConst fc = force_const(sp);
             for (;;) {
                 uint32_t j = 0;
                 const int st = nb_step(w, P, g, cur, i, ai, h2, stride, j);
                 if (!__any(st != kWalkDone)) break;
-                if (st == kWalkPair) force_pair(f, ai, vi, ci2, *at(sA, j), *at(sV, j), *at(c2, j), fc);
+                if (st == kWalkPair) {
+                    float4 aj, vj;
+                    float cj;
+                    nb_record(*at(fr, 2u * j), *at(fr, 2u * j + 1u), j, sA, sp.mass, own_mass, aj, vj, cj);
+                    force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
+                }
             }
         } else {
-            // The walk of WALK = 1 with the next pair's records loaded before
+            // The walk of WALK = 1 with the next pair's record loaded before
             // the current pair is evaluated (same pairs, same order).
             Cursor cur{};
             if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
-            NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur);
+            NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur, rowtab);
             const ForceConst fc = force_const(sp);
             uint32_t jn = 0;
             int stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
-            float4 an = ai, vn = vi;
-            float cn = 0.0f;
+            float4 rn0 = ai, rn1 = vi;
             if (stn == kWalkPair) {
-                an = *at(sA, jn);
-                vn = *at(sV, jn);
-                cn = *at(c2, jn);
+                rn0 = *at(fr, 2u * jn);
+                rn1 = *at(fr, 2u * jn + 1u);
             }
             for (;;) {
                 const int st = stn;
                 if (!__any(st != kWalkDone)) break;
-                const float4 aj = an, vj = vn;
-                const float cj = cn;
+                const float4 r0 = rn0, r1 = rn1;
+                const uint32_t j = jn;
                 if (st != kWalkDone) {
                     stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
                     if (stn == kWalkPair) {
-                        an = *at(sA, jn);
-                        vn = *at(sV, jn);
-                        cn = *at(c2, jn);
+                        rn0 = *at(fr, 2u * jn);
+                        rn1 = *at(fr, 2u * jn + 1u);
                     }
                 }
-                if (st == kWalkPair) force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
+                if (st == kWalkPair) {
+                    float4 aj, vj;
+                    float cj;
+                    nb_record(r0, r1, j, sA, sp.mass, own_mass, aj, vj, cj);
+                    force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
+                }
             }
         }
         float ox = ai.x, oy = ai.y, oz = ai.z;
@@ -952,8 +999,8 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
             if (!dyn) continue;
         }
         const SceneGrid &g = P.grid[s];
-        const float4 vj = P.sV[j];
-        const float rho_j = vj.w, cj2 = P.c2[j], mj = -aj.w;
+        const float4 vj = P.fr[2u * j + 1u];
+        const float rho_j = -vj.w, cj2 = P.fr[2u * j].w, mj = -aj.w;  // a boundary record: rho negated
         const Cursor cur = make_cursor(g, sp.h, aj.x, aj.y, aj.z);
         const float h = sp.h, h2 = sp.h2, ks = sp.ks, alpha_c = sp.alpha_c;
         uint32_t kid[kReactMaxNeighbours];
@@ -968,8 +1015,8 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
                 const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
                 const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                 if (!(r2 <= h2)) continue;
-                const float4 vi = P.sV[i];
-                const float rho_i = vi.w, ci2 = P.c2[i], mi = ai.w;
+                const float4 vi = P.fr[2u * i + 1u];
+                const float rho_i = vi.w, ci2 = P.fr[2u * i].w, mi = ai.w;
                 const float r = sqrtf(r2);
                 const float t = h - r;
                 const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
@@ -1080,8 +1127,12 @@ __global__ __launch_bounds__(kBlock) void k_unsort_fields(Pool P, float *odens, 
         if (i >= sp.pbase + sp.n) continue;
         const uint32_t gi = P.sg[i];
         const uint32_t d = sp.pbase + gi;
-        odens[d] = P.sV[i].w;
-        opress[d] = P.press[i];
+        // rho from the density record, p from it by the same Tait step
+        const float rho = fabsf(P.fr[2u * i + 1u].w);
+        float p, c2;
+        tait(sp, rho, p, c2);
+        odens[d] = rho;
+        opress[d] = p;
         if (oncount && P.ncount) oncount[d] = P.ncount[i];
         if (oacc && P.acc && gi < sp.nf) {
             const float4 a = P.acc[i], x = P.xsph[i];

@@ -45,6 +45,7 @@ struct SceneParams {
     uint32_t react;      // boundary reactions: 0 none, kReactAll every span, kReactDynamic dynamic bodies' spans
     uint32_t rbody0;     // first dynamic rigid body of the scene (pool rigid table)
     uint32_t nrbody;     // dynamic rigid bodies of the scene (0: none)
+    uint32_t fmass_uniform;  // every fluid particle's mass is `mass`, bit for bit
 };
 
 // A dynamic Box body coupled to its scene's fluid (the rigid leg of

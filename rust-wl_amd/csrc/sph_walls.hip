@@ -99,12 +99,11 @@ __global__ __launch_bounds__(kBlock) void k_wall_density(Pool P) {
             }
         });
         if (acc == 0.0f) continue;
-        const float rho = P.sV[i].w + sp.mass * acc;
+        const float rho = P.fr[2u * i + 1u].w + sp.mass * acc;  // a fluid particle: rho positive
         float p, c2;
         tait(sp, rho, p, c2);
-        P.sV[i].w = rho;
-        P.press[i] = p;
-        P.c2[i] = c2;
+        P.fr[2u * i].w = c2;
+        P.fr[2u * i + 1u].w = rho;
     }
 }
 
