@@ -1,7 +1,8 @@
 """The reference's Akinci boundary-particle tests (crates/vxl-phys-fluid/src/
-tests.rs:387-604) through the library: floor density restored, fluid column
-held, buoyant reaction on a submerged box in the Tank provider, coupling
-deterministic. Each scene also matches the oracle bit for bit."""
+tests.rs:387-604, crates/vxl-phys-gpu/tests/frozen_boundary.rs) through the
+library: floor density restored, fluid column held, buoyant reaction on a
+submerged box in the Tank provider, coupling deterministic, boundary particles
+frozen. Each scene also matches the oracle bit for bit."""
 import numpy as np
 import pytest
 
@@ -132,3 +133,34 @@ def test_reference_boundary_coupling_is_deterministic():
     a, b = digest(), digest()
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x, y)
+
+
+def test_reference_frozen_boundary():
+    """crates/vxl-phys-gpu/tests/frozen_boundary.rs: a 13^3 lattice (2197 fluid
+    particles, 21 past a multiple of 64) stepped 5 ticks, then a Box of half 2
+    spacings submerged at the origin; 3 more ticks of 2 substeps leave every
+    boundary particle's position and velocity bit for bit as set, and the
+    fluid equal to the oracle."""
+    n = 13
+    ocfg = O.default_config(substeps=2, threads=2)
+    orc = O.System.lattice(ocfg, (-n * S * 0.5, 0.0, -n * S * 0.5), (n, n, n), S)
+    for _ in range(5):
+        orc.step(float(DT))
+    nf = len(orc.pos())
+    assert nf % 64 != 0
+    body = dict(id=0, half=(2 * S, 2 * S, 2 * S), pos=(0.0, 0.0, 0.0))
+    assert orc.set_boundary_boxes([body]) > 0
+    cfg = V.fluid_config(substeps=2, particle_mass=float(orc.mass), spacing=S, dt_tick=float(DT))
+    pos, vel = orc.pos(), orc.vel()
+    pm = orc.pmass()
+    eng = V.Engine()
+    sid = eng.add(cfg, V.SceneData(pos, vel, pm, nf, []), reactions=False)
+    eng.step(3)
+    for _ in range(3):
+        orc.step(float(DT))
+    gpos, gvel = eng.read_state(sid)
+    np.testing.assert_array_equal(bits(gpos[nf:]), bits(pos[nf:]))
+    np.testing.assert_array_equal(bits(gvel[nf:]), bits(vel[nf:]))
+    assert not np.array_equal(bits(gpos[:nf]), bits(pos[:nf]))  # the fluid moved
+    np.testing.assert_array_equal(bits(gpos), bits(orc.pos()))
+    np.testing.assert_array_equal(bits(gvel), bits(orc.vel()))
