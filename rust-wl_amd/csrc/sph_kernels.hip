@@ -653,11 +653,16 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
         // neighbour rows in every lane; only the x windows differ. Each row's
         // union of windows is one contiguous slot range, loaded into LDS once
         // for the wave; every lane then scans its own window from there in
-        // the same order. (Vector memory data return bounds this kernel.)
+        // the same order. A wave that runs from the end of cell row (y, z)
+        // into the start of (y + 1, z) stages too: a lane's row (dz, dy) is
+        // then row y + dy or y + 1 + dy of plane z + dz, and the two are
+        // consecutive in slot order, so the union is still one range (the
+        // tail of the one and the head of the other). (Vector memory data
+        // return bounds this kernel.)
         const uint32_t lane = threadIdx.x & 63u;
         const bool full_wave = __builtin_amdgcn_read_exec() == ~0ull;
         const int cy0 = __builtin_amdgcn_readfirstlane(cur.cy), cz0 = __builtin_amdgcn_readfirstlane(cur.cz);
-        const bool staged = full_wave && __all(cur.cy == cy0 && cur.cz == cz0);
+        const bool staged = full_wave && __all(cur.cz == cz0 && (cur.cy == cy0 || cur.cy == cy0 + 1));
         float4 *stage = dens_stage[threadIdx.x >> 6];
         // The full scan: every candidate tested, sums taken through selects.
         auto full_scan = [&](auto &da) {
@@ -669,13 +674,12 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
                     if (z < 0 || z >= nz) continue;
                     const float ez = dz < 0 ? cur.fz : (dz > 0 ? 1.0f - cur.fz : 0.0f);
                     for (int dy = -1; dy <= 1; ++dy) {
-                        const int y = cy0 + dy;
-                        if (y < 0 || y >= ny) continue;
+                        const int y = cur.cy + dy;  // the lane's own row
                         const uint32_t r = (uint32_t)(3 * (dz + 1) + dy + 1);
                         // the lane's window, as for_row_windows forms it
                         const float ey = dy < 0 ? cur.fy : (dy > 0 ? 1.0f - cur.fy : 0.0f);
                         const float rd = ez * ez + ey * ey;
-                        const bool has = !(rd > cur.cut);  // the skip test of for_row_windows
+                        const bool has = y >= 0 && y < ny && !(rd > cur.cut);  // the skips of for_row_windows
                         const int xl = (cur.cx > 0 && rd + lx <= cur.cut) ? cur.cx - 1 : cur.cx;
                         const int xr = (cur.cx + 1 < nx && rd + rx <= cur.cut) ? cur.cx + 1 : cur.cx;
                         const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
