@@ -11,7 +11,9 @@ namespace vxl {
 //
 // Particle records are packed for one 16-byte load per neighbour candidate:
 //   A = (x, y, z, m)   m = +pmass for fluid, -pmass for boundary particles
-//   V = (vx, vy, vz, rho)
+//   V = (vx, vy, vz, 0)
+// and the density pass writes each particle's 32-byte density record fr
+// (below), which holds rho.
 // Cell tables are stored with x fastest (internal cell = (z*ny + y)*nx + x),
 // so the three cells dx = -1, 0, +1 of one (dz, dy) row of the reference's
 // dz -> dy -> dx enumeration are one contiguous slot range.
