@@ -697,7 +697,12 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
                             if (has) scan_row(da, gsrc, r, xl < cur.cx, j0, j1);
                             continue;
                         }
+                        // The wave's LDS operations complete in order; the wave
+                        // barriers keep the compiler from moving the copy and the
+                        // reads of other lanes' records across each other.
+                        __builtin_amdgcn_wave_barrier();
                         for (uint32_t k = lane; k < n + 4u; k += 64u) stage[k] = *at(sA, base + k);
+                        __builtin_amdgcn_wave_barrier();
                         const LdsSrc lsrc{stage, base};
                         if (has) scan_row(da, lsrc, r, xl < cur.cx, j0, j1);
                     }
