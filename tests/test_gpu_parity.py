@@ -303,6 +303,51 @@ def test_kernel_variants_bit_exact(force_kernel, persist, budget):
     assert eng.state_hash(sid) == orc.state_hash()
 
 
+def test_unclamped_pressure_bit_exact():
+    """tensile_instability_suppression off (config.rs:16): negative pressures
+    and negative c2 in the density record, every force kernel, 6 steps against
+    the oracle bit for bit."""
+    cfg, data, tank, d = small_scene(seed=17, block=(14, 10, 9), tank=(0.5, 0.4, 0.4))
+    cfg.tensile_clamp = 0
+    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD):
+        orc = oracle_dam_break(cfg, data, tank, d)
+        eng = V.Engine(diagnostics=True)
+        sid = eng.add(cfg, data, reactions=False)
+        eng.set_tuning(force_kernel=fk)
+        for _ in range(6):
+            eng.step(1)
+            orc.step(cfg.dt_tick)
+        f = eng.read_fields(sid, diagnostics=True)
+        assert (orc.press() < 0).any()  # the branch the clamp would cut
+        np.testing.assert_array_equal(bits(f["press"]), bits(orc.press()))
+        np.testing.assert_array_equal(bits(f["acc"]), bits(orc.acc()))
+        assert eng.state_hash(sid) == orc.state_hash()
+        eng.close()
+
+
+def test_speed_clamp_bit_exact():
+    """The CFL clamp of fluid_step.rs:110-123 (|v| > vmax = max_speed_frac * h
+    / dt -> rescaled to vmax): a block thrown at ~41 m/s against a limit of
+    32 m/s (h = 0.04 m, dt = 5e-4 s), 4 steps against the oracle bit for bit."""
+    cfg, data, tank, d = small_scene(seed=19, block=(10, 8, 8), tank=(0.6, 0.5, 0.5))
+    vel = data.vel.copy()
+    vel[:data.n_fluid] = np.array([36.0, -9.0, 18.0], np.float32)
+    data = V.SceneData(data.pos, vel, data.pmass, data.n_fluid, data.spans)
+    vmax = np.float32(np.float32(cfg.max_speed_frac) * np.float32(cfg.smoothing_radius)) / np.float32(cfg.dt_tick)
+    assert np.float32(np.linalg.norm(vel[0])) > vmax
+    orc = oracle_dam_break(cfg, data, tank, d)
+    eng = V.Engine()
+    sid = eng.add(cfg, data, reactions=False)
+    for _ in range(4):
+        eng.step(1)
+        orc.step(cfg.dt_tick)
+    pos, gvel = eng.read_state(sid)
+    speed = np.linalg.norm(gvel[:data.n_fluid].astype(np.float64), axis=1)
+    assert speed.max() <= vmax * (1 + 1e-5)  # clamped
+    np.testing.assert_array_equal(bits(gvel), bits(orc.vel()))
+    np.testing.assert_array_equal(bits(pos), bits(orc.pos()))
+
+
 def test_batch_equal_alone_and_oracle():
     reqs = V.sample_batch(11, 5, nmin=2000, nmax=30000)
     built = [V.request_scene(r) + (tuple(r.dam.tank), r.dam.d) for r in reqs]
