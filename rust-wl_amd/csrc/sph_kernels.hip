@@ -624,8 +624,17 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
         const SceneParams &sp = P.params[s];
         const uint32_t i = b * kBlock + threadIdx.x;
         if (i >= sp.pbase + sp.n) continue;
-        const SceneGrid &g = P.grid[s];
         const float4 ai = P.sA[i];
+        if (sp.nf == 0u) {
+            // A scene without fluid is not stepped (fluid_step.rs:92-95): its
+            // densities keep their initial zeros.
+            const float4 vi = P.sV[i];
+            P.fr[2u * i] = make_float4(ai.x, ai.y, ai.z, 0.0f);
+            P.fr[2u * i + 1u] = make_float4(vi.x, vi.y, vi.z, -0.0f);
+            if (P.ncount) P.ncount[i] = 0u;
+            continue;
+        }
+        const SceneGrid &g = P.grid[s];
         const bool fluid_i = ai.w > 0.0f;
         const Cursor cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
         const float h2 = sp.h2, k6 = sp.k6;
@@ -1136,10 +1145,11 @@ __global__ __launch_bounds__(kBlock) void k_unsort_fields(Pool P, float *odens, 
         if (i >= sp.pbase + sp.n) continue;
         const uint32_t gi = P.sg[i];
         const uint32_t d = sp.pbase + gi;
-        // rho from the density record, p from it by the same Tait step
+        // rho from the density record, p from it by the same Tait step (a
+        // scene without fluid is never stepped: both stay zero)
         const float rho = fabsf(P.fr[2u * i + 1u].w);
-        float p, c2;
-        tait(sp, rho, p, c2);
+        float p = 0.0f, c2;
+        if (sp.nf != 0u) tait(sp, rho, p, c2);
         odens[d] = rho;
         opress[d] = p;
         if (oncount && P.ncount) oncount[d] = P.ncount[i];

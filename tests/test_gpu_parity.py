@@ -348,6 +348,52 @@ def test_speed_clamp_bit_exact():
     np.testing.assert_array_equal(bits(pos), bits(orc.pos()))
 
 
+def test_ragged_scene_sizes_in_one_pool():
+    """Scenes of 1, 2, 63, 64, 65, 255, 256, 257 and 300 fluid particles (the
+    first n points of a jittered lattice), a scene of boundary particles only
+    and one fluid particle on a floor, stepped together: each equals its own
+    oracle bit for bit (positions, velocities, densities)."""
+    rng = np.random.default_rng(23)
+    S = np.float32(0.05)
+    grid = np.stack(np.meshgrid(*[np.arange(8)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    lattice = ((grid + 0.5) * S + rng.uniform(-0.004, 0.004, grid.shape)).astype(np.float32)
+    cfg = V.fluid_config(substeps=1, particle_mass=0.125, spacing=float(S), dt_tick=1.0 / 120.0)
+    mass = np.float32(cfg.particle_mass)
+    ocfg = O.default_config(substeps=1)
+    floor = dict(id=3, half=(0.3, 0.05, 0.3), pos=(0.2, -0.05, 0.2))
+    scenes = []
+    for nf in (1, 2, 63, 64, 65, 255, 256, 257, 300):
+        pos = lattice[:nf]
+        vel = rng.uniform(-0.2, 0.2, (nf, 3)).astype(np.float32)
+        scenes.append((V.SceneData(pos, vel, np.full(nf, mass, np.float32), nf, []), None))
+    for nf in (0, 1):
+        fpos, fvel = lattice[:nf] + np.float32(0.1), np.zeros((nf, 3), np.float32)
+        bpos, bvel, bpm, spans = V.box_particles(cfg, [floor], nf)
+        data = V.SceneData(np.concatenate([fpos, bpos]), np.concatenate([fvel, bvel]),
+                           np.concatenate([np.full(nf, mass, np.float32), bpm]), nf, spans)
+        scenes.append((data, floor))
+    eng = V.Engine()
+    ids, orcs = [], []
+    for data, body in scenes:
+        ids.append(eng.add(cfg, data, reactions=False))
+        nf = data.n_fluid
+        o = O.System.particles(ocfg, data.pos[:nf], data.vel[:nf], mass, S)
+        if body:
+            o.set_boundary_boxes([body])
+        np.testing.assert_array_equal(bits(o.pos()), bits(data.pos))
+        orcs.append(o)
+    eng.step(5)
+    for o in orcs:
+        for _ in range(5):
+            o.step(float(cfg.dt_tick))
+    for sid, o, (data, _) in zip(ids, orcs, scenes):
+        pos, vel = eng.read_state(sid)
+        dens = eng.read_fields(sid)["dens"]
+        np.testing.assert_array_equal(bits(pos), bits(o.pos()), err_msg=f"n_fluid {data.n_fluid}")
+        np.testing.assert_array_equal(bits(vel), bits(o.vel()), err_msg=f"n_fluid {data.n_fluid}")
+        np.testing.assert_array_equal(bits(dens), bits(o.dens()), err_msg=f"n_fluid {data.n_fluid}")
+
+
 def test_batch_equal_alone_and_oracle():
     reqs = V.sample_batch(11, 5, nmin=2000, nmax=30000)
     built = [V.request_scene(r) + (tuple(r.dam.tank), r.dam.d) for r in reqs]
