@@ -234,9 +234,13 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P) {
                 bin *= 2.0f;
             }
             SceneGrid &g = P.grid[s];
-            // A box the rule cannot bring under budget (non-finite extent)
-            // abandons the substep instead of indexing past the tables.
-            if ((uint64_t)dims[0] * dims[1] * dims[2] > sp.max_bins) {
+            // A box with a non-finite corner (NaN or infinite positions; the
+            // rule would give it a 1 x 1 x 1 grid and every particle one cell),
+            // or one the rule cannot bring under budget, abandons the substep
+            // instead of indexing past the tables.
+            bool finite = true;
+            for (int a = 0; a < 3; ++a) finite = finite && isfinite(lo[a]) && isfinite(hi[a]);
+            if (!finite || (uint64_t)dims[0] * dims[1] * dims[2] > sp.max_bins) {
                 atomicOr(&bad, 1u);
                 dims[0] = dims[1] = dims[2] = 1;
             }

@@ -394,6 +394,22 @@ def test_ragged_scene_sizes_in_one_pool():
         np.testing.assert_array_equal(bits(dens), bits(o.dens()), err_msg=f"n_fluid {data.n_fluid}")
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf])
+def test_non_finite_positions_abandon_the_substep(bad):
+    """A NaN or infinite position makes the scene's particle box non-finite;
+    the substep is abandoned on the device and the host reports it (rather
+    than binning every particle into one cell)."""
+    cfg, data, tank, d = small_scene(seed=3)
+    pos = data.pos.copy()
+    pos[5, 1] = bad
+    eng = V.Engine()
+    eng.add(cfg, V.SceneData(pos, data.vel, data.pmass, data.n_fluid, data.spans), reactions=False)
+    eng.step(1)
+    with pytest.raises(V.VxlSphError, match="not finite"):
+        eng.sync()
+    eng.close()
+
+
 def test_batch_equal_alone_and_oracle():
     reqs = V.sample_batch(11, 5, nmin=2000, nmax=30000)
     built = [V.request_scene(r) + (tuple(r.dam.tank), r.dam.d) for r in reqs]
