@@ -84,7 +84,7 @@ def cpu_baseline(cfg, data, steps, threads):
     return nf * steps / dt, dt
 
 
-def bench_batched(V, args):
+def bench_batched(V, args, one_at_a_time=True):
     """64 uneven dam-break scenes in one pool (one launch per stage, one graph
     per step) against the same scenes stepped one at a time."""
     reqs = V.sample_batch(4, args.batch_scenes)
@@ -92,7 +92,11 @@ def bench_batched(V, args):
     nf_total = sum(d.n_fluid for _, d in built)
     eng = V.Engine()
     ids = [eng.add(c, d, reactions=False) for c, d in built]
-    tuned = eng.autotune(2)
+    if args.force_kernel is not None:
+        eng.set_tuning(force_kernel=args.force_kernel)
+        tuned = eng.get_tuning()
+    else:
+        tuned = eng.autotune(2)
     eng.step(3)
     eng.sync()
     t0 = time.perf_counter()
@@ -103,6 +107,19 @@ def bench_batched(V, args):
     mix_hash = eng.state_hash(ids[0])
     eng.close()
     batched_ups = nf_total * args.batch_steps / wall
+    pooled = {"scenes": len(built), "fluid_particles": nf_total,
+              "total_particles": sum(d.n for _, d in built),
+              "value": batched_ups, "unit": "particle-updates/s",
+              "ms_per_step": 1e3 * wall / args.batch_steps,
+              "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+              "tuning": tuned,
+              "workload": "seeded set: fluid counts log-uniform [8192, 524288], tank x,z ~ U[1,4] m, y = 1 m, "
+                          "fill ~ U[0.1,0.5], c0 ~ U[15,30], alpha ~ U[0.01,0.05], d = 0.02 m, dt = 5e-4 s"}
+    if not one_at_a_time:
+        print(f"# batched: {len(built)} scenes, {nf_total} fluid, {1e3 * wall / args.batch_steps:.3f} ms/step "
+              f"(p50 {pooled['step_latency_ms']['p50']:.3f} / p99 {pooled['step_latency_ms']['p99']:.3f} ms)",
+              file=sys.stderr, flush=True)
+        return pooled
     # One scene at a time through the same engine code (plain single-scene path).
     seq_ms = 0.0
     one = V.Engine()
@@ -119,15 +136,9 @@ def bench_batched(V, args):
     seq_ups = nf_total / (seq_ms * 1e-3)
     print(f"# batched: {len(built)} scenes, {nf_total} fluid, {1e3 * wall / args.batch_steps:.3f} ms/step "
           f"vs one-at-a-time {seq_ms:.3f} ms -> {batched_ups / seq_ups:.2f}x", file=sys.stderr, flush=True)
-    return {"scenes": len(built), "fluid_particles": nf_total,
-            "total_particles": sum(d.n for _, d in built),
-            "value": batched_ups, "unit": "particle-updates/s",
-            "ms_per_step": 1e3 * wall / args.batch_steps,
-            "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-            "one_at_a_time": {"value": seq_ups, "ms_per_step_all_scenes": seq_ms},
-            "speedup_vs_one_at_a_time": batched_ups / seq_ups, "tuning": tuned,
-            "workload": "seeded set: fluid counts log-uniform [8192, 524288], tank x,z ~ U[1,4] m, y = 1 m, "
-                        "fill ~ U[0.1,0.5], c0 ~ U[15,30], alpha ~ U[0.01,0.05], d = 0.02 m, dt = 5e-4 s"}
+    pooled.update({"one_at_a_time": {"value": seq_ups, "ms_per_step_all_scenes": seq_ms},
+                   "speedup_vs_one_at_a_time": batched_ups / seq_ups})
+    return pooled
 
 
 def bench_served(V, args):
@@ -238,6 +249,9 @@ def main():
                     help="force kernel instead of the autotuner's choice: 1 candidate scan, 2 neighbour walk, 3 walk with loads one pair ahead")
     ap.add_argument("--sweep", action="store_true", help="print per-kernel times for every tuning variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run only the pooled 64-scene step (no 10M scene, no one-at-a-time comparison) and print "
+                         "its figures as one JSON line; for profiling the batched kernels")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=30)
@@ -267,6 +281,10 @@ def main():
         sys.exit(2)
 
     import vxl_sph as V
+
+    if args.only_batch:
+        print(json.dumps({"batched": bench_batched(V, args, one_at_a_time=False)}), flush=True)
+        return
 
     d = 0.01
     tank = (4.0, 2.5, 2.5)
