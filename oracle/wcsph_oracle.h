@@ -8,6 +8,13 @@
  *
  * Every function cites the reference file:line whose operation order it
  * follows. All arithmetic is IEEE f32, built with -ffp-contract=off.
+ *
+ * Pins: every function is checked against the reference's known answers,
+ * recorded hashes and readouts, its own tests' invariants or closed forms
+ * (tests/test_oracle.py, test_golden.py, test_walls_oracle.py,
+ * test_reference_boundary_oracle.py). One branch has no such check and its
+ * parity is unpinned: the Tait equation for gamma != 7 (the reference's
+ * powf, not bit-reproducible here; the product rejects gamma != 7).
  */
 #ifndef WCSPH_ORACLE_H
 #define WCSPH_ORACLE_H
