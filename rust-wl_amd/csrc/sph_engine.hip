@@ -29,9 +29,9 @@ uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
 }  // namespace
 
 uint64_t Engine::footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins) {
-    // cur + spare (36 B each), grid-ordered copy 36, density record 32, sort
-    // 16, neighbour record 40 B per slot; counts + start 8 B per reserved cell.
-    const uint64_t slot_bytes = 36 + 36 + 36 + 32 + 16 + 40;
+    // cur + spare (36 B each), grid-ordered A + gid 20, density record 32,
+    // sort 16, neighbour record 40 B per slot; counts + start 8 B per reserved cell.
+    const uint64_t slot_bytes = 36 + 36 + 20 + 32 + 16 + 40;
     return slot_bytes * round_up(n, kBlock) + 8ull * cell_reserve(cells_hint, max_bins);
 }
 
@@ -282,7 +282,7 @@ void Engine::alloc_pool(uint32_t slots) {
     F4(spare_cA_); F4(spare_cV_); U(spare_cg_);
     P_.sA = dalloc<float4>((size_t)slots + kReadPad);
     pool_allocs_.push_back(P_.sA);
-    F4(P_.sV); U(P_.sg);
+    U(P_.sg);
     P_.part = dalloc<SceneBox>(slots / kBlock + 1);
     pool_allocs_.push_back(P_.part);
     P_.fr = dalloc<float4>(2ull * slots);

@@ -12,8 +12,9 @@ namespace vxl {
 // Particle records are packed for one 16-byte load per neighbour candidate:
 //   A = (x, y, z, m)   m = +pmass for fluid, -pmass for boundary particles
 //   V = (vx, vy, vz, 0)
-// and the density pass writes each particle's 32-byte density record fr
-// (below), which holds rho.
+// In grid order the gather writes A and the position and velocity halves of
+// each particle's 32-byte density record fr (below), to which the density
+// pass adds c2 and rho.
 // Cell tables are stored with x fastest (internal cell = (z*ny + y)*nx + x),
 // so the three cells dx = -1, 0, +1 of one (dz, dy) row of the reference's
 // dz -> dy -> dx enumeration are one contiguous slot range.
@@ -22,7 +23,7 @@ struct Pool {
     float4 *cA, *cV;
     uint32_t *cg;
     // state gathered into this substep's grid order
-    float4 *sA, *sV;
+    float4 *sA;
     uint32_t *sg;
     // [2 * slots] the density pass's record of each particle, one 32-byte
     // pair (x, y, z, c2), (v, rho) with c2 = p / (rho * rho) and rho negated

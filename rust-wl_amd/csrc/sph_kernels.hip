@@ -449,9 +449,13 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
             nfl += gk < sp.nf ? 1u : 0u;
         }
         const uint32_t d = s0 + rank;
-        P.sA[d] = P.cA[m];
-        P.sV[d] = P.cV[m];
+        const float4 a = P.cA[m], v = P.cV[m];
+        P.sA[d] = a;
         P.sg[d] = g;
+        // the position and velocity halves of the density record (the density
+        // pass adds c2 and rho)
+        P.fr[2u * d] = make_float4(a.x, a.y, a.z, 0.0f);
+        P.fr[2u * d + 1u] = make_float4(v.x, v.y, v.z, 0.0f);
         // Fluid items (smaller indices) head the cell; the cell's first item
         // records how many there are (stale for empty cells: readers clamp).
         if (rank == 0) P.fcount[c] = nfl;
@@ -631,10 +635,8 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
         const float4 ai = P.sA[i];
         if (sp.nf == 0u) {
             // A scene without fluid is not stepped (fluid_step.rs:92-95): its
-            // densities keep their initial zeros.
-            const float4 vi = P.sV[i];
-            P.fr[2u * i] = make_float4(ai.x, ai.y, ai.z, 0.0f);
-            P.fr[2u * i + 1u] = make_float4(vi.x, vi.y, vi.z, -0.0f);
+            // densities keep their initial zeros (the gather left the record's
+            // rho and c2 at zero).
             if (P.ncount) P.ncount[i] = 0u;
             continue;
         }
@@ -772,9 +774,8 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
         const float rho = fluid_i ? sp.mass * sum + sum_b : (-ai.w) * sp.w0 + sum;
         float p, c2;
         tait(sp, rho, p, c2);
-        const float4 vi = P.sV[i];
-        P.fr[2u * i] = make_float4(ai.x, ai.y, ai.z, c2);
-        P.fr[2u * i + 1u] = make_float4(vi.x, vi.y, vi.z, fluid_i ? rho : -rho);
+        P.fr[2u * i].w = c2;  // beside the position and velocity the gather wrote
+        P.fr[2u * i + 1u].w = fluid_i ? rho : -rho;
         if (P.ncount) P.ncount[i] = cnt;
     }
 }
