@@ -842,7 +842,7 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     if (nspans_) {
         hipLaunchKernelGGL(k_bforce, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
         M("bforce");
-        hipLaunchKernelGGL(k_reactions, dim3((nspans_ + 63) / 64), dim3(64), 0, stream_, P_);
+        hipLaunchKernelGGL(k_reactions, dim3(nspans_), dim3(kReactChunk), 0, stream_, P_);
         M("reactions");
     }
     hip_check(hipGetLastError(), "substep launch");

@@ -1083,34 +1083,50 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
 // Per span: F = sum bforce[k], tau += (pos[k] - origin) x f_k in index order
 // (fluid_boundary.rs:10-31), accumulated over the substeps of a tick and
 // scaled by 1/substeps at its end (fluid_step.rs:75-89).
-__global__ __launch_bounds__(64) void k_reactions(Pool P) {
+//
+// One wave per span. The terms (f_k and the cross products) do not depend on
+// the running sums: the lanes load 64 consecutive particles and form their
+// terms in LDS, and lane 0 adds them in index order while the lanes' loads of
+// the next 64 are in flight. Same operations, same order, same bits as a
+// serial fold.
+__global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
+    __shared__ float term[kReactChunk][8];
     if (P.ctl->err) return;
-    const uint32_t k = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t k = blockIdx.x;
     if (k >= P.nspans) return;
     const SpanDev sd = P.spans[k];
     if (!sd.active) return;
+    const uint32_t lane = threadIdx.x;
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
-    auto fold = [&](const float4 f, const float4 p) {
-        Fx = Fx + f.x; Fy = Fy + f.y; Fz = Fz + f.z;
-        const float rx = p.x - sd.origin[0], ry = p.y - sd.origin[1], rz = p.z - sd.origin[2];
-        Tx = Tx + (ry * f.z - rz * f.y);
-        Ty = Ty + (rz * f.x - rx * f.z);
-        Tz = Tz + (rx * f.y - ry * f.x);
-    };
-    // Serial in index order; loads are issued kU at a time ahead of the fold.
-    constexpr uint32_t kU = 16;
-    uint32_t q = sd.start;
-    for (; q + kU <= sd.end; q += kU) {
-        float4 f[kU], p[kU];
-#pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) {
-            f[u] = P.bf[q + u];
-            p[u] = P.bp[q + u];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kU; ++u) fold(f[u], p[u]);
+    float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p = f;
+    if (sd.start + lane < sd.end) {
+        f = P.bf[sd.start + lane];
+        p = P.bp[sd.start + lane];
     }
-    for (; q < sd.end; ++q) fold(P.bf[q], P.bp[q]);
+    for (uint32_t q = sd.start; q < sd.end; q += kReactChunk) {
+        const float rx = p.x - sd.origin[0], ry = p.y - sd.origin[1], rz = p.z - sd.origin[2];
+        float *t = term[lane];
+        t[0] = f.x; t[1] = f.y; t[2] = f.z;
+        t[3] = ry * f.z - rz * f.y;
+        t[4] = rz * f.x - rx * f.z;
+        t[5] = rx * f.y - ry * f.x;
+        const uint32_t qn = q + kReactChunk;
+        if (qn + lane < sd.end) {
+            f = P.bf[qn + lane];
+            p = P.bp[qn + lane];
+        }
+        __syncthreads();
+        if (lane == 0u) {
+            const uint32_t nq = min(kReactChunk, sd.end - q);
+            for (uint32_t u = 0; u < nq; ++u) {
+                const float *w = term[u];
+                Fx = Fx + w[0]; Fy = Fy + w[1]; Fz = Fz + w[2];
+                Tx = Tx + w[3]; Ty = Ty + w[4]; Tz = Tz + w[5];
+            }
+        }
+        __syncthreads();  // the terms are read before the next chunk's are written
+    }
+    if (lane != 0u) return;
     const uint32_t sub = sd.substeps;
     const uint32_t phase = (P.ctl->step - 1u) % sub;  // substep index inside the tick
     float *o = P.react + 6ull * k;
