@@ -1031,40 +1031,49 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
         uint32_t cnt = 0;
         bool over = false;
         for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t i0, uint32_t i1) {
-            for (uint32_t i = i0; i < i1; ++i) {
-                const float4 ai = P.sA[i];
-                if (!(ai.w > 0.0f)) continue;  // fluid neighbours only
-                // d = p_i - p_j as seen from the fluid particle i.
-                const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
-                const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                if (!(r2 <= h2)) continue;
-                const float4 vi = P.fr[2u * i + 1u];
-                const float rho_i = vi.w, ci2 = P.fr[2u * i].w, mi = ai.w;
-                const float r = sqrtf(r2);
-                const float t = h - r;
-                const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
-                const float denom = fmaxf(r, 1e-9f);
-                const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
-                const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
-                float cv = 0.0f;
-                if (vdn > 0.0f) {
-                    const float mu = (vdn * h) / (r2 + 0.01f * h2);
-                    cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+            for (uint32_t ib = i0; ib < i1; ib += 4u) {
+                // four candidates' records in flight at once (the grid-ordered
+                // array is padded past its end)
+                float4 a4[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) a4[u] = *at(P.sA, ib + u);
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) {
+                    const uint32_t i = ib + u;
+                    const float4 ai = a4[u];
+                    if (i >= i1 || !(ai.w > 0.0f)) continue;  // fluid neighbours only
+                    // d = p_i - p_j as seen from the fluid particle i.
+                    const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (!(r2 <= h2)) continue;
+                    const float4 vi = P.fr[2u * i + 1u];
+                    const float rho_i = vi.w, ci2 = P.fr[2u * i].w, mi = ai.w;
+                    const float r = sqrtf(r2);
+                    const float t = h - r;
+                    const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
+                    const float denom = fmaxf(r, 1e-9f);
+                    const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
+                    const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
+                    float cv = 0.0f;
+                    if (vdn > 0.0f) {
+                        const float mu = (vdn * h) / (r2 + 0.01f * h2);
+                        cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+                    }
+                    const float q = (mi * (coef + cv)) / denom;
+                    if (cnt == kReactMaxNeighbours) {
+                        over = true;
+                        continue;
+                    }
+                    const uint32_t gi = P.sg[i];
+                    uint32_t k = cnt++;  // insertion by original index
+                    while (k > 0 && kid[k - 1] > gi) {
+                        kid[k] = kid[k - 1];
+                        kx[k] = kx[k - 1]; ky[k] = ky[k - 1]; kz[k] = kz[k - 1];
+                        --k;
+                    }
+                    kid[k] = gi;
+                    kx[k] = ddx * q; ky[k] = ddy * q; kz[k] = ddz * q;
                 }
-                const float q = (mi * (coef + cv)) / denom;
-                if (cnt == kReactMaxNeighbours) {
-                    over = true;
-                    continue;
-                }
-                const uint32_t gi = P.sg[i];
-                uint32_t k = cnt++;  // insertion by original index
-                while (k > 0 && kid[k - 1] > gi) {
-                    kid[k] = kid[k - 1];
-                    kx[k] = kx[k - 1]; ky[k] = ky[k - 1]; kz[k] = kz[k - 1];
-                    --k;
-                }
-                kid[k] = gi;
-                kx[k] = ddx * q; ky[k] = ddy * q; kz[k] = ddz * q;
             }
         });
         if (over) atomicOr(&P.ctl->overflow, 1u);

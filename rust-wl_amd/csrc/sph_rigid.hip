@@ -171,6 +171,8 @@ __global__ __launch_bounds__(64) void k_rigid(Pool P) {
 // world = p + q lp, velocity = v + w x (world - p)) and recompute the AABB of
 // every scene that has such bodies into box[parity], which this kernel resets
 // for k_box_reduce to refill.
+constexpr uint32_t kNoBody = 0xffffffffu;
+
 __global__ __launch_bounds__(kBlock) void k_body_refresh(Pool P) {
     if (P.ctl->err) return;
     const uint32_t par = P.ctl->step & 1u;
@@ -180,25 +182,13 @@ __global__ __launch_bounds__(kBlock) void k_body_refresh(Pool P) {
         if (sp.nrbody == 0u) continue;
         const uint32_t m = b * kBlock + threadIdx.x;
         const bool live = m < sp.pbase + sp.n;
-        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        // the body whose particle this slot holds, if any
+        uint32_t gid = 0, body = kNoBody;
         if (live) {
-            a = P.cA[m];
-            const uint32_t gid = P.cg[m];
-            if (gid >= sp.nf) {
-                for (uint32_t k = sp.rbody0; k < sp.rbody0 + sp.nrbody; ++k) {
-                    const RigidDev &B = P.rigid[k];
-                    if (gid < B.gstart || gid >= B.gend) continue;
-                    const float4 l = P.rlat[B.lat + (gid - B.gstart)];
-                    const V3 pos = ld3(B.pos);
-                    const Quat q{B.rot[0], B.rot[1], B.rot[2], B.rot[3]};
-                    const V3 wp = vadd(pos, qrot(q, v3(l.x, l.y, l.z)));
-                    const V3 wv = vadd(ld3(B.linvel), vcross(ld3(B.angvel), vsub(wp, pos)));
-                    a = make_float4(wp.x, wp.y, wp.z, a.w);
-                    P.cA[m] = a;
-                    const float4 v = P.cV[m];
-                    P.cV[m] = make_float4(wv.x, wv.y, wv.z, v.w);
-                }
-            }
+            gid = P.cg[m];
+            if (gid >= sp.nf)
+                for (uint32_t k = sp.rbody0; k < sp.rbody0 + sp.nrbody; ++k)
+                    if (gid >= P.rigid[k].gstart && gid < P.rigid[k].gend) body = k;
         }
         if (b == sp.pbase / kBlock && threadIdx.x == 0) {
             SceneBox &o = P.box[par * P.nscenes + s];
@@ -206,6 +196,23 @@ __global__ __launch_bounds__(kBlock) void k_body_refresh(Pool P) {
                 o.lo[ax] = 0xffffffffu;
                 o.hi[ax] = 0u;
             }
+        }
+        // A block without body particles keeps the partial box of its
+        // positions that the last force pass (or the upload) wrote.
+        if (!__syncthreads_or(body != kNoBody)) continue;
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (live) a = P.cA[m];
+        if (body != kNoBody) {
+            const RigidDev &B = P.rigid[body];
+            const float4 l = P.rlat[B.lat + (gid - B.gstart)];
+            const V3 pos = ld3(B.pos);
+            const Quat q{B.rot[0], B.rot[1], B.rot[2], B.rot[3]};
+            const V3 wp = vadd(pos, qrot(q, v3(l.x, l.y, l.z)));
+            const V3 wv = vadd(ld3(B.linvel), vcross(ld3(B.angvel), vsub(wp, pos)));
+            a = make_float4(wp.x, wp.y, wp.z, a.w);
+            P.cA[m] = a;
+            const float4 v = P.cV[m];
+            P.cV[m] = make_float4(wv.x, wv.y, wv.z, v.w);
         }
         dev::block_box(live, a.x, a.y, a.z, &P.part[b]);
     }
