@@ -1012,15 +1012,15 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
         if (!sp.react) continue;
         const uint32_t j = b * kBlock + threadIdx.x;
         if (j >= sp.pbase + sp.n) continue;
-        const float4 aj = P.sA[j];
-        if (aj.w > 0.0f) continue;
         const uint32_t gj = P.sg[j];
+        if (gj < sp.nf) continue;  // a fluid particle (the index prefix)
         if (sp.react == kReactDynamic) {  // only dynamic bodies consume a reaction
             bool dyn = false;
             for (uint32_t k = sp.rbody0; k < sp.rbody0 + sp.nrbody; ++k)
                 dyn = dyn || (gj >= P.rigid[k].gstart && gj < P.rigid[k].gend);
             if (!dyn) continue;
         }
+        const float4 aj = P.sA[j];
         const SceneGrid &g = P.grid[s];
         const float4 vj = P.fr[2u * j + 1u];
         const float rho_j = -vj.w, cj2 = P.fr[2u * j].w, mj = -aj.w;  // a boundary record: rho negated
