@@ -253,6 +253,9 @@ def main():
                     help="run only the pooled 64-scene step (no 10M scene, no one-at-a-time comparison) and print "
                          "its figures as one JSON line; for profiling the batched kernels")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--only-serve", action="store_true",
+                    help="run only the served stream and print its figures as one JSON line; for profiling the "
+                         "served pool's kernels")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -284,6 +287,9 @@ def main():
 
     if args.only_batch:
         print(json.dumps({"batched": bench_batched(V, args, one_at_a_time=False)}), flush=True)
+        return
+    if args.only_serve:
+        print(json.dumps({"served": bench_served(V, args)}), flush=True)
         return
 
     d = 0.01
