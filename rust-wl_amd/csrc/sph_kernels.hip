@@ -1099,7 +1099,7 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
 // the next 64 are in flight. Same operations, same order, same bits as a
 // serial fold.
 __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
-    __shared__ float term[kReactChunk][8];
+    __shared__ float term[kReactChunk][8];  // six terms per particle, rows padded to 32 bytes
     if (P.ctl->err) return;
     const uint32_t k = blockIdx.x;
     if (k >= P.nspans) return;

@@ -167,12 +167,12 @@ __global__ __launch_bounds__(64) void k_rigid(Pool P) {
     }
 }
 
+constexpr uint32_t kNoBody = 0xffffffffu;  // a slot that holds no body particle
+
 // Rebuild the dynamic bodies' particles from their pose (fluid_step.rs:32-39:
 // world = p + q lp, velocity = v + w x (world - p)) and recompute the AABB of
 // every scene that has such bodies into box[parity], which this kernel resets
 // for k_box_reduce to refill.
-constexpr uint32_t kNoBody = 0xffffffffu;
-
 __global__ __launch_bounds__(kBlock) void k_body_refresh(Pool P) {
     if (P.ctl->err) return;
     const uint32_t par = P.ctl->step & 1u;
