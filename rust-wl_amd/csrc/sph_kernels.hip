@@ -383,15 +383,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(Pool P, const uint32_t *t
     const uint32_t ntiles = (ctot + kScanTile - 1) / kScanTile;
     __shared__ uint32_t sh[kBlock];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        // Each thread owns kScanItems consecutive cells.
+        // Each thread owns kScanItems consecutive cells (two 16-byte loads
+        // when all of them are in range: base is a multiple of 8).
         const uint32_t base = t * kScanTile + threadIdx.x * kScanItems;
         uint32_t v[kScanItems];
         uint32_t local = 0;
-        for (uint32_t k = 0; k < kScanItems; ++k) {
-            const uint32_t c = base + k;
-            v[k] = c < ctot ? P.counts[c] : 0u;
-            local += v[k];
+        const bool whole = base + kScanItems <= ctot;
+        if (whole) {
+            const uint4 a = *(const uint4 *)(P.counts + base), b4 = *(const uint4 *)(P.counts + base + 4u);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b4.x; v[5] = b4.y; v[6] = b4.z; v[7] = b4.w;
+        } else {
+            for (uint32_t k = 0; k < kScanItems; ++k) v[k] = base + k < ctot ? P.counts[base + k] : 0u;
         }
+        for (uint32_t k = 0; k < kScanItems; ++k) local += v[k];
         sh[threadIdx.x] = local;
         __syncthreads();
         for (uint32_t o = 1; o < kBlock; o <<= 1) {
@@ -401,12 +406,25 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(Pool P, const uint32_t *t
             __syncthreads();
         }
         uint32_t run = tile_sums[t] + sh[threadIdx.x] - local;
-        for (uint32_t k = 0; k < kScanItems; ++k) {
-            const uint32_t c = base + k;
-            if (c < ctot) {
-                P.start[c] = run;
-                P.counts[c] = 0u;
+        if (whole) {
+            uint32_t st[kScanItems];
+            for (uint32_t k = 0; k < kScanItems; ++k) {
+                st[k] = run;
                 run += v[k];
+            }
+            // start is 16-byte aligned and base a multiple of 8
+            *(uint4 *)(P.start + base) = make_uint4(st[0], st[1], st[2], st[3]);
+            *(uint4 *)(P.start + base + 4u) = make_uint4(st[4], st[5], st[6], st[7]);
+            *(uint4 *)(P.counts + base) = make_uint4(0u, 0u, 0u, 0u);
+            *(uint4 *)(P.counts + base + 4u) = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            for (uint32_t k = 0; k < kScanItems; ++k) {
+                const uint32_t c = base + k;
+                if (c < ctot) {
+                    P.start[c] = run;
+                    P.counts[c] = 0u;
+                    run += v[k];
+                }
             }
         }
         __syncthreads();
