@@ -287,7 +287,9 @@ void Engine::alloc_pool(uint32_t slots) {
     pool_allocs_.push_back(P_.part);
     P_.fr = dalloc<float4>(2ull * slots);
     pool_allocs_.push_back(P_.fr);
-    U(P_.key); U(P_.slot); U(P_.items_m); U(P_.items_g);
+    U(P_.key); U(P_.slot);
+    P_.items = dalloc<uint2>(slots);
+    pool_allocs_.push_back(P_.items);
     P_.nbm = dalloc<uint32_t>(9ull * slots);
     pool_allocs_.push_back(P_.nbm);
     U(P_.nbs);

@@ -38,7 +38,8 @@ struct Pool {
     //           tail j0 + 32 .. end is tested again by the reader)
     uint32_t *nbm, *nbs;
     // sort
-    uint32_t *key, *slot, *items_m, *items_g;
+    uint32_t *key, *slot;
+    uint2 *items;              // [slots] (cur slot, gid) of each cell member in arrival order
     uint32_t *counts, *start;  // counts[ccap], start[ccap + 1]
     uint32_t *fcount;          // [ccap] fluid items heading each non-empty cell (this substep)
     // optional diagnostics (grid order; null = off)

@@ -440,8 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_place(Pool P) {
         const uint32_t m = b * kBlock + threadIdx.x;
         if (m >= sp.pbase + sp.n) continue;
         const uint32_t dst = P.start[P.key[m]] + P.slot[m];
-        P.items_m[dst] = m;
-        P.items_g[dst] = P.cg[m];
+        P.items[dst] = make_uint2(m, P.cg[m]);
     }
 }
 
@@ -456,13 +455,13 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
         const SceneParams &sp = P.params[s];
         const uint32_t u = b * kBlock + threadIdx.x;
         if (u >= sp.pbase + sp.n) continue;
-        const uint32_t m = P.items_m[u];
-        const uint32_t g = P.items_g[u];
+        const uint2 it = P.items[u];
+        const uint32_t m = it.x, g = it.y;
         const uint32_t c = P.key[m];
         const uint32_t s0 = P.start[c], e0 = P.start[c + 1];
         uint32_t rank = 0, nfl = 0;
         for (uint32_t k = s0; k < e0; ++k) {
-            const uint32_t gk = P.items_g[k];
+            const uint32_t gk = P.items[k].y;
             rank += gk < g ? 1u : 0u;
             nfl += gk < sp.nf ? 1u : 0u;
         }
