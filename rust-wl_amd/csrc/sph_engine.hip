@@ -30,9 +30,10 @@ uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
 
 uint64_t Engine::footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins) {
     // cur + spare (36 B each), grid-ordered A + gid 20, density record 32,
-    // sort 16, neighbour record 40 B per slot; counts + start 8 B per reserved cell.
+    // sort 16, neighbour record 40 B per slot; counts, start and fluid count
+    // 12 B per reserved cell.
     const uint64_t slot_bytes = 36 + 36 + 20 + 32 + 16 + 40;
-    return slot_bytes * round_up(n, kBlock) + 8ull * cell_reserve(cells_hint, max_bins);
+    return slot_bytes * round_up(n, kBlock) + 12ull * cell_reserve(cells_hint, max_bins);
 }
 
 Engine::Engine(int device) : device_(device) {
