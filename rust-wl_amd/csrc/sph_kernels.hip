@@ -716,9 +716,21 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
                         const int xr = (cur.cx + 1 < nx && rd + rx <= cur.cut) ? cur.cx + 1 : cur.cx;
                         const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
                         const uint32_t cf = row + (uint32_t)xl, cl = row + (uint32_t)xr;
-                        if (!__any(has)) continue;
-                        const uint32_t ulo = wave_min_u32(has ? cf : 0xffffffffu);
-                        const uint32_t uhi = wave_max_u32(has ? cl : 0u);
+                        const uint64_t hm = __ballot(has);
+                        if (hm == 0u) continue;
+                        // The lanes' own cells (row + cx) rise with the lane index
+                        // (grid order), so the union runs from the first lane's cell
+                        // (or the one left of it) to the last lane's cell (or the
+                        // one right of it): at most a cell wider on each side than
+                        // the exact union, which costs a few staged records only.
+                        const int fl = __builtin_ctzll(hm), ll = 63 - __builtin_clzll(hm);
+                        const uint32_t own = row + (uint32_t)cur.cx;
+                        const uint32_t ofl = (uint32_t)__builtin_amdgcn_readlane((int)own, fl);
+                        const int cxf = __builtin_amdgcn_readlane(cur.cx, fl);
+                        const uint32_t oll = (uint32_t)__builtin_amdgcn_readlane((int)own, ll);
+                        const int cxl = __builtin_amdgcn_readlane(cur.cx, ll);
+                        const uint32_t ulo = ofl - (cxf > 0 ? 1u : 0u);
+                        const uint32_t uhi = oll + (cxl + 1 < nx ? 1u : 0u);
                         const uint32_t base = P.start[ulo], n = P.start[uhi + 1u] - base;
                         uint32_t j0 = 0, j1 = 0;
                         if (has) {
