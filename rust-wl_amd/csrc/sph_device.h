@@ -8,14 +8,10 @@
 namespace vxl {
 namespace dev {
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-    return v;
-}
+// Minimum / maximum over the wave's active lanes (the device library's DPP
+// wave reductions; every caller has the whole wave active).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) { return __ockl_wfred_min_u32(v); }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return __ockl_wfred_max_u32(v); }
 
 // Block-wide AABB of the positions handed in by live lanes, written as this
 // block's partial box (ordered u32; the per-scene reduction runs after the
