@@ -382,6 +382,16 @@ VxlSphStatus vxl_sph_request_config(const VxlSphRequest *req, float dt, uint64_t
 VxlSphStatus vxl_sph_request_bodies(const VxlSphRequest *req, uint32_t cap, VxlSphRigidBox *out, uint32_t *count,
                                     VxlSphRigidConfig *rc);
 
+/* Peak device bytes of serving this request alone: its pool (state, sort,
+ * neighbour and cell arrays; see vxl_sph_pool_limits), readback staging and
+ * scene tables. No counterpart in the reference; for sizing budget_bytes. */
+VxlSphStatus vxl_sph_request_bytes(const VxlSphRequest *req, float dt, uint64_t max_bins, uint64_t *out);
+/* Capacity of one engine's pool: particle slots, grid cells, and the bytes per
+ * slot of the widest per-slot record the kernels address with 32-bit offsets
+ * (max_slots x record_bytes stays within 2^32). No counterpart in the reference
+ * (its packet has a fixed size, crates/vxl-phys-gpu/src/pipeline/gpu_types.rs:38-43). */
+VxlSphStatus vxl_sph_pool_limits(uint64_t *max_slots, uint64_t *max_cells, uint32_t *record_bytes);
+
 typedef struct VxlSphServeConfig {
     VxlSphHeader hdr;
     uint64_t budget_bytes;   /* device bytes the resident scenes may use */
@@ -402,8 +412,10 @@ typedef struct VxlSphServeStats {
     double loop_ms_p50, loop_ms_p99;
     double latency_s_p50, latency_s_p99;
     double wait_s_p50, wait_s_p99;
-    uint32_t peak_resident, reserved;
-    uint64_t peak_bytes, layouts;
+    uint32_t peak_resident;
+    uint32_t rejected;        /* requests whose scene alone exceeds budget_bytes (not served) */
+    uint64_t peak_bytes;      /* highest device bytes held (never above budget_bytes) */
+    uint64_t layouts;
 } VxlSphServeStats;
 
 typedef struct VxlSphServeResult {
@@ -413,8 +425,12 @@ typedef struct VxlSphServeResult {
 } VxlSphServeResult;
 
 /* Continuous batching: requests are built after they arrive, admitted in
- * arrival order as the HBM budget allows, co-stepped in one launch per stage
- * with every other resident request, and retired at step boundaries. */
+ * arrival order as the HBM budget allows (the engine's projected peak device
+ * bytes, pool growth included, stay within budget_bytes), co-stepped in one
+ * launch per stage with every other resident request, and retired at step
+ * boundaries. A request whose scene alone exceeds the budget is rejected:
+ * its result has steps = 0, the others are served, and the call returns
+ * VXL_SPH_CAPACITY with stats and results filled in. */
 VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSphRequest *reqs,
                            VxlSphServeStats *stats, VxlSphServeResult *results);
 

@@ -457,7 +457,8 @@ void force_pass(OrSystem *s) {
                 float vdn = -dot(vij, d);
                 if (vdn > 0.0f) {
                     float mu = (vdn * h) / (r2 + 0.01f * h2);
-                    float c = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+                    const float rho_bar = s->fault == 6 ? rho_i + rho_j : 0.5f * (rho_i + rho_j);
+                    float c = (mj * ((alpha_c * mu) / rho_bar)) * ((ks * t) * t);
                     acc = add(acc, mul(d, c / denom));
                 }
                 float tt = h2 - r2;
@@ -631,7 +632,7 @@ void substep(OrSystem *s, float dt) {
     for (size_t i = 0; i < nf; ++i) {
         V3 v = add(add(s->vel[i], mul(s->acc[i], dt)), mul(s->xsph[i], eps));
         float s2 = len2(v);
-        if (s2 > vmax2) v = mul(v, vmax / sqrtf(s2));
+        if (s2 > vmax2 && s->fault != 7) v = mul(v, vmax / sqrtf(s2));
         s->vel[i] = v;
         s->pos[i] = add(s->pos[i], mul(v, dt));
     }
@@ -901,7 +902,10 @@ extern "C" void or_set_threads(OrSystem *s, uint32_t threads) { s->cfg.threads =
 
 // Test-only: plant a plausible mistake so tests can show the pins catch it.
 // 1 = poly6 product reassociated k6*((t*t)*t); 2 = XSPH term dropped;
-// 3 = neighbour cells enumerated dx outer / dz inner.
+// 3 = neighbour cells enumerated dx outer / dz inner; 4 = mirror ghost
+// reflected the wrong way; 5 = coincident separation in descending order;
+// 6 = artificial viscosity over the density sum instead of its mean;
+// 7 = CFL speed clamp skipped.
 extern "C" void or_set_fault(OrSystem *s, int fault) {
     s->fault = fault;
     s->grid.fault = fault;

@@ -67,9 +67,11 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t nb) {
     return x * q + min(x, r) + k;
 }
 
-// Record `idx` of a per-slot array through a 32-bit byte offset from the
-// array base (a pool holds at most 2^28 slots; records are at most 16 bytes),
-// which lets the load use a scalar base and a 32-bit vector offset.
+// Record `idx` of a per-slot or per-cell array through a 32-bit byte offset
+// from the array base, which lets the load use a scalar base and a 32-bit
+// vector offset. Callers keep idx * sizeof(T) below 2^32: a pool holds at
+// most kMaxPoolSlots slots (density records: 2 x 16 bytes per slot) and
+// kMaxCells cells (sph_layout.h).
 template <class T>
 __device__ __forceinline__ const T *at(const T *__restrict__ base, uint32_t idx) {
     return (const T *)((const char *)base + idx * (uint32_t)sizeof(T));

@@ -127,6 +127,9 @@ public:
 
     // Overwrite the boundary segment pos/vel of a resident scene (index order).
     void upload_boundary(uint32_t id, uint32_t nb, const float *pos, const float *vel);
+    // Replace the torque origins of a scene's reaction spans (same count and
+    // ranges as at creation; the FluidStepper passes them every tick).
+    void set_span_origins(uint32_t id, const Span *spans, uint32_t nspans);
 
     // Advance every scene by `ticks` ticks of its own substeps (enqueue only).
     void step(uint32_t ticks);
@@ -173,10 +176,29 @@ public:
     // Device bytes a scene of n particles (cells_hint grid cells) adds to the
     // pool: per-slot state and sort arrays plus its share of the cell tables.
     static uint64_t footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins);
+    // Device bytes of a pool of `slots` slots and `ccap` cells (state, sort,
+    // neighbour and cell arrays as the layout allocates them).
+    static uint64_t pool_bytes(uint64_t slots, uint64_t ccap, bool diagnostics, bool reactions);
+    // Peak device bytes of this engine while the next layout adds a scene of
+    // n particles (the old and the new pool are both alive while resident
+    // scenes move), including the readback staging its read_state needs.
+    uint64_t projected_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions) const;
+    // projected_peak for an engine that holds no scene.
+    static uint64_t solo_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions);
+    // Scene-table bytes a layout allocates (bounded above).
+    static uint64_t table_allowance(uint64_t nscenes, uint64_t slots, bool reactions);
+    // Pool growth stays within `bytes` (0 = no limit): the x1.5 headroom is
+    // dropped when it would not fit beside the old pool.
+    void set_memory_limit(uint64_t bytes) { mem_limit_ = bytes; }
+    uint64_t peak_bytes() const { return peak_bytes_; }
+    // Free every device array when no scene is left (the pool never shrinks
+    // otherwise).
+    void release_if_empty();
 
 private:
     void layout();            // assign slot ranges, (re)allocate, move + upload
     uint32_t box_reduce_grid() const;
+    uint32_t grown_capacity(uint32_t need, uint64_t cells, bool reactions) const;
     uint32_t pool_substeps() const;  // substeps per tick (shared by every scene of the pool)
     // The launch sequence of one substep; `mark` (if set) is called after each
     // kernel launch with the kernel's name.
@@ -251,6 +273,8 @@ private:
     uint64_t substeps_done_ = 0;
     uint32_t ctl_step_base_ = 0;
     uint64_t bytes_ = 0;
+    uint64_t peak_bytes_ = 0;
+    uint64_t mem_limit_ = 0;
 };
 
 }  // namespace vxl

@@ -28,6 +28,16 @@ uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
 
 }  // namespace
 
+uint64_t Engine::pool_bytes(uint64_t slots, uint64_t ccap, bool diagnostics, bool reactions) {
+    // Per slot: cur + spare state 36 + 36, grid-ordered A + gid 20, density
+    // record 32, sort 16, neighbour record 40 (alloc_pool); diagnostics 36,
+    // reaction buffers 32. Per cell: counts, start, fluid count (alloc_cells).
+    const uint64_t per = 180ull + (diagnostics ? 36ull : 0ull) + (reactions ? 32ull : 0ull);
+    const uint64_t blocks = slots / kBlock;
+    return per * slots + 16ull * kReadPad + sizeof(SceneBox) * (blocks + 1) + 12ull * ccap + 4ull +
+           4ull * ((ccap + 2047) / 2048 + 1);
+}
+
 uint64_t Engine::footprint(uint32_t n, uint32_t cells_hint, uint64_t max_bins) {
     // cur + spare (36 B each), grid-ordered A + gid 20, density record 32,
     // sort 16, neighbour record 40 B per slot; counts, start and fluid count
@@ -64,6 +74,7 @@ T *Engine::dalloc(size_t n) {
     hipError_t e = hipMalloc(&p, n * sizeof(T));
     if (e != hipSuccess) throw DeviceError(e, std::string("hipMalloc: ") + hipGetErrorString(e));
     bytes_ += n * sizeof(T);
+    peak_bytes_ = std::max(peak_bytes_, bytes_);
     alloc_size_[p] = n * sizeof(T);
     return (T *)p;
 }
@@ -126,7 +137,7 @@ uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, cons
     uint64_t slots = round_up(n, kBlock);
     for (const auto &s : scenes_) slots += round_up(s.params.n, kBlock);
     if (n > kMaxPoolSlots || slots > kMaxPoolSlots - kBlock)
-        throw CapacityError("a pool holds at most 2^28 particle slots");
+        throw CapacityError("a pool holds at most 2^27 particle slots");
     if (fabsf(cfg.gamma_tait - 7.0f) >= 1e-6f)
         throw std::domain_error("only the gamma = 7 Tait branch (fluid_force.rs:17-21) is supported");
     SceneSlot sc;
@@ -372,7 +383,7 @@ void Engine::layout() {
     }();
     uint64_t total = 0;
     for (const auto &sc : scenes_) total += round_up(sc.params.n, kBlock);
-    if (total > kMaxPoolSlots - kBlock) throw CapacityError("a pool holds at most 2^28 particle slots");
+    if (total > kMaxPoolSlots - kBlock) throw CapacityError("a pool holds at most 2^27 particle slots");
     uint32_t used = 0;
     uint64_t cells = 0;
     std::vector<uint32_t> block_scene;
@@ -396,18 +407,20 @@ void Engine::layout() {
         // New pool; the old one stays alive until the resident state is copied.
         old_allocs.swap(pool_allocs_);
         const uint32_t ccap_keep = ccap_;
-        const uint32_t cap = std::min<uint32_t>(std::max<uint32_t>(need, slots_ + slots_ / 2), kMaxPoolSlots - kBlock);
-        want_reaction_buffers_ = want_reaction_buffers_ || reactions;
+        const uint64_t new_cells = std::min<uint64_t>(std::max<uint64_t>({cells, (uint64_t)ccap_keep, 4096ull}),
+                                                      (uint64_t)kMaxCells);
+        const bool react = want_reaction_buffers_ || reactions;
+        const uint32_t cap = grown_capacity(need, new_cells, react);
+        want_reaction_buffers_ = react;
         diag_alloc_ = diagnostics_;
-        alloc_pool(round_up(cap, kBlock));
+        alloc_pool(cap);
         ccap_ = 0;
         dfree(d_tile_sums_);
         d_tile_sums_ = nullptr;
-        alloc_cells((uint32_t)std::min<uint64_t>(std::max<uint64_t>({cells, (uint64_t)ccap_keep, 4096ull}),
-                                                 0xfffffff0ull));
+        alloc_cells((uint32_t)new_cells);
     } else if (cells > ccap_) {
         free_cells();
-        alloc_cells((uint32_t)std::min<uint64_t>(cells, 0xfffffff0ull));
+        alloc_cells((uint32_t)std::min<uint64_t>(cells, (uint64_t)kMaxCells));
     }
     // Move resident scenes: old position (src arrays) -> new position (dst).
     float4 *dstA = grow ? P_.cA : spare_cA_, *dstV = grow ? P_.cV : spare_cV_;
@@ -596,6 +609,78 @@ void Engine::layout() {
     substeps_done_ = 0;  // substep index of the new layout (tick phase, replay)
     ++layouts_;
     dirty_ = false;
+}
+
+uint32_t Engine::grown_capacity(uint32_t need, uint64_t cells, bool reactions) const {
+    // Grow geometrically (x1.5) so that a stream of admissions relayouts
+    // rarely; under a memory limit, only as far as the old pool (alive while
+    // the resident scenes move) and the new one fit together.
+    const uint32_t lim = kMaxPoolSlots - kBlock;
+    uint32_t cap = round_up(std::min<uint32_t>(std::max<uint32_t>(need, slots_ + slots_ / 2), lim), kBlock);
+    const uint32_t exact = round_up(std::min<uint32_t>(need, lim), kBlock);
+    if (mem_limit_ && cap > exact && bytes_ + pool_bytes(cap, cells, diagnostics_, reactions) > mem_limit_)
+        cap = exact;
+    return cap;
+}
+
+uint64_t Engine::projected_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions) const {
+    uint64_t need = round_up(n, kBlock), cells = cell_reserve(cells_hint, max_bins);
+    bool react = reactions || want_reaction_buffers_;
+    uint64_t maxn = n, nscenes = 1;
+    for (const auto &sc : scenes_) {
+        need += round_up(sc.params.n, kBlock);
+        cells += cell_reserve(sc.cells_hint, sc.params.max_bins);
+        react = react || sc.want_reactions;
+        maxn = std::max<uint64_t>(maxn, sc.params.n);
+        ++nscenes;
+    }
+    if (need > kMaxPoolSlots - kBlock) return ~0ull;
+    // Everything allocated now except the readback staging, which grows to the
+    // largest scene read back (24 bytes per particle); the scene tables the
+    // layout rebuilds (params, grid, boxes, block map, spans, rigid bodies and
+    // their lattices; freed before they are reallocated, bounded generously).
+    const uint64_t base = bytes_ - staging_bytes_;
+    const uint64_t stage = std::max<uint64_t>(staging_bytes_, 24ull * maxn);
+    const uint64_t tables = table_allowance(nscenes, need, react);
+    const uint64_t cells_now = ccap_ ? 12ull * ccap_ + 4ull * tiles_cap_ + 4ull : 0ull;
+    const bool grow = need > slots_ || (react && !want_reaction_buffers_) || diag_alloc_ != diagnostics_;
+    if (grow) {
+        const uint64_t new_cells = std::min<uint64_t>(std::max<uint64_t>({cells, (uint64_t)ccap_, 4096ull}),
+                                                      (uint64_t)kMaxCells);
+        const uint32_t cap = grown_capacity((uint32_t)need, new_cells, react);
+        // old pool and cells alive until the resident scenes have moved
+        return base + pool_bytes(cap, new_cells, diagnostics_, react) + stage + tables;
+    }
+    if (cells > ccap_) {
+        const uint64_t c = std::min<uint64_t>(cells, (uint64_t)kMaxCells);
+        return base - cells_now + 12ull * c + 4ull * ((c + 2047) / 2048 + 1) + 4ull + stage + tables;
+    }
+    return base + stage + tables;
+}
+
+uint64_t Engine::table_allowance(uint64_t nscenes, uint64_t slots, bool reactions) {
+    return 256ull * nscenes + slots / 32ull + 4096ull + (reactions ? 16ull * slots + 4096ull : 0ull);
+}
+
+uint64_t Engine::solo_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions) {
+    // projected_peak of an engine holding nothing but its control block
+    const uint64_t need = round_up(n, kBlock);
+    const uint64_t cells =
+        std::min<uint64_t>(std::max<uint64_t>(cell_reserve(cells_hint, max_bins), 4096ull), (uint64_t)kMaxCells);
+    return sizeof(Control) + pool_bytes(need, cells, false, reactions) + 24ull * n +
+           table_allowance(1, need, reactions);
+}
+
+void Engine::release_if_empty() {
+    if (!scenes_.empty()) return;
+    hip_check(hipStreamSynchronize(stream_), "sync before release");
+    invalidate_graph();
+    free_all();
+    want_reaction_buffers_ = false;
+    diag_alloc_ = false;
+    used_slots_ = 0;
+    P_.nblocks = 0;
+    dirty_ = true;
 }
 
 SceneSlot &Engine::slot_of(uint32_t id) {
@@ -1026,8 +1111,6 @@ void Engine::step(uint32_t ticks) {
 void Engine::check_error_state() {
     hip_check(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(Control), hipMemcpyDeviceToHost, stream_), "ctl read");
     hip_check(hipStreamSynchronize(stream_), "sync");
-    if (h_ctl_->overflow)
-        throw CapacityError("a boundary particle has more fluid neighbours than the reaction pass holds");
     if (h_ctl_->err & kErrGrid)
         throw std::domain_error("a scene's particle box is not finite (NaN or infinite positions)");
     if (h_ctl_->err & kErrCellCapacity) {
@@ -1035,7 +1118,7 @@ void Engine::check_error_state() {
         // grow the cell tables and replay the missing substeps.
         const uint64_t done = h_ctl_->step - ctl_step_base_;
         const uint64_t missing = substeps_done_ - done;
-        const uint32_t new_cap = (uint32_t)std::min<uint64_t>((uint64_t)ccap_ * 2ull, 0xfffffff0ull);
+        const uint32_t new_cap = (uint32_t)std::min<uint64_t>((uint64_t)ccap_ * 2ull, (uint64_t)kMaxCells);
         if (new_cap <= ccap_) throw CapacityError("cell table capacity exhausted");
         invalidate_graph();
         free_cells();
@@ -1067,41 +1150,76 @@ void Engine::set_diagnostics(bool on) {
 void Engine::upload_boundary(uint32_t id, uint32_t nb, const float *pos, const float *vel) {
     if (dirty_) layout();
     sync();
-    const SceneSlot &sc = scene(id);
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    const SceneSlot &sc = scenes_[k];
     const uint32_t n = sc.params.n, nf = sc.params.nf;
     if (nb != n - nf) throw std::length_error("boundary segment length differs from the scene's");
     if (nb == 0) return;
-    // Device holds cur arrays in grid order: scatter through the slot map.
-    const uint32_t pb = sc.params.pbase;
-    std::vector<uint32_t> gid(n);
-    hip_check(hipMemcpy(gid.data(), P_.cg + pb, 4ull * n, hipMemcpyDeviceToHost), "gid");
-    std::vector<float4> a(n), v(n);
-    hip_check(hipMemcpy(a.data(), P_.cA + pb, 16ull * n, hipMemcpyDeviceToHost), "A");
-    hip_check(hipMemcpy(v.data(), P_.cV + pb, 16ull * n, hipMemcpyDeviceToHost), "V");
-    for (uint32_t m = 0; m < n; ++m) {
-        if (gid[m] < nf) continue;
-        const size_t q = 3ull * (gid[m] - nf);
-        a[m].x = pos[q]; a[m].y = pos[q + 1]; a[m].z = pos[q + 2];
-        v[m].x = vel[q]; v[m].y = vel[q + 1]; v[m].z = vel[q + 2];
-    }
-    hip_check(hipMemcpy(P_.cA + pb, a.data(), 16ull * n, hipMemcpyHostToDevice), "A");
-    hip_check(hipMemcpy(P_.cV + pb, v.data(), 16ull * n, hipMemcpyHostToDevice), "V");
-    // Boundary motion changes the AABB of the next grid build: recompute it.
-    hipLaunchKernelGGL(k_box, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_);
-    hipLaunchKernelGGL(k_box_reduce, dim3(std::min<uint32_t>(P_.nscenes, 4096)), dim3(kBlock), 0, stream_, P_);
-    hip_check(hipGetLastError(), "k_box");
+    if (!pos || !vel) throw std::invalid_argument("null boundary segment");
+    // Only the segment crosses PCIe (24 B per boundary particle); the device
+    // scatters it into the grid-ordered state through the slot map and
+    // rebuilds the scene's AABB from the new positions.
+    float *d = (float *)staging(24ull * nb);
+    hip_check(hipMemcpyAsync(d, pos, 12ull * nb, hipMemcpyHostToDevice, stream_), "boundary pos");
+    hip_check(hipMemcpyAsync(d + 3ull * nb, vel, 12ull * nb, hipMemcpyHostToDevice, stream_), "boundary vel");
+    const uint32_t nblk = sc.params.alloc / kBlock;
+    hipLaunchKernelGGL(k_boundary_scatter, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nblk, 8192u))),
+                       dim3(kBlock), 0, stream_, P_, k, (const float *)d, (const float *)(d + 3ull * nb));
+    hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
+    hip_check(hipGetLastError(), "k_boundary_scatter");
+    // The caller may reuse its buffers once this returns.
     hip_check(hipStreamSynchronize(stream_), "sync");
+}
+
+void Engine::set_span_origins(uint32_t id, const Span *spans, uint32_t nspans) {
+    if (dirty_) layout();
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    SceneSlot &sc = scenes_[k];
+    if (nspans != sc.spans.size()) throw std::length_error("span count differs from the scene's");
+    for (uint32_t q = 0; q < nspans; ++q)
+        if (spans[q].start != sc.spans[q].start || spans[q].end != sc.spans[q].end)
+            throw std::length_error("span ranges differ from the scene's");
+    bool moved = false;
+    for (uint32_t q = 0; q < nspans; ++q) {
+        sc.spans[q].body = spans[q].body;
+        for (int a = 0; a < 3; ++a) {
+            moved = moved || std::memcmp(&sc.spans[q].origin[a], &spans[q].origin[a], sizeof(float)) != 0;
+            sc.spans[q].origin[a] = spans[q].origin[a];
+        }
+    }
+    if (!moved || !sc.want_reactions || !d_spans_) return;
+    // The reaction kernel reads the torque origins from the span table at run
+    // time, so the captured tick stays valid.
+    std::vector<SpanDev> dev(nspans);
+    hip_check(hipMemcpy(dev.data(), d_spans_ + span_first_[k], sizeof(SpanDev) * nspans, hipMemcpyDeviceToHost),
+              "spans");
+    for (uint32_t q = 0; q < nspans; ++q)
+        for (int a = 0; a < 3; ++a) dev[q].origin[a] = spans[q].origin[a];
+    hip_check(hipMemcpy(d_spans_ + span_first_[k], dev.data(), sizeof(SpanDev) * nspans, hipMemcpyHostToDevice),
+              "spans");
 }
 
 void Engine::read_state(uint32_t id, float *pos, float *vel) {
     sync();
-    const SceneSlot &sc = scene(id);
-    const uint32_t n = sc.params.n, pb = sc.params.pbase;
-    float *op = (float *)staging(24ull * slots_), *ov = op + 3ull * slots_;
-    hipLaunchKernelGGL(k_unsort, dim3(P_.nblocks), dim3(kBlock), 0, stream_, P_, op, ov);
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    const SceneSlot &sc = scenes_[k];
+    const uint32_t n = sc.params.n;
+    float *op = (float *)staging(24ull * n), *ov = op + 3ull * n;
+    const uint32_t nblk = sc.params.alloc / kBlock;
+    hipLaunchKernelGGL(k_unsort, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nblk, 8192u))), dim3(kBlock), 0,
+                       stream_, P_, k, op, ov);
     hip_check(hipGetLastError(), "k_unsort");
-    if (pos) hip_check(hipMemcpyAsync(pos, op + 3ull * pb, 12ull * n, hipMemcpyDeviceToHost, stream_), "pos");
-    if (vel) hip_check(hipMemcpyAsync(vel, ov + 3ull * pb, 12ull * n, hipMemcpyDeviceToHost, stream_), "vel");
+    if (pos) hip_check(hipMemcpyAsync(pos, op, 12ull * n, hipMemcpyDeviceToHost, stream_), "pos");
+    if (vel) hip_check(hipMemcpyAsync(vel, ov, 12ull * n, hipMemcpyDeviceToHost, stream_), "vel");
     hip_check(hipStreamSynchronize(stream_), "sync");
 }
 

@@ -80,6 +80,10 @@ struct Pool {
 };
 
 __global__ void k_box(Pool P);
+// Boundary segment of scene s from index-order device arrays (3 floats per
+// boundary particle), the scene's partial boxes, its AABB emptied for
+// k_box_reduce.
+__global__ void k_boundary_scatter(Pool P, uint32_t s, const float *bpos, const float *bvel);
 constexpr uint32_t kBoxReduceSpan = kBlock * 8;  // partial boxes per k_box_reduce block
 __global__ void k_box_reduce(Pool P);
 __global__ void k_grid_box(Pool P);
@@ -124,7 +128,7 @@ __global__ void k_reactions(Pool P);
 __global__ void k_rigid(Pool P);
 __global__ void k_body_refresh(Pool P);
 __global__ void k_rigid_collect(Pool P);
-__global__ void k_unsort(Pool P, float *opos, float *ovel);
+__global__ void k_unsort(Pool P, uint32_t s, float *opos, float *ovel);  // scene s, index order
 __global__ void k_unsort_fields(Pool P, float *odens, float *opress, uint32_t *oncount, float *oacc, float *oxs);
 
 }  // namespace vxl

@@ -112,7 +112,7 @@ __device__ __forceinline__ int nb_step(NbWalk &w, const Pool &P, const SceneGrid
         const uint32_t row = w.rowb + t.cell;
         const uint32_t first = row - ((w.summ >> (9u + r)) & 1u);
         w.j0 = *at(P.start, first);
-        w.mk = *at(P.nbm, t.mask + i);
+        w.mk = P.nbm[t.mask + i];  // 36 bytes per slot: a 64-bit offset
         if ((w.summ >> (18u + r)) & 1u) {
             w.jt = w.j0 + 32u;
             w.je = P.start[row - (uint32_t)c.cx + row_last_cell(g, c, r) + 1u];
@@ -136,6 +136,50 @@ __global__ __launch_bounds__(kBlock) void k_box(Pool P) {
         const bool live = m < sp.pbase + sp.n;
         float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (live) a = P.cA[m];
+        block_box(live, a.x, a.y, a.z, &P.part[b]);
+    }
+}
+
+// A scene's boundary segment replaced from index-order device arrays (three
+// floats per boundary particle, original index nf + q at 3 q): every slot of
+// scene s whose original index lies in the segment takes its new position
+// and velocity (the state stays in grid order), the scene's blocks rebuild
+// their partial boxes, and the scene's current AABB is emptied so that the
+// following k_box_reduce folds exactly the new positions into it (the
+// FluidStepper's per-tick boundary upload, pipeline/coupling.rs:20-50, and
+// the box refresh of pipeline.rs:557-566).
+__global__ __launch_bounds__(kBlock) void k_boundary_scatter(Pool P, uint32_t s, const float *__restrict__ bpos,
+                                                             const float *__restrict__ bvel) {
+    const SceneParams &sp = P.params[s];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SceneBox &bx = P.box[(P.ctl->step & 1u) * P.nscenes + s];
+        for (int a = 0; a < 3; ++a) {
+            bx.lo[a] = 0xffffffffu;
+            bx.hi[a] = 0u;
+        }
+    }
+    const uint32_t b0 = sp.pbase / kBlock, nblk = sp.alloc / kBlock;
+    for (uint32_t k = blockIdx.x; k < nblk; k += gridDim.x) {
+        const uint32_t b = b0 + k;
+        const uint32_t m = b * kBlock + threadIdx.x;
+        const bool live = m < sp.pbase + sp.n;
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (live) {
+            a = P.cA[m];
+            const uint32_t g = P.cg[m];
+            if (g >= sp.nf) {
+                const size_t q = 3ull * (g - sp.nf);
+                a.x = bpos[q];
+                a.y = bpos[q + 1];
+                a.z = bpos[q + 2];
+                P.cA[m] = a;
+                float4 v = P.cV[m];
+                v.x = bvel[q];
+                v.y = bvel[q + 1];
+                v.z = bvel[q + 2];
+                P.cV[m] = v;
+            }
+        }
         block_box(live, a.x, a.y, a.z, &P.part[b]);
     }
 }
@@ -1030,8 +1074,12 @@ template __global__ void k_force_t<2>(Pool P);
 // bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending
 // index, X = d * ((m_i * (coef + cv)) / denom) with d = p_i - p_j (the
 // reference accumulates it from the fluid side, i ascending:
-// fluid_force.rs:196-206). Fluid neighbours are collected with their index
-// and subtracted in index order, so the sum is the reference's bit for bit.
+// fluid_force.rs:196-206). The neighbours arrive in cell order, so they are
+// subtracted in batches: each pass over the candidates keeps, sorted by
+// original index, the kReactBatch smallest indices above the last one already
+// subtracted, subtracts them in order, and passes again while any were left
+// out. Any neighbour count gives the reference's sum bit for bit; one pass
+// covers the usual case (a wall particle sees about half a neighbourhood).
 __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
     if (P.ctl->err) return;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
@@ -1055,62 +1103,70 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
         const float rho_j = -vj.w, cj2 = P.fr[2u * j].w, mj = -aj.w;  // a boundary record: rho negated
         const Cursor cur = make_cursor(g, sp.h, aj.x, aj.y, aj.z);
         const float h = sp.h, h2 = sp.h2, ks = sp.ks, alpha_c = sp.alpha_c;
-        uint32_t kid[kReactMaxNeighbours];
-        float kx[kReactMaxNeighbours], ky[kReactMaxNeighbours], kz[kReactMaxNeighbours];
-        uint32_t cnt = 0;
-        bool over = false;
-        for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t i0, uint32_t i1) {
-            for (uint32_t ib = i0; ib < i1; ib += 4u) {
-                // four candidates' records in flight at once (the grid-ordered
-                // array is padded past its end)
-                float4 a4[4];
-#pragma unroll
-                for (uint32_t u = 0; u < 4u; ++u) a4[u] = *at(P.sA, ib + u);
-#pragma unroll
-                for (uint32_t u = 0; u < 4u; ++u) {
-                    const uint32_t i = ib + u;
-                    const float4 ai = a4[u];
-                    if (i >= i1 || !(ai.w > 0.0f)) continue;  // fluid neighbours only
-                    // d = p_i - p_j as seen from the fluid particle i.
-                    const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
-                    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (!(r2 <= h2)) continue;
-                    const float4 vi = P.fr[2u * i + 1u];
-                    const float rho_i = vi.w, ci2 = P.fr[2u * i].w, mi = ai.w;
-                    const float r = sqrtf(r2);
-                    const float t = h - r;
-                    const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
-                    const float denom = fmaxf(r, 1e-9f);
-                    const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
-                    const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
-                    float cv = 0.0f;
-                    if (vdn > 0.0f) {
-                        const float mu = (vdn * h) / (r2 + 0.01f * h2);
-                        cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
-                    }
-                    const float q = (mi * (coef + cv)) / denom;
-                    if (cnt == kReactMaxNeighbours) {
-                        over = true;
-                        continue;
-                    }
-                    const uint32_t gi = P.sg[i];
-                    uint32_t k = cnt++;  // insertion by original index
-                    while (k > 0 && kid[k - 1] > gi) {
-                        kid[k] = kid[k - 1];
-                        kx[k] = kx[k - 1]; ky[k] = ky[k - 1]; kz[k] = kz[k - 1];
-                        --k;
-                    }
-                    kid[k] = gi;
-                    kx[k] = ddx * q; ky[k] = ddy * q; kz[k] = ddz * q;
-                }
-            }
-        });
-        if (over) atomicOr(&P.ctl->overflow, 1u);
+        uint32_t kid[kReactBatch];
+        float kx[kReactBatch], ky[kReactBatch], kz[kReactBatch];
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        for (uint32_t k = 0; k < cnt; ++k) {
-            fx = fx - kx[k];
-            fy = fy - ky[k];
-            fz = fz - kz[k];
+        bool first = true;
+        uint32_t last = 0;  // original index of the last neighbour subtracted
+        for (;;) {
+            uint32_t cnt = 0;
+            bool more = false;
+            for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t i0, uint32_t i1) {
+                for (uint32_t ib = i0; ib < i1; ib += 4u) {
+                    // four candidates' records in flight at once (the grid-ordered
+                    // array is padded past its end)
+                    float4 a4[4];
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) a4[u] = *at(P.sA, ib + u);
+#pragma unroll
+                    for (uint32_t u = 0; u < 4u; ++u) {
+                        const uint32_t i = ib + u;
+                        const float4 ai = a4[u];
+                        if (i >= i1 || !(ai.w > 0.0f)) continue;  // fluid neighbours only
+                        // d = p_i - p_j as seen from the fluid particle i.
+                        const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        if (!(r2 <= h2)) continue;
+                        const uint32_t gi = P.sg[i];
+                        if (!first && gi <= last) continue;  // subtracted by an earlier pass
+                        if (cnt == kReactBatch) {
+                            more = true;
+                            if (gi > kid[kReactBatch - 1]) continue;  // a later pass takes it
+                        }
+                        const float4 vi = P.fr[2u * i + 1u];
+                        const float rho_i = vi.w, ci2 = P.fr[2u * i].w, mi = ai.w;
+                        const float r = sqrtf(r2);
+                        const float t = h - r;
+                        const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
+                        const float denom = fmaxf(r, 1e-9f);
+                        const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
+                        const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
+                        float cv = 0.0f;
+                        if (vdn > 0.0f) {
+                            const float mu = (vdn * h) / (r2 + 0.01f * h2);
+                            cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+                        }
+                        const float q = (mi * (coef + cv)) / denom;
+                        // insertion by original index (a full batch drops its largest)
+                        uint32_t k = cnt < kReactBatch ? cnt++ : kReactBatch - 1u;
+                        while (k > 0 && kid[k - 1] > gi) {
+                            kid[k] = kid[k - 1];
+                            kx[k] = kx[k - 1]; ky[k] = ky[k - 1]; kz[k] = kz[k - 1];
+                            --k;
+                        }
+                        kid[k] = gi;
+                        kx[k] = ddx * q; ky[k] = ddy * q; kz[k] = ddz * q;
+                    }
+                }
+            });
+            for (uint32_t k = 0; k < cnt; ++k) {
+                fx = fx - kx[k];
+                fy = fy - ky[k];
+                fz = fz - kz[k];
+            }
+            if (!more) break;
+            first = false;
+            last = kid[kReactBatch - 1];
         }
         const uint32_t d = sp.pbase + gj;
         P.bf[d] = make_float4(fx, fy, fz, 0.0f);
@@ -1180,14 +1236,15 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
     }
 }
 
-// Scatter cur state to index order (AoS, pool slot pbase + gid).
-__global__ __launch_bounds__(kBlock) void k_unsort(Pool P, float *opos, float *ovel) {
-    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
-        const uint32_t s = P.block_scene[b];
-        const SceneParams &sp = P.params[s];
-        const uint32_t m = b * kBlock + threadIdx.x;
+// Scatter scene s's cur state to index order (AoS, 3 floats per particle at
+// its original index).
+__global__ __launch_bounds__(kBlock) void k_unsort(Pool P, uint32_t s, float *opos, float *ovel) {
+    const SceneParams &sp = P.params[s];
+    const uint32_t b0 = sp.pbase / kBlock, nblk = sp.alloc / kBlock;
+    for (uint32_t k = blockIdx.x; k < nblk; k += gridDim.x) {
+        const uint32_t m = (b0 + k) * kBlock + threadIdx.x;
         if (m >= sp.pbase + sp.n) continue;
-        const size_t d = 3ull * (sp.pbase + P.cg[m]);
+        const size_t d = 3ull * P.cg[m];
         const float4 a = P.cA[m], v = P.cV[m];
         opos[d] = a.x; opos[d + 1] = a.y; opos[d + 2] = a.z;
         ovel[d] = v.x; ovel[d + 1] = v.y; ovel[d + 2] = v.z;

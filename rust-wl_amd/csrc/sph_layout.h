@@ -20,9 +20,21 @@ namespace vxl {
 
 constexpr uint32_t kBlock = 256;        // slots per block (4 wave64)
 constexpr uint32_t kGridDimMax = 1u << 14;  // core/grid.rs:8
-// Slots of one pool: every per-slot array is addressed with 32-bit byte
-// offsets (16-byte records), so a pool holds at most 2^28 slots.
-constexpr uint32_t kMaxPoolSlots = 1u << 28;
+// Slots of one pool. The neighbour loops address per-slot arrays through
+// 32-bit byte offsets from a scalar base (dev::at); the widest array read
+// that way is the 32-byte density record, so a pool holds at most 2^27 slots
+// (the record's last byte offset stays below 2^32). Arrays wider per slot
+// (the 36-byte neighbour masks) are indexed in 64 bits.
+constexpr uint32_t kMaxPoolSlots = 1u << 27;
+// Cells of the cell tables: `start` is read through 32-bit byte offsets too.
+constexpr uint32_t kMaxCells = (1u << 30) - 16u;
+// Bytes per slot of the widest array addressed through 32-bit offsets, and
+// per cell of the cell tables.
+constexpr uint32_t kAtRecordBytes = 32;
+constexpr uint32_t kAtCellBytes = 4;
+static_assert((uint64_t)kMaxPoolSlots * kAtRecordBytes <= (1ull << 32), "density record offsets must fit 32 bits");
+static_assert(((uint64_t)kMaxPoolSlots + 4u) * 16u <= (1ull << 32), "A record offsets must fit 32 bits");
+static_assert(((uint64_t)kMaxCells + 1u) * kAtCellBytes <= (1ull << 32), "cell start offsets must fit 32 bits");
 // Extra records after the last slot of the grid-ordered arrays: neighbour
 // candidates are loaded in groups of 4 without clamping to the range end.
 constexpr uint32_t kReadPad = 4;
@@ -87,7 +99,7 @@ struct SpanDev {
 constexpr uint32_t kReactAll = 1;      // every span (FluidStepper semantics)
 constexpr uint32_t kReactDynamic = 2;  // only spans of dynamic rigid bodies (coupled scenes)
 
-constexpr uint32_t kReactMaxNeighbours = 96;  // fluid neighbours per boundary particle
+constexpr uint32_t kReactBatch = 32;  // fluid neighbours a boundary particle's reaction pass holds at once
 
 // Provider walls of a scene (crates/vxl-phys-core/src/interop.rs:166-224).
 //
@@ -166,7 +178,7 @@ struct Control {
     uint32_t step;       // substep counter: AABB ping-pong parity
     uint32_t ctot;       // cells used this substep (sum of total + 1)
     uint32_t err;        // nonzero: the substep was abandoned (see kErr*)
-    uint32_t overflow;   // reactions: a boundary particle exceeded kReactMaxNeighbours
+    uint32_t pad;
 };
 
 constexpr uint32_t kErrCellCapacity = 1u;

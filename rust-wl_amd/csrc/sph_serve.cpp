@@ -95,6 +95,32 @@ void build_request(const SceneSample &req, const ServeConfig &cfg, SceneConfig &
     append_boundary_boxes(sc, *bodies, pos, vel, pmass, spans);
 }
 
+uint32_t scene_cells_hint(const std::vector<float> &pos, size_t n, float h, uint64_t max_bins) {
+    float lo[3], hi[3], gmin[3], bin;
+    uint32_t dims[3];
+    for (int a = 0; a < 3; ++a) lo[a] = hi[a] = pos[a];
+    for (size_t i = 1; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], pos[3 * i + a]);
+            hi[a] = fmaxf(hi[a], pos[3 * i + a]);
+        }
+    grid_box(lo, hi, h, max_bins, gmin, bin, dims);
+    return dims[0] * dims[1] * dims[2];
+}
+
+uint64_t request_solo_bytes(const SceneSample &req, const ServeConfig &cfg) {
+    SceneConfig sc;
+    std::vector<float> pos, vel, pmass, density;
+    std::vector<Span> spans;
+    std::vector<BoxBody> bodies;
+    RigidConfig rc;
+    uint32_t nf = 0;
+    build_request(req, cfg, sc, pos, vel, pmass, spans, nf, &bodies, &density, &rc);
+    const uint32_t n = (uint32_t)pmass.size();
+    return Engine::solo_peak(n, scene_cells_hint(pos, n, sc.smoothing_radius, sc.max_bins), sc.max_bins,
+                             !bodies.empty());
+}
+
 ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeConfig &cfg,
                  std::vector<ServeResult> &results) {
     ServeStats st;
@@ -132,16 +158,7 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
             try {
                 build_request(reqs[b.index], cfg, b.sc, b.pos, b.vel, b.pmass, b.spans, b.nf, &b.bodies, &b.density,
                               &b.rc);
-                float lo[3], hi[3], gmin[3], bin;
-                uint32_t dims[3];
-                for (int a = 0; a < 3; ++a) lo[a] = hi[a] = b.pos[a];
-                for (size_t i = 1; i < b.pmass.size(); ++i)
-                    for (int a = 0; a < 3; ++a) {
-                        lo[a] = fminf(lo[a], b.pos[3 * i + a]);
-                        hi[a] = fmaxf(hi[a], b.pos[3 * i + a]);
-                    }
-                grid_box(lo, hi, b.sc.smoothing_radius, b.sc.max_bins, gmin, bin, dims);
-                b.cells_hint = dims[0] * dims[1] * dims[2];
+                b.cells_hint = scene_cells_hint(b.pos, b.pmass.size(), b.sc.smoothing_radius, b.sc.max_bins);
             } catch (const std::exception &e) {
                 std::lock_guard<std::mutex> lk(mu);
                 build_error = e.what();
@@ -169,15 +186,17 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
 
     struct Resident {
         uint32_t index, scene, steps_left, nf, n;
-        uint64_t bytes;
     };
     std::vector<Resident> resident;
     size_t next_admit = 0;  // position in `order`
-    uint64_t bytes_in_use = 0;
     std::vector<double> step_ms, loop_ms, lat, wait;
     std::string error;
+    // Pool growth stays inside the budget; admission checks the engine's
+    // projected peak (old and new pool alive together while scenes move,
+    // readback staging included), so device bytes never exceed the budget.
+    eng.set_memory_limit(cfg.budget_bytes);
     try {
-        while (st.completed < st.requests) {
+        while (st.completed + st.rejected < st.requests) {
             if (now_s() > cfg.max_wall_s) break;
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -193,24 +212,38 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
                     std::lock_guard<std::mutex> lk(mu);
                     auto it = ready.find((uint32_t)next_admit);
                     if (it == ready.end()) break;
-                    const uint64_t need =
-                        Engine::footprint((uint32_t)it->second.pmass.size(), it->second.cells_hint, cfg.max_bins);
-                    if (!resident.empty() && bytes_in_use + need > cfg.budget_bytes) break;
                     b = std::move(it->second);
                     ready.erase(it);
                 }
                 const uint32_t n = (uint32_t)b.pmass.size();
-                const uint64_t need = Engine::footprint(n, b.cells_hint, cfg.max_bins);
-                if (need > cfg.budget_bytes)
-                    throw CapacityError("a request alone exceeds the serving HBM budget");
+                if (resident.empty()) eng.release_if_empty();
+                const uint64_t peak = eng.projected_peak(n, b.cells_hint, cfg.max_bins, !b.bodies.empty());
+                if (peak > cfg.budget_bytes) {
+                    if (!resident.empty()) {
+                        // wait for retirements; keep the built scene for the next try
+                        std::lock_guard<std::mutex> lk(mu);
+                        ready.emplace((uint32_t)next_admit, std::move(b));
+                        break;
+                    }
+                    // Too large even alone: rejected (steps = 0 in its result).
+                    ServeResult &res = results[b.index];
+                    res.index = b.index;
+                    res.n = n;
+                    res.nf = b.nf;
+                    res.steps = 0;
+                    res.arrival_s = reqs[b.index].arrival_s;
+                    res.admit_s = res.finish_s = sched_now();
+                    ++st.rejected;
+                    ++next_admit;
+                    continue;
+                }
                 const uint32_t sid =
                     b.bodies.empty()
                         ? eng.add_scene(b.sc, n, b.nf, b.pos.data(), b.vel.data(), b.pmass.data(), b.spans.data(),
                                         (uint32_t)b.spans.size(), false)
                         : eng.add_coupled_scene(b.sc, b.nf, b.pos.data(), b.vel.data(), b.bodies, b.density, b.rc);
                 const SceneSample &rq = reqs[b.index];
-                resident.push_back(Resident{b.index, sid, std::max<uint32_t>(rq.steps, 1), b.nf, n, need});
-                bytes_in_use += need;
+                resident.push_back(Resident{b.index, sid, std::max<uint32_t>(rq.steps, 1), b.nf, n});
                 ServeResult &res = results[b.index];
                 res.index = b.index;
                 res.n = n;
@@ -229,7 +262,6 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
             // One pool step for every resident scene.
             const std::vector<float> ms = eng.step_timed(1);
             step_ms.push_back(ms.empty() ? 0.0 : ms[0]);
-            st.peak_bytes = std::max<uint64_t>(st.peak_bytes, eng.device_bytes());
             ++st.pool_steps;
             // Retire finished scenes.
             for (size_t k = 0; k < resident.size();) {
@@ -243,7 +275,6 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
                     res.finish_s = sched_now();
                     lat.push_back(res.finish_s - res.arrival_s);
                     eng.remove_scene(r.scene);
-                    bytes_in_use -= r.bytes;
                     ++st.completed;
                     resident[k] = resident.back();
                     resident.pop_back();
@@ -280,6 +311,7 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
     st.wait_s_p50 = pct(wait, 0.50);
     st.wait_s_p99 = pct(wait, 0.99);
     st.layouts = eng.layouts();
+    st.peak_bytes = eng.peak_bytes();
     return st;
 }
 

@@ -46,7 +46,8 @@ struct ServeStats {
     double latency_s_p50 = 0.0, latency_s_p99 = 0.0;                    // arrival -> finish (schedule s)
     double wait_s_p50 = 0.0, wait_s_p99 = 0.0;                          // arrival -> admission
     uint32_t peak_resident = 0;
-    uint64_t peak_bytes = 0;
+    uint32_t rejected = 0;    // requests larger than the budget on their own (not served)
+    uint64_t peak_bytes = 0;  // highest device bytes the engine held
     uint64_t layouts = 0;
 };
 
@@ -64,6 +65,11 @@ void build_request(const SceneSample &req, const ServeConfig &cfg, SceneConfig &
 // rigid configuration (PhysConfig defaults, contacts with the request's tank).
 void request_bodies(const SceneSample &req, std::vector<BoxBody> &bodies, std::vector<float> &density,
                     RigidConfig &rc);
+
+// Grid cells of the initial box of a scene's positions (sizes its cell tables).
+uint32_t scene_cells_hint(const std::vector<float> &pos, size_t n, float h, uint64_t max_bins);
+// Peak device bytes of serving this request alone (Engine::solo_peak of its scene).
+uint64_t request_solo_bytes(const SceneSample &req, const ServeConfig &cfg);
 
 ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeConfig &cfg,
                  std::vector<ServeResult> &results);

@@ -486,6 +486,11 @@ VxlSphStatus vxl_sph_stepper_step(VxlSphStepper *s, float dt_tick, uint32_t n_fl
             if (n_fluid != s->nf) throw std::length_error("fluid prefix length differs from the stepper's");
             if (n != s->n) throw std::length_error("particle count differs from the stepper's");
             if (dt_tick != s->cfg.dt_tick) throw std::invalid_argument("tick length differs from the first step's");
+            // Spans come every tick (tick_average takes them, stepper.rs:97-109):
+            // same ranges, torque origins that follow the bodies.
+            std::vector<Span> sp = to_spans(spans, nspans);
+            s->eng.set_span_origins(s->scene, sp.data(), nspans);
+            s->spans = std::move(sp);
             s->eng.upload_boundary(s->scene, n - n_fluid, pos + 3ull * n_fluid, vel + 3ull * n_fluid);
         }
         s->eng.step(1);
@@ -934,6 +939,24 @@ VxlSphStatus vxl_sph_request_bodies(const VxlSphRequest *req, uint32_t cap, VxlS
     });
 }
 
+VxlSphStatus vxl_sph_request_bytes(const VxlSphRequest *req, float dt, uint64_t max_bins, uint64_t *out) {
+    if (!req || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    return guard([&] {
+        ServeConfig sc;
+        sc.dt = dt > 0 ? dt : 5e-4f;
+        sc.max_bins = max_bins ? max_bins : (1ull << 21);
+        *out = request_solo_bytes(from_request(*req), sc);
+    });
+}
+
+VxlSphStatus vxl_sph_pool_limits(uint64_t *max_slots, uint64_t *max_cells, uint32_t *record_bytes) {
+    if (!max_slots || !max_cells || !record_bytes) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    *max_slots = kMaxPoolSlots;
+    *max_cells = kMaxCells;
+    *record_bytes = kAtRecordBytes;
+    return VXL_SPH_OK;
+}
+
 VxlSphStatus vxl_sph_request_config(const VxlSphRequest *req, float dt, uint64_t max_bins, VxlSphFluidConfig *out) {
     if (!req || !out) return fail(VXL_SPH_NULL_POINTER, "null argument");
     return guard([&] {
@@ -974,7 +997,8 @@ VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSp
     if (!cfg || !stats || (n && !reqs)) return fail(VXL_SPH_NULL_POINTER, "null argument");
     VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphServeConfig));
     if (st != VXL_SPH_OK) return st;
-    return guard([&] {
+    uint32_t rejected = 0;
+    const VxlSphStatus rc = guard([&] {
         ServeConfig sc;
         sc.budget_bytes = cfg->budget_bytes;
         sc.time_scale = cfg->time_scale > 0 ? cfg->time_scale : 1.0;
@@ -1007,6 +1031,8 @@ VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSp
         stats->wait_s_p50 = s.wait_s_p50;
         stats->wait_s_p99 = s.wait_s_p99;
         stats->peak_resident = s.peak_resident;
+        stats->rejected = s.rejected;
+        rejected = s.rejected;
         stats->peak_bytes = s.peak_bytes;
         stats->layouts = s.layouts;
         if (results)
@@ -1021,6 +1047,9 @@ VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSp
                 results[k].state_hash = res[k].state_hash;
             }
     });
+    if (rc == VXL_SPH_OK && rejected)
+        return fail(VXL_SPH_CAPACITY, std::to_string(rejected) + " request(s) exceed the HBM budget on their own");
+    return rc;
 }
 
 }  // extern "C"

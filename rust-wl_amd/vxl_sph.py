@@ -131,7 +131,7 @@ class ServeStats(C.Structure):
                 ("step_ms_p50", C.c_double), ("step_ms_p99", C.c_double), ("step_ms_mean", C.c_double),
                 ("loop_ms_p50", C.c_double), ("loop_ms_p99", C.c_double), ("latency_s_p50", C.c_double),
                 ("latency_s_p99", C.c_double), ("wait_s_p50", C.c_double), ("wait_s_p99", C.c_double),
-                ("peak_resident", C.c_uint32), ("reserved", C.c_uint32), ("peak_bytes", C.c_uint64),
+                ("peak_resident", C.c_uint32), ("rejected", C.c_uint32), ("peak_bytes", C.c_uint64),
                 ("layouts", C.c_uint64)]
 
 
@@ -163,6 +163,7 @@ EXPORTS = (
     "vxl_sph_sample_batch", "vxl_sph_sample_requests", "vxl_sph_request_config", "vxl_sph_serve",
     "vxl_sph_scene_set_halfspace_walls", "vxl_sph_scene_set_wall_tables", "vxl_sph_scene_clear_walls",
     "vxl_sph_stepper_set_wall_tables", "vxl_sph_stepper_set_halfspace_walls",
+    "vxl_sph_request_bytes", "vxl_sph_pool_limits",
 )
 
 
@@ -273,6 +274,10 @@ def lib():
                                           C.c_uint32, C.c_uint32, C.POINTER(Request)]
     L.vxl_sph_request_config.restype = st
     L.vxl_sph_request_config.argtypes = [C.POINTER(Request), C.c_float, C.c_uint64, C.POINTER(FluidConfig)]
+    L.vxl_sph_request_bytes.restype = st
+    L.vxl_sph_request_bytes.argtypes = [C.POINTER(Request), C.c_float, C.c_uint64, C.POINTER(C.c_uint64)]
+    L.vxl_sph_pool_limits.restype = st
+    L.vxl_sph_pool_limits.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), up]
     L.vxl_sph_serve.restype = st
     L.vxl_sph_serve.argtypes = [C.POINTER(ServeConfig), C.c_uint32, C.POINTER(Request), C.POINTER(ServeStats),
                                 C.POINTER(ServeResult)]
@@ -436,6 +441,22 @@ def dam_break(cfg, tank, block, d, jitter=0.05, seed=1, walls=True):
     return SceneData(pos, vel, pm, nf.value, spans)
 
 
+def _vec3(a, n, what):
+    """(n, 3) contiguous f32 copy of a per-particle vector array; anything of
+    another length is a size mismatch (the C side reads exactly 3 n floats)."""
+    a = np.ascontiguousarray(a, np.float32)
+    if a.size != 3 * n or (a.ndim == 2 and a.shape[1] != 3) or a.ndim > 2:
+        raise VxlSphError(3, f"{what} holds {a.size} floats, expected {n} x 3")
+    return a.reshape(n, 3)
+
+
+def _scalars(a, n, what):
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    if a.size != n:
+        raise VxlSphError(3, f"{what} holds {a.size} values, expected {n}")
+    return a
+
+
 def _spans_arr(spans):
     arr = (Span * max(len(spans), 1))()
     for k, (body, origin, s, e) in enumerate(spans):
@@ -483,9 +504,9 @@ class Engine:
             pass
 
     def add(self, cfg, data, reactions=True):
-        pos = np.ascontiguousarray(data.pos, np.float32)
-        vel = np.ascontiguousarray(data.vel, np.float32)
-        pm = np.ascontiguousarray(data.pmass, np.float32)
+        pm = np.ascontiguousarray(data.pmass, np.float32).reshape(-1)
+        pos = _vec3(data.pos, len(pm), "pos")
+        vel = _vec3(data.vel, len(pm), "vel")
         spans = data.spans if reactions else []
         sid = C.c_uint32()
         _check(lib().vxl_sph_scene_add(self.h, C.byref(cfg), data.n, data.n_fluid, _f(pos), _f(vel), _f(pm),
@@ -545,8 +566,8 @@ class Engine:
         return ms[:ticks]
 
     def upload_boundary(self, sid, pos, vel):
-        pos = np.ascontiguousarray(pos, np.float32)
-        vel = np.ascontiguousarray(vel, np.float32)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        vel = _vec3(vel, len(pos), "vel")
         _check(lib().vxl_sph_scene_upload_boundary(self.h, sid, len(pos), _f(pos), _f(vel)))
 
     def read_state(self, sid):
@@ -666,9 +687,10 @@ class FluidStepper:
             pass
 
     def step(self, dt_tick, n_fluid, pos, vel, pmass, spans):
-        pos = np.ascontiguousarray(pos, np.float32)
-        vel = np.ascontiguousarray(vel, np.float32)
-        pmass = np.ascontiguousarray(pmass, np.float32)
+        # GpuFluidStepper::step asserts these lengths (pipeline/stepper.rs:84-96).
+        pmass = np.ascontiguousarray(pmass, np.float32).reshape(-1)
+        pos = _vec3(pos, len(pmass), "pos")
+        vel = _vec3(vel, len(pmass), "vel")
         if getattr(self, "_gather", None) is not None:
             # gather_walls before the first step: the positions this step starts from.
             h, providers = self._gather
@@ -955,6 +977,20 @@ def request_bodies(req):
     return (rc if cnt.value else None), bodies
 
 
+def request_bytes(req, dt=5e-4, max_bins=1 << 21):
+    """Peak device bytes of serving `req` alone (pool, staging, tables)."""
+    v = C.c_uint64()
+    _check(lib().vxl_sph_request_bytes(C.byref(req), C.c_float(dt), max_bins, C.byref(v)))
+    return v.value
+
+
+def pool_limits():
+    """(max slots, max cells, bytes per slot of the widest 32-bit-addressed record)."""
+    a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint32()
+    _check(lib().vxl_sph_pool_limits(C.byref(a), C.byref(b), C.byref(c)))
+    return a.value, b.value, c.value
+
+
 def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_resident=0, builders=4,
           max_bins=1 << 21, dt=5e-4, device=0):
     """Continuous-batching run over a request list; returns (stats dict, results list)."""
@@ -973,8 +1009,13 @@ def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_r
     arr = (Request * max(n, 1))(*reqs)
     st = ServeStats()
     res = (ServeResult * max(n, 1))()
-    _check(lib().vxl_sph_serve(C.byref(cfg), n, arr, C.byref(st), res))
-    stats = {f: getattr(st, f) for f, _ in ServeStats._fields_ if f != "reserved"}
+    code = lib().vxl_sph_serve(C.byref(cfg), n, arr, C.byref(st), res)
+    if code not in (0, 20):
+        _check(code)
+    # CAPACITY: some request alone exceeds the budget; it is not served (steps
+    # = 0 in its result) and the other requests' results are complete.
+    stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
+    stats["status"] = STATUS[code]
     results = [dict(index=r.index, n=r.n, n_fluid=r.n_fluid, steps=r.steps, arrival_s=r.arrival_s,
                     admit_s=r.admit_s, finish_s=r.finish_s, state_hash=r.state_hash) for r in res[:n]]
     return stats, results

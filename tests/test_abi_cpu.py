@@ -208,3 +208,31 @@ def test_request_bodies_are_tank_then_boxes():
         with_tank = V.dam_break(cfg, tuple(r.dam.tank), tuple(r.dam.block), r.dam.d, jitter=r.dam.jitter,
                                 seed=r.dam.seed, walls=True)
         np.testing.assert_array_equal(tank_pos, with_tank.pos[with_tank.n_fluid:])
+
+
+def test_pool_limits_keep_32bit_offsets():
+    """The neighbour loops read per-slot records through 32-bit byte offsets
+    from a scalar base: at the largest pool the library admits, the last byte
+    of the widest such record (the 32-byte density record), of the padded
+    16-byte A records and of the cell starts stays below 2^32. The 36-byte
+    neighbour masks would not fit, which is why they are indexed in 64 bits."""
+    max_slots, max_cells, rec = V.pool_limits()
+    assert rec == 32
+    assert max_slots * rec <= 1 << 32
+    assert (max_slots + 4) * 16 <= 1 << 32
+    assert (max_cells + 1) * 4 <= 1 << 32
+    assert max_slots * 36 > 1 << 32
+    assert max_slots >= 32 * 1024 * 1024  # room for the 10M scene several times over
+
+
+def test_request_bytes_cover_the_scene():
+    """A request's projected device bytes hold at least its pool state (180
+    bytes per particle slot) and its readback staging."""
+    reqs = V.sample_requests(9, 6, nmin=4096, nmax=65536, smin=10, smax=20)
+    sizes = []
+    for r in reqs:
+        cfg, data = V.request_scene(r)
+        b = V.request_bytes(r)
+        assert b >= (180 + 24) * data.n
+        sizes.append(b)
+    assert len(set(sizes)) == len(sizes)

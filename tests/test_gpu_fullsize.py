@@ -79,3 +79,14 @@ def test_10m_dam_break_full_size():
     assert data.n_fluid == 9_938_375
     check_against_oracle(cfg, data, tank, d, steps=1, diagnostics=False)
     long_runs(cfg, data, tank, steps=500)
+
+
+def test_10m_dam_break_benched_budget():
+    """The benchmark's configuration of the same scene: grid budget 2^22, which
+    keeps bin = h (202 x 126 x 127 cells, 3.2M cells). One step against the
+    oracle, bit for bit: cell keys, start, items, density, pressure,
+    positions, velocities."""
+    d, tank = 0.01, (4.0, 2.5, 2.5)
+    cfg = V.dam_break_config(d, dt=2.5e-4, max_bins=1 << 22)
+    data = V.dam_break(cfg, tank, (215, 215, 215), d, seed=1)
+    check_against_oracle(cfg, data, tank, d, steps=1, diagnostics=False)

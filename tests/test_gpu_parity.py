@@ -461,22 +461,38 @@ def test_batch_equal_alone_and_oracle():
 def test_serve_results_equal_alone():
     """Served form: requests arriving over time, admitted as they arrive and
     co-stepped with whatever else is resident; every result is bit-identical
-    to the same request stepped alone."""
+    to the same request stepped alone and to the oracle (requests with boxes
+    through the reference's coupled tick order: rigid substeps, Akinci
+    particles from the new poses, fluid tick)."""
+    from test_gpu_coupled import OracleCoupled
+
     reqs = V.sample_requests(5, 8, rate=200.0, nmin=1500, nmax=12000, smin=3, smax=12, max_boxes=3)
     assert any(r.nboxes for r in reqs) and any(not r.nboxes for r in reqs)
     stats, results = V.serve(reqs, budget_bytes=1 << 30, time_scale=1.0, builders=2)
-    assert stats["completed"] == len(reqs)
+    assert stats["completed"] == len(reqs) and stats["status"] == "OK"
     assert stats["peak_resident"] >= 2  # requests overlapped on the device
     for r, res in zip(reqs, results):
         cfg, data = V.request_scene(r)
         rc, bodies = V.request_bodies(r)
+        tank, d = tuple(r.dam.tank), r.dam.d
         eng = V.Engine()
         if bodies:
             sid = eng.add_coupled(cfg, rc, data.pos[:data.n_fluid], bodies)
+            statics = [b for b in bodies if not b["density"] > 0]
+            boxes = [b for b in bodies if b["density"] > 0]
+            orc = OracleCoupled(cfg, data, statics, boxes, tank)
+            for _ in range(r.steps):
+                orc.tick()
+            ohash = orc.sys.state_hash()
         else:
             sid = eng.add(cfg, data, reactions=False)
+            orc = oracle_dam_break(cfg, data, tank, d, threads=8)
+            for _ in range(r.steps):
+                orc.step(cfg.dt_tick)
+            ohash = orc.state_hash()
         eng.step(r.steps)
         assert eng.state_hash(sid) == res["state_hash"]
+        assert res["state_hash"] == ohash
         eng.close()
 
 

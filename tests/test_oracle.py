@@ -251,6 +251,74 @@ def test_single_pair_closed_form():
     assert abs(s.xsph()[0, 0] - xs) <= 1e-5 * abs(xs)
 
 
+def pair_viscosity_error(fault=0):
+    """Relative error of the Monaghan viscosity term of an approaching pair
+    against its closed form (fluid_force.rs:190-197): the acceleration with
+    alpha > 0 minus the one with alpha = 0 is d * (cv / r) with
+    cv = m (alpha c mu / rho_bar) ks (h - r)^2, mu = vdn h / (r^2 + 0.01 h^2),
+    vdn = -(v_ij . d) > 0 and rho_bar the mean density of the pair."""
+    r = np.float32(0.06)
+    pos = np.array([[0, 0.5, 0], [r, 0.5, 0]], np.float32)
+    vel = np.array([[0.1, 0, 0], [-0.1, 0, 0]], np.float32)  # approaching
+    m = np.float32(1.0)
+    acc = {}
+    for alpha in (0.0, 0.5):
+        cfg = O.default_config(gravity=(0.0, 0.0, 0.0), substeps=1, artificial_viscosity=alpha)
+        s = O.System.particles(cfg, pos, vel, m, 0.05)
+        s.set_fault(fault)
+        s.step(1e-4)
+        acc[alpha] = s.acc()[0].astype(np.float64)
+        rho = np.float64(s.dens()[0])
+        c = s.constants()
+    h = np.float64(c["h"])
+    ks = 45.0 / (np.pi * h ** 6)
+    alpha_c = 0.5 * np.float64(cfg.sound_speed)
+    rr = np.float64(r)
+    vdn = 0.2 * rr  # -(v_ij . d) with v_ij = (0.2, 0, 0), d = (-r, 0, 0)
+    mu = vdn * h / (rr * rr + 0.01 * h * h)
+    cv = np.float64(m) * (alpha_c * mu / rho) * ks * (h - rr) ** 2
+    diff = acc[0.5] - acc[0.0]
+    assert abs(diff[1]) == 0.0 and abs(diff[2]) == 0.0
+    return abs(diff[0] - (-cv)) / cv
+
+
+def test_approaching_pair_viscosity_closed_form():
+    """Monaghan viscosity (alpha > 0) of an approaching pair against the
+    closed form; a planted mistake (the density sum in place of its mean)
+    fails it."""
+    assert pair_viscosity_error() < 1e-4
+    assert pair_viscosity_error(fault=6) > 0.4
+
+
+def clamp_case(v0, fault=0):
+    """One isolated fluid particle (no neighbours, no gravity) moving at v0:
+    fluid_step.rs:110-123 keeps its velocity below vmax = frac h / dt."""
+    cfg = O.default_config(gravity=(0.0, 0.0, 0.0), substeps=1)
+    s = O.System.particles(cfg, np.array([[0.0, 0.5, 0.0]], np.float32), np.array([v0], np.float32),
+                           np.float32(1.0), 0.05)
+    s.set_fault(fault)
+    dt = 1e-3
+    s.step(dt)
+    c = s.constants()
+    vmax = float(cfg.max_speed_frac) * float(c["h"]) / dt
+    return s.vel()[0].astype(np.float64), vmax
+
+
+def test_speed_clamp_closed_form():
+    """Faster than vmax: scaled to |v| = vmax along the same direction;
+    slower: untouched. A planted mistake (clamp skipped) fails it."""
+    v0 = np.array([30.0, 40.0, 0.0])
+    v, vmax = clamp_case(v0)
+    assert vmax == pytest.approx(40.0, rel=1e-6)
+    assert abs(np.linalg.norm(v) / vmax - 1.0) < 1e-6
+    assert np.allclose(v / np.linalg.norm(v), v0 / np.linalg.norm(v0), atol=1e-6)
+    slow = np.array([3.0, -4.0, 1.0])
+    v_slow, _ = clamp_case(slow)
+    np.testing.assert_array_equal(v_slow.astype(np.float32), slow.astype(np.float32))
+    v_bad, _ = clamp_case(v0, fault=7)
+    assert abs(np.linalg.norm(v_bad) / vmax - 1.0) > 0.2
+
+
 def test_time_weighted_reactions_over_substeps():
     """With substeps > 1 the reaction is the mean over substeps
     (fluid_step.rs:75-89; src/tests.rs:668-712)."""
