@@ -226,6 +226,18 @@ VxlSphStatus vxl_sph_scene_set_wall_tables(VxlSphEngine *e, uint32_t scene, uint
 /* Remove a scene's walls (either kind). */
 VxlSphStatus vxl_sph_scene_clear_walls(VxlSphEngine *e, uint32_t scene);
 
+/* ---- one scene in caller-owned device arrays ---- */
+/* FluidSystem::step (fluid_step.rs:75-127) for `ticks` ticks of a scene held
+ * in device memory of this process: pos/vel (3 floats per particle) and pmass
+ * (1 per particle) of all n particles, fluid prefix first, index order. pos
+ * and vel are read at the start and overwritten with the new state at the
+ * end, only when every substep completed. No reactions, no walls. `stream`
+ * is a hipStream_t (NULL: the null stream); the call returns after the
+ * stream has finished its work. A device workspace for the scene's pool is
+ * allocated and freed by the call. No counterpart in the reference (its
+ * FluidSystem owns host vectors). */
+VxlSphStatus vxl_sph_step_device(const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid, float *pos, float *vel,
+                                 const float *pmass, uint32_t ticks, void *stream);
 /* ---- FluidStepper drop-in (single scene; GpuFluidStepper semantics) ---- */
 VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,
                                     VxlSphStepper **out);

@@ -183,6 +183,7 @@ struct Control {
 
 constexpr uint32_t kErrCellCapacity = 1u;
 constexpr uint32_t kErrGrid = 2u;  // non-finite particle box
+constexpr uint32_t kErrMass = 4u;  // a particle mass not positive and finite (device-array step)
 
 // Order-preserving map f32 -> u32 (total order; -0 < +0).
 __host__ __device__ inline uint32_t f2o(float f) {

@@ -5,10 +5,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
 
+#include "sph_device_step.h"
 #include "sph_engine.h"
 #include "sph_serve.h"
 #include "sph_vxld.h"
@@ -175,6 +177,26 @@ uint64_t fnv_state(const float *pos, const float *vel, uint32_t nf) {
         mix(vel[3 * i]); mix(vel[3 * i + 1]); mix(vel[3 * i + 2]);
     }
     return h;
+}
+
+}  // namespace
+
+namespace {
+
+// Arguments of the device-array step: a valid config, a supported scene, and
+// arrays that are device memory long enough for n particles.
+SceneConfig device_step_args(const VxlSphFluidConfig &cfg, uint32_t n, uint32_t n_fluid, const float *pos,
+                             const float *vel, const float *pmass, uint32_t max_n) {
+    const SceneConfig sc = to_scene_config(cfg);
+    if (n == 0) throw std::invalid_argument("empty scene");
+    if (n_fluid > n) throw std::invalid_argument("n_fluid exceeds particle count");
+    if (n > max_n) throw CapacityError("more particles than the step holds");
+    if (fabsf(sc.gamma_tait - 7.0f) >= 1e-6f)
+        throw std::domain_error("only the gamma = 7 Tait branch (fluid_force.rs:17-21) is supported");
+    check_device_range(pos, 12ull * n, "pos");
+    check_device_range(vel, 12ull * n, "vel");
+    check_device_range(pmass, 4ull * n, "pmass");
+    return sc;
 }
 
 }  // namespace
@@ -444,6 +466,33 @@ VxlSphStatus vxl_sph_scene_state_hash_all(VxlSphEngine *e, uint32_t scene, uint6
         bytes(p);
         bytes(v);
         *out = h;
+    });
+}
+
+// ---- one scene in caller-owned device arrays ----
+
+
+VxlSphStatus vxl_sph_step_device(const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid, float *pos, float *vel,
+                                 const float *pmass, uint32_t ticks, void *stream) {
+    if (!cfg || !pos || !vel || !pmass) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        const SceneConfig sc = device_step_args(*cfg, n, n_fluid, pos, vel, pmass, kMaxPoolSlots - kBlock);
+        if (ticks == 0) return;
+        // The grid rule keeps a scene within max_bins cells: no capacity retry.
+        const uint32_t ccap = (uint32_t)std::min<uint64_t>(sc.max_bins, (uint64_t)kMaxCells - 1u) + 1u;
+        const size_t bytes = device_step_workspace(n, ccap, nullptr, nullptr, nullptr, nullptr, nullptr);
+        char *ws = nullptr;
+        const hipError_t e = hipMalloc((void **)&ws, bytes);
+        if (e != hipSuccess) throw DeviceError(e, std::string("hipMalloc: ") + hipGetErrorString(e));
+        try {
+            device_step(sc, n, n_fluid, pos, vel, pmass, ticks, (hipStream_t)stream, ws, bytes, ccap);
+        } catch (...) {
+            (void)hipFree(ws);
+            throw;
+        }
+        (void)hipFree(ws);
     });
 }
 
