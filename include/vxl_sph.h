@@ -238,6 +238,19 @@ VxlSphStatus vxl_sph_scene_clear_walls(VxlSphEngine *e, uint32_t scene);
  * FluidSystem owns host vectors). */
 VxlSphStatus vxl_sph_step_device(const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid, float *pos, float *vel,
                                  const float *pmass, uint32_t ticks, void *stream);
+/* vxl_sph_step_device for callers that cannot allocate (code between HIP
+ * calls that must not allocate, tight serving loops, harnesses that forbid
+ * hipMalloc): the same kernels, results equal bit for bit, the workspace taken
+ * from a device array of the library (about 26 MiB, one per device, held until
+ * the process ends) instead of an allocation. Nothing is allocated. Limits:
+ * n <= 131072 particles and a grid of at most 524287 cells; beyond them the
+ * call returns VXL_SPH_CAPACITY and leaves the arrays untouched. Calls on one
+ * device share the array and are serialised; each call sets every word it
+ * reads before reading it. No counterpart in the reference: it exists for the
+ * callers that cannot allocate. */
+VxlSphStatus vxl_sph_step_device_noalloc(const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid, float *pos,
+                                         float *vel, const float *pmass, uint32_t ticks, void *stream);
+
 /* ---- FluidStepper drop-in (single scene; GpuFluidStepper semantics) ---- */
 VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,
                                     VxlSphStepper **out);

@@ -30,6 +30,20 @@ size_t device_step_workspace(uint32_t n, uint32_t ccap, char *base, Pool *out, u
 void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, float *vel, const float *pmass,
                  uint32_t ticks, hipStream_t stream, char *ws, size_t ws_bytes, uint32_t ccap);
 
+// Largest scene of device_step_noalloc.
+constexpr uint32_t kNoAllocParticles = 1u << 17;  // 131,072 particles (fluid + boundary)
+constexpr uint32_t kNoAllocCells = 1u << 19;      // grid cells + the pad cell
+
+// device_step with the workspace taken from a file-scope device array of the
+// library (one per device, kept for the life of the process) instead of an
+// allocation: nothing is allocated. Throws CapacityError beyond
+// kNoAllocParticles particles (before any launch) or kNoAllocCells - 1 grid
+// cells. Calls on one device are serialised (they share the array); the
+// array keeps the last call's bytes, and every word a call reads first is
+// set by that call.
+void device_step_noalloc(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, float *vel, const float *pmass,
+                         uint32_t ticks, hipStream_t stream);
+
 // Throws unless [p, p + bytes) lies inside one device allocation of this
 // process (a host pointer or a short array would make the kernels fault).
 void check_device_range(const void *p, size_t bytes, const char *what);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -147,6 +148,59 @@ void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, fl
     hipLaunchKernelGGL(k_unsort, dim3(std::min<uint32_t>(nb, 8192u)), dim3(kBlock), 0, stream, P, 0u, pos, vel);
     hip_ok(hipGetLastError(), "k_unsort");
     hip_ok(hipStreamSynchronize(stream), "device step");
+}
+
+// ---- the allocation-free variant: the workspace is a file-scope device array ----
+
+namespace {
+
+// One scene of up to kNoAllocParticles particles and kNoAllocCells cells
+// (device_step_workspace(kNoAllocParticles, kNoAllocCells) bytes, checked
+// when the address is taken). Each device holds its own copy for the life of
+// the process; calls on one device share it.
+constexpr size_t kScratchBytes = 26ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch[kMaxDevices] = {};
+std::mutex g_device_mu[kMaxDevices];
+
+// Device address of this device's scratch: looked up once per device, with
+// that device current. In host code `scratch` names a host-side stand-in, so
+// only this address may reach a kernel or a copy.
+char *scratch_of(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) throw std::invalid_argument("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        if (device_step_workspace(kNoAllocParticles, kNoAllocCells, nullptr, nullptr, nullptr, nullptr, nullptr) >
+            kScratchBytes)
+            throw std::logic_error("allocation-free scratch smaller than its stated maximum scene");
+        void *p = nullptr;
+        hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw DeviceError(hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch): null address");
+        g_scratch[dev] = (char *)p;
+    }
+    return g_scratch[dev];
+}
+
+}  // namespace
+
+void device_step_noalloc(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, float *vel, const float *pmass,
+                         uint32_t ticks, hipStream_t stream) {
+    if (n > kNoAllocParticles)
+        throw CapacityError("the allocation-free step holds at most " + std::to_string(kNoAllocParticles) +
+                            " particles (" + std::to_string(n) + " given)");
+    int dev = 0;
+    hip_ok(hipGetDevice(&dev), "hipGetDevice");
+    char *ws = scratch_of(dev);
+    if (ticks == 0) return;
+    // Cells beyond the scratch's table end the call with CapacityError (the
+    // substep is abandoned on the device, the arrays are left untouched).
+    const uint64_t cells = std::min<uint64_t>(cfg.max_bins ? cfg.max_bins : (1ull << 21), (uint64_t)kMaxCells - 1u);
+    const uint32_t ccap = (uint32_t)std::min<uint64_t>(cells + 1u, kNoAllocCells);
+    std::lock_guard<std::mutex> lk(g_device_mu[dev]);
+    device_step(cfg, n, nf, pos, vel, pmass, ticks, stream, ws, kScratchBytes, ccap);
 }
 
 void check_device_range(const void *p, size_t bytes, const char *what) {

@@ -496,6 +496,17 @@ VxlSphStatus vxl_sph_step_device(const VxlSphFluidConfig *cfg, uint32_t n, uint3
     });
 }
 
+VxlSphStatus vxl_sph_step_device_noalloc(const VxlSphFluidConfig *cfg, uint32_t n, uint32_t n_fluid, float *pos,
+                                         float *vel, const float *pmass, uint32_t ticks, void *stream) {
+    if (!cfg || !pos || !vel || !pmass) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    VxlSphStatus st = check_header(cfg->hdr, sizeof(VxlSphFluidConfig));
+    if (st != VXL_SPH_OK) return st;
+    return guard([&] {
+        const SceneConfig sc = device_step_args(*cfg, n, n_fluid, pos, vel, pmass, ~0u);
+        device_step_noalloc(sc, n, n_fluid, pos, vel, pmass, ticks, (hipStream_t)stream);
+    });
+}
+
 // ---- FluidStepper drop-in ----
 
 VxlSphStatus vxl_sph_stepper_create(uint32_t abi_version, const VxlSphFluidConfig *cfg, int32_t device,

@@ -163,7 +163,7 @@ EXPORTS = (
     "vxl_sph_sample_batch", "vxl_sph_sample_requests", "vxl_sph_request_config", "vxl_sph_serve",
     "vxl_sph_scene_set_halfspace_walls", "vxl_sph_scene_set_wall_tables", "vxl_sph_scene_clear_walls",
     "vxl_sph_stepper_set_wall_tables", "vxl_sph_stepper_set_halfspace_walls",
-    "vxl_sph_request_bytes", "vxl_sph_pool_limits", "vxl_sph_step_device",
+    "vxl_sph_request_bytes", "vxl_sph_pool_limits", "vxl_sph_step_device", "vxl_sph_step_device_noalloc",
 )
 
 
@@ -277,6 +277,8 @@ def lib():
     L.vxl_sph_step_device.restype = st
     L.vxl_sph_step_device.argtypes = [C.POINTER(FluidConfig), C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                       C.c_void_p, C.c_uint32, C.c_void_p]
+    L.vxl_sph_step_device_noalloc.restype = st
+    L.vxl_sph_step_device_noalloc.argtypes = L.vxl_sph_step_device.argtypes
     L.vxl_sph_request_bytes.restype = st
     L.vxl_sph_request_bytes.argtypes = [C.POINTER(Request), C.c_float, C.c_uint64, C.POINTER(C.c_uint64)]
     L.vxl_sph_pool_limits.restype = st
@@ -984,6 +986,12 @@ def step_device(cfg, n, n_fluid, pos_ptr, vel_ptr, pmass_ptr, ticks, stream=None
     """Advance a scene held in device arrays (integer device addresses: pos and
     vel n x 3 f32, pmass n f32, index order) by `ticks` ticks in place."""
     _check(lib().vxl_sph_step_device(C.byref(cfg), n, n_fluid, pos_ptr, vel_ptr, pmass_ptr, ticks, stream))
+
+
+def step_device_noalloc(cfg, n, n_fluid, pos_ptr, vel_ptr, pmass_ptr, ticks, stream=None):
+    """step_device without any allocation (the library's per-device scratch;
+    at most 131072 particles)."""
+    _check(lib().vxl_sph_step_device_noalloc(C.byref(cfg), n, n_fluid, pos_ptr, vel_ptr, pmass_ptr, ticks, stream))
 
 
 def request_bytes(req, dt=5e-4, max_bins=1 << 21):
