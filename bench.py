@@ -308,15 +308,25 @@ def main():
     else:
         tuned = eng.autotune(2)
         print(f"# autotuned: {tuned}", file=sys.stderr, flush=True)
+    sweep = None
     if args.sweep:
+        # Each knob over its values around the tuned point (the others held):
+        # per-kernel device ms per substep, mean of 3 eager substeps.
         eng.step(2)
         saved = eng.get_tuning()
-        for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD):
-            for pb in (0, 2048):
-                eng.set_tuning(force_kernel=fk, persist_blocks=pb)
-                pr = eng.profile(3)
-                print(f"# sweep force_kernel={fk} persist_blocks={pb}: force {pr['force'] / 3:.3f} ms, "
-                      f"density {pr['density'] / 3:.3f} ms", file=sys.stderr, flush=True)
+        sweep = []
+        for field, values in (("force_kernel", (1, 2, 3)), ("block_threads", (64, 128, 256)),
+                              ("stage_records", (128, 256, 512)), ("scan_tile_cells", (1024, 2048, 4096)),
+                              ("persist_blocks", (0, 2048))):
+            for v in values:
+                t = dict(saved)
+                t[field] = v
+                eng.set_tuning(**t)
+                eng.profile(1)
+                pr = {k: round(x / 3, 4) for k, x in eng.profile(3).items()}
+                sweep.append({"knob": field, "value": v, "kernel_ms": pr, "total_ms": round(sum(pr.values()), 4)})
+                print(f"# sweep {field}={v}: total {sum(pr.values()):.3f} ms (force {pr.get('force', 0):.3f}, "
+                      f"density {pr.get('density', 0):.3f})", file=sys.stderr, flush=True)
         eng.set_tuning(**saved)
     eng.step(args.warmup)
     eng.sync()
@@ -396,6 +406,7 @@ def main():
         "kernel_ms_per_step": {k: round(v, 4) for k, v in kern.items()},
         "kernel_algorithmic_GBs": stage_gbs,
         "tuning": tuning,
+        "tuning_sweep": sweep,
         "batched": batched,
         "served": served,
         "walls": walls,

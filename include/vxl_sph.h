@@ -115,9 +115,12 @@ VxlSphStatus vxl_sph_engine_set_diagnostics(VxlSphEngine *e, int32_t on);
  * crates/vxl-phys-gpu/src/density.wgsl:52). 0 keeps a field's default. */
 typedef struct VxlSphTuning {
     VxlSphHeader hdr;
-    uint32_t persist_blocks; /* grid size cap of the per-slot kernels (0 = one block per 256 slots) */
-    uint32_t force_kernel;   /* VXL_SPH_FORCE_* */
-    uint32_t reserved[2];
+    uint32_t persist_blocks;  /* grid size cap of the per-slot kernels (0 = one block per 256 slots) */
+    uint32_t force_kernel;    /* VXL_SPH_FORCE_* */
+    uint32_t block_threads;   /* workgroup size of the density and force kernels: 64, 128, 256 (0 = 256) */
+    uint32_t stage_records;   /* LDS staging depth of the density pass, records per wave and row: 128, 256, 512 (0 = 256) */
+    uint32_t scan_tile_cells; /* cells per tile of the cell-count scan: 1024, 2048, 4096 (0 = 2048) */
+    uint32_t reserved;
 } VxlSphTuning;
 /* Force pass: every particle tests its neighbour candidates again (SCAN), or
  * walks the accepted neighbours the density pass recorded (WALK). */
@@ -125,11 +128,13 @@ typedef struct VxlSphTuning {
 #define VXL_SPH_FORCE_SCAN 1u
 #define VXL_SPH_FORCE_WALK 2u
 #define VXL_SPH_FORCE_WALK_AHEAD 3u  /* the walk with each pair's loads issued one pair ahead */
+#define VXL_SPH_FORCE_PAIRS 4u       /* row by row, the wave's recorded pairs spread evenly over its lanes */
 VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t);
 VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out);
-/* Autotuner: times every variant for `substeps` real substeps each (results
- * are identical for all variants) and keeps the fastest; cached per
- * (arch, log2 particle count) for the process. `out` may be NULL. */
+/* Autotuner: coordinate descent over every field above, each candidate timed
+ * for `substeps` real substeps on the resident pool (results are identical
+ * for all of them), the fastest kept; cached per (arch, log2 particle count)
+ * for the process. `out` may be NULL. */
 VxlSphStatus vxl_sph_engine_autotune(VxlSphEngine *e, uint32_t substeps, VxlSphTuning *out);
 /* Replay each substep as one hipGraph (default on). */
 VxlSphStatus vxl_sph_engine_set_graphs(VxlSphEngine *e, int32_t on);

@@ -86,7 +86,7 @@ size_t device_step_workspace(uint32_t n, uint32_t ccap, char *base, Pool *out, u
     P.params = pr;
     P.grid = (SceneGrid *)take(sizeof(SceneGrid));
     P.box = (SceneBox *)take(2 * sizeof(SceneBox));
-    P.part = (SceneBox *)take(sizeof(SceneBox) * (blocks + 1));
+    P.part = (SceneBox *)take(sizeof(SceneBox) * (slots / 64 + 1));
     P.ctl = (Control *)take(sizeof(Control));
     uint32_t *bs = (uint32_t *)take(4 * blocks);
     P.block_scene = bs;
@@ -114,9 +114,10 @@ void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, fl
     const uint32_t nb = P.nblocks;
     const uint32_t grid_load = std::max<uint32_t>(1u, std::min<uint32_t>((std::max(n, ccap) + kBlock - 1) / kBlock,
                                                                          8192u));
-    const uint32_t grid_scan = std::min<uint32_t>((ccap + 2047) / 2048 + 1, 4096u);
-    const uint32_t grid_reduce = std::max<uint32_t>(1u, std::min<uint32_t>((nb + kBoxReduceSpan - 1) / kBoxReduceSpan,
-                                                                           4096u));
+    const uint32_t grid_scan = std::min<uint32_t>((ccap + 2047) / 2048 + 1, 4096u);  // tiles of 2048 cells
+    const uint32_t npart = nb * (kBlock / 64u);  // partial boxes (one per wave of slots)
+    const uint32_t grid_reduce =
+        std::max<uint32_t>(1u, std::min<uint32_t>((npart + kBoxReduceSpan - 1) / kBoxReduceSpan, 4096u));
     hipLaunchKernelGGL(k_ds_init, dim3(1), dim3(64), 0, stream, P, sp, params);
     hipLaunchKernelGGL(k_ds_load, dim3(grid_load), dim3(kBlock), 0, stream, P, n, nf, (const float *)pos,
                        (const float *)vel, pmass, block_scene);
@@ -129,13 +130,13 @@ void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, fl
     for (uint64_t k = 0; k < substeps; ++k) {
         hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream, P);
         hipLaunchKernelGGL(k_keys, dim3(nb), dim3(kBlock), 0, stream, P);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(grid_scan), dim3(kBlock), 0, stream, P, tile_sums);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream, P, tile_sums);
-        hipLaunchKernelGGL(k_scan_apply, dim3(grid_scan), dim3(kBlock), 0, stream, P, (const uint32_t *)tile_sums);
+        hipLaunchKernelGGL(k_scan_reduce<8>, dim3(grid_scan), dim3(kBlock), 0, stream, P, tile_sums);
+        hipLaunchKernelGGL(k_scan_tiles<8>, dim3(1), dim3(1024), 0, stream, P, tile_sums);
+        hipLaunchKernelGGL(k_scan_apply<8>, dim3(grid_scan), dim3(kBlock), 0, stream, P, (const uint32_t *)tile_sums);
         hipLaunchKernelGGL(k_place, dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_canon_gather, dim3(nb), dim3(kBlock), 0, stream, P);
-        hipLaunchKernelGGL(k_density, dim3(nb), dim3(kBlock), 0, stream, P);
-        hipLaunchKernelGGL(k_force_t<2>, dim3(nb), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL((k_density<256, 256>), dim3(nb), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL((k_force_t<2, 256>), dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_box_reduce, dim3(grid_reduce), dim3(kBlock), 0, stream, P);
     }
     hip_ok(hipGetLastError(), "device step launch");

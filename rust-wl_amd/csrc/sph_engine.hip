@@ -33,8 +33,7 @@ uint64_t Engine::pool_bytes(uint64_t slots, uint64_t ccap, bool diagnostics, boo
     // record 32, sort 16, neighbour record 40 (alloc_pool); diagnostics 36,
     // reaction buffers 32. Per cell: counts, start, fluid count (alloc_cells).
     const uint64_t per = 180ull + (diagnostics ? 36ull : 0ull) + (reactions ? 32ull : 0ull);
-    const uint64_t blocks = slots / kBlock;
-    return per * slots + 16ull * kReadPad + sizeof(SceneBox) * (blocks + 1) + 12ull * ccap + 4ull +
+    return per * slots + 16ull * kReadPad + sizeof(SceneBox) * (slots / kPartSlots + 1) + 12ull * ccap + 4ull +
            4ull * ((ccap + 2047) / 2048 + 1);
 }
 
@@ -295,7 +294,7 @@ void Engine::alloc_pool(uint32_t slots) {
     P_.sA = dalloc<float4>((size_t)slots + kReadPad);
     pool_allocs_.push_back(P_.sA);
     U(P_.sg);
-    P_.part = dalloc<SceneBox>(slots / kBlock + 1);
+    P_.part = dalloc<SceneBox>(slots / kPartSlots + 1);
     pool_allocs_.push_back(P_.part);
     P_.fr = dalloc<float4>(2ull * slots);
     pool_allocs_.push_back(P_.fr);
@@ -328,7 +327,7 @@ void Engine::alloc_cells(uint32_t ccap) {
     P_.fcount = dalloc<uint32_t>(ccap);
     pool_allocs_.push_back(P_.fcount);
     hip_check(hipMemset(P_.counts, 0, sizeof(uint32_t) * ccap), "memset counts");
-    tiles_cap_ = (ccap + 2047) / 2048 + 1;
+    tiles_cap_ = (ccap + kScanMinTile - 1) / kScanMinTile + 1;
     d_tile_sums_ = dalloc<uint32_t>(tiles_cap_);
     P_.ccap = ccap;
 }
@@ -868,9 +867,56 @@ void Engine::clear_walls(uint32_t id) {
     sc.wall_mode = kWallsOff;
 }
 
+namespace {
+
+// Template instance of a tuned kernel for its runtime knobs.
+using KernelFn = void (*)(Pool);
+KernelFn density_kernel(uint32_t bs, uint32_t stage) {
+    switch (bs * 1000u + stage) {
+        case 64128: return k_density<64, 128>;
+        case 64256: return k_density<64, 256>;
+        case 64512: return k_density<64, 512>;
+        case 128128: return k_density<128, 128>;
+        case 128256: return k_density<128, 256>;
+        case 128512: return k_density<128, 512>;
+        case 256128: return k_density<256, 128>;
+        case 256512: return k_density<256, 512>;
+        default: return k_density<256, 256>;
+    }
+}
+KernelFn density_walls_kernel(uint32_t bs) {
+    return bs == 64 ? k_density_walls<64> : bs == 128 ? k_density_walls<128> : k_density_walls<256>;
+}
+KernelFn force_kernel(uint32_t variant, uint32_t bs) {
+    const uint32_t w = variant == Engine::kForceScan        ? 0u
+                       : variant == Engine::kForceWalkAhead ? 2u
+                       : variant == Engine::kForcePairs     ? 3u
+                                                            : 1u;
+    switch (w * 1000u + bs) {
+        case 3064: return k_force_t<3, 64>;
+        case 3128: return k_force_t<3, 128>;
+        case 3256: return k_force_t<3, 256>;
+        case 64: return k_force_t<0, 64>;
+        case 128: return k_force_t<0, 128>;
+        case 256: return k_force_t<0, 256>;
+        case 1064: return k_force_t<1, 64>;
+        case 1128: return k_force_t<1, 128>;
+        case 2064: return k_force_t<2, 64>;
+        case 2128: return k_force_t<2, 128>;
+        case 2256: return k_force_t<2, 256>;
+        default: return k_force_t<1, 256>;
+    }
+}
+
+}  // namespace
+
 void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     const uint32_t nb = P_.nblocks;
     const uint32_t grid_slots = tuning_.persist_blocks ? std::min(nb, tuning_.persist_blocks) : nb;
+    // density and force: workgroups of block_threads slots
+    const uint32_t nsub = nb * (kBlock / tuning_.block_threads);
+    const uint32_t grid_nb = tuning_.persist_blocks ? std::min(nsub, tuning_.persist_blocks) : nsub;
+    const uint32_t bs = tuning_.block_threads;
     const uint32_t grid_scan = std::min<uint32_t>(tiles_cap_, 4096);
     auto M = [&](const char *name) {
         if (mark) (*mark)(name);
@@ -879,32 +925,48 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     M("grid_box");
     hipLaunchKernelGGL(k_keys, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("keys");
-    hipLaunchKernelGGL(k_scan_reduce, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
-    M("scan_reduce");
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, stream_, P_, d_tile_sums_);
-    M("scan_tiles");
-    hipLaunchKernelGGL(k_scan_apply, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+    switch (tuning_.scan_tile_cells) {
+        case 1024:
+            hipLaunchKernelGGL(k_scan_reduce<4>, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+            M("scan_reduce");
+            hipLaunchKernelGGL(k_scan_tiles<4>, dim3(1), dim3(1024), 0, stream_, P_, d_tile_sums_);
+            M("scan_tiles");
+            hipLaunchKernelGGL(k_scan_apply<4>, dim3(grid_scan), dim3(kBlock), 0, stream_, P_,
+                               (const uint32_t *)d_tile_sums_);
+            break;
+        case 4096:
+            hipLaunchKernelGGL(k_scan_reduce<16>, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+            M("scan_reduce");
+            hipLaunchKernelGGL(k_scan_tiles<16>, dim3(1), dim3(1024), 0, stream_, P_, d_tile_sums_);
+            M("scan_tiles");
+            hipLaunchKernelGGL(k_scan_apply<16>, dim3(grid_scan), dim3(kBlock), 0, stream_, P_,
+                               (const uint32_t *)d_tile_sums_);
+            break;
+        default:
+            hipLaunchKernelGGL(k_scan_reduce<8>, dim3(grid_scan), dim3(kBlock), 0, stream_, P_, d_tile_sums_);
+            M("scan_reduce");
+            hipLaunchKernelGGL(k_scan_tiles<8>, dim3(1), dim3(1024), 0, stream_, P_, d_tile_sums_);
+            M("scan_tiles");
+            hipLaunchKernelGGL(k_scan_apply<8>, dim3(grid_scan), dim3(kBlock), 0, stream_, P_,
+                               (const uint32_t *)d_tile_sums_);
+            break;
+    }
     M("scan_apply");
     hipLaunchKernelGGL(k_place, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("place");
     hipLaunchKernelGGL(k_canon_gather, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("canon_gather");
     if (wall_halfspace_)
-        hipLaunchKernelGGL(k_density_walls, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(density_walls_kernel(bs), dim3(grid_nb), dim3(bs), 0, stream_, P_);
     else
-        hipLaunchKernelGGL(k_density, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(density_kernel(bs, tuning_.stage_records), dim3(grid_nb), dim3(bs), 0, stream_, P_);
     M("density");
     if (wall_table_) {
         hipLaunchKernelGGL(k_wall_band, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
         hipLaunchKernelGGL(k_wall_density, dim3(std::min<uint32_t>(nb, 2048)), dim3(kBlock), 0, stream_, P_);
         M("wall_density");
     }
-    if (tuning_.force_kernel == kForceScan)
-        hipLaunchKernelGGL(k_force_t<0>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
-    else if (tuning_.force_kernel == kForceWalkAhead)
-        hipLaunchKernelGGL(k_force_t<2>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
-    else
-        hipLaunchKernelGGL(k_force_t<1>, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(force_kernel(tuning_.force_kernel, bs), dim3(grid_nb), dim3(bs), 0, stream_, P_);
     M("force");
     if (nwall_) {
         hipLaunchKernelGGL(k_wall_project, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
@@ -1020,26 +1082,49 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
         }
     }
     const uint32_t cus = (uint32_t)prop.multiProcessorCount;
-    std::vector<Tuning> space;
-    for (uint32_t fk : {(uint32_t)kForceScan, (uint32_t)kForceWalk, (uint32_t)kForceWalkAhead})
-        for (uint32_t pb : {0u, 8u * cus}) {
-            Tuning t;
-            t.force_kernel = fk;
-            t.persist_blocks = pb;
-            space.push_back(t);
-        }
-    const Tuning saved = tuning_;
-    Tuning best = saved;
-    float best_ms = 1e30f;
     const uint32_t n = std::max<uint32_t>(substeps, 1);
-    for (const Tuning &t : space) {
+    // Device time of n real substeps with tuning t (every variant gives the
+    // same results, so tuning never perturbs the trajectory).
+    auto cost = [&](const Tuning &t) {
         set_tuning(t);
+        (void)profile(1);  // first launch of a variant loads its code object: not timed
         float ms = 0.0f;
         for (const auto &kv : profile(n)) ms += kv.second;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = t;
-        }
+        return ms;
+    };
+    // Coordinate descent: each knob in turn over its values with the others
+    // held, until a pass changes nothing (each knob acts mostly on one
+    // kernel, so the knobs hardly interact). A change must win by 1 % to
+    // stand over run-to-run noise.
+    struct Axis {
+        uint32_t Tuning::*field;
+        std::vector<uint32_t> values;
+    };
+    const std::vector<Axis> axes = {
+        {&Tuning::force_kernel,
+         {(uint32_t)kForceScan, (uint32_t)kForceWalk, (uint32_t)kForceWalkAhead, (uint32_t)kForcePairs}},
+        {&Tuning::block_threads, {64u, 128u, 256u}},
+        {&Tuning::stage_records, {128u, 256u, 512u}},
+        {&Tuning::scan_tile_cells, {1024u, 2048u, 4096u}},
+        {&Tuning::persist_blocks, {0u, 8u * cus}},
+    };
+    Tuning best = tuning_;
+    float best_ms = cost(best);
+    for (int pass = 0; pass < 3; ++pass) {
+        bool changed = false;
+        for (const Axis &ax : axes)
+            for (uint32_t v : ax.values) {
+                if (best.*(ax.field) == v) continue;
+                Tuning t = best;
+                t.*(ax.field) = v;
+                const float ms = cost(t);
+                if (ms < 0.99f * best_ms) {
+                    best_ms = ms;
+                    best = t;
+                    changed = true;
+                }
+            }
+        if (!changed) break;
     }
     set_tuning(best);
     std::lock_guard<std::mutex> lk(mu);
@@ -1065,13 +1150,25 @@ std::vector<float> Engine::step_timed(uint32_t ticks) {
 }
 
 uint32_t Engine::box_reduce_grid() const {
-    return std::max<uint32_t>(1u, std::min<uint32_t>((P_.nblocks + kBoxReduceSpan - 1) / kBoxReduceSpan, 4096u));
+    const uint32_t npart = P_.nblocks * (kBlock / kPartSlots);
+    return std::max<uint32_t>(1u, std::min<uint32_t>((npart + kBoxReduceSpan - 1) / kBoxReduceSpan, 4096u));
 }
 
 void Engine::set_tuning(const Tuning &t) {
-    tuning_ = t;
-    if (tuning_.force_kernel == 0) tuning_.force_kernel = Tuning{}.force_kernel;
-    if (tuning_.force_kernel > kForceWalkAhead) throw std::invalid_argument("unknown kernel variant");
+    Tuning u = t;
+    const Tuning d{};
+    if (u.force_kernel == 0) u.force_kernel = d.force_kernel;
+    if (u.block_threads == 0) u.block_threads = d.block_threads;
+    if (u.stage_records == 0) u.stage_records = d.stage_records;
+    if (u.scan_tile_cells == 0) u.scan_tile_cells = d.scan_tile_cells;
+    if (u.force_kernel > kForcePairs) throw std::invalid_argument("unknown kernel variant");
+    if (u.block_threads != 64 && u.block_threads != 128 && u.block_threads != 256)
+        throw std::invalid_argument("block_threads must be 64, 128 or 256");
+    if (u.stage_records != 128 && u.stage_records != 256 && u.stage_records != 512)
+        throw std::invalid_argument("stage_records must be 128, 256 or 512");
+    if (u.scan_tile_cells != 1024 && u.scan_tile_cells != 2048 && u.scan_tile_cells != 4096)
+        throw std::invalid_argument("scan_tile_cells must be 1024, 2048 or 4096");
+    tuning_ = u;
     invalidate_graph();
 }
 

@@ -88,14 +88,23 @@ constexpr uint32_t kBoxReduceSpan = kBlock * 8;  // partial boxes per k_box_redu
 __global__ void k_box_reduce(Pool P);
 __global__ void k_grid_box(Pool P);
 __global__ void k_keys(Pool P);
+// ITEMS cells per thread: a scan tile holds kBlock * ITEMS cells (4, 8, 16).
+template <uint32_t ITEMS>
 __global__ void k_scan_reduce(Pool P, uint32_t *tile_sums);
+template <uint32_t ITEMS>
 __global__ void k_scan_tiles(Pool P, uint32_t *tile_sums);
+template <uint32_t ITEMS>
 __global__ void k_scan_apply(Pool P, const uint32_t *tile_sums);
+constexpr uint32_t kScanMinTile = kBlock * 4;  // smallest tile (sizes the tile-sum array)
 __global__ void k_place(Pool P);
 __global__ void k_canon_gather(Pool P);
+// BS threads per workgroup (64, 128, 256); STAGE records of LDS staging per
+// wave and neighbour row (128, 256, 512).
+template <uint32_t BS, uint32_t STAGE>
 __global__ void k_density(Pool P);
+template <uint32_t BS>
 __global__ void k_density_walls(Pool P);  // pools with half-space walls: mirror ghosts in the sums
-template <uint32_t WALK>
+template <uint32_t WALK, uint32_t BS>
 __global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks;
                                       // 2: the walk, loads one pair ahead
 __global__ void k_bforce(Pool P);

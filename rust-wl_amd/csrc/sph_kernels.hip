@@ -24,7 +24,7 @@ namespace {
 
 using dev::at;
 using dev::axis_bin;
-using dev::block_box;
+using dev::wave_box;
 using dev::Cursor;
 using dev::for_row_windows;
 using dev::for_rows;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void k_box(Pool P) {
         const bool live = m < sp.pbase + sp.n;
         float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (live) a = P.cA[m];
-        block_box(live, a.x, a.y, a.z, &P.part[b]);
+        wave_box(live, a.x, a.y, a.z, &P.part[m / kPartSlots]);
     }
 }
 
@@ -180,13 +180,13 @@ __global__ __launch_bounds__(kBlock) void k_boundary_scatter(Pool P, uint32_t s,
                 P.cV[m] = v;
             }
         }
-        block_box(live, a.x, a.y, a.z, &P.part[b]);
+        wave_box(live, a.x, a.y, a.z, &P.part[m / kPartSlots]);
     }
 }
 
-// Per-scene AABB = reduction of its blocks' partial boxes into box[parity]
+// Per-scene AABB = reduction of its waves' partial boxes into box[parity]
 // (reset to the empty box by k_grid_box / the layout). Each thread folds
-// kBoxChunk consecutive partial boxes (scenes own contiguous block ranges, so
+// kBoxChunk consecutive partial boxes (scenes own contiguous slot ranges, so
 // a thread meets few scene boundaries), each wave folds its lanes when they
 // share one scene, and one lane per wave merges the result with atomics.
 constexpr uint32_t kBoxChunk = kBoxReduceSpan / kBlock;
@@ -202,13 +202,14 @@ __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
     if (P.ctl->err) return;
     const uint32_t par = P.ctl->step & 1u;
     SceneBox *out = P.box + par * P.nscenes;
-    for (uint32_t base = blockIdx.x * kBlock * kBoxChunk; base < P.nblocks; base += gridDim.x * kBlock * kBoxChunk) {
+    const uint32_t npart = P.nblocks * (kBlock / kPartSlots);
+    for (uint32_t base = blockIdx.x * kBlock * kBoxChunk; base < npart; base += gridDim.x * kBlock * kBoxChunk) {
         uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
         uint32_t cur = 0xffffffffu;  // scene of the running fold (none yet)
-        const uint32_t b0 = base + threadIdx.x * kBoxChunk;
-        for (uint32_t k = 0; k < kBoxChunk && b0 + k < P.nblocks; ++k) {
-            const uint32_t b = b0 + k;
-            const uint32_t sc = P.block_scene[b];
+        const uint32_t w0 = base + threadIdx.x * kBoxChunk;
+        for (uint32_t k = 0; k < kBoxChunk && w0 + k < npart; ++k) {
+            const uint32_t w = w0 + k;
+            const uint32_t sc = P.block_scene[w / (kBlock / kPartSlots)];
             if (sc != cur) {
                 if (cur != 0xffffffffu) box_merge(&out[cur], lo, hi);
                 cur = sc;
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
                     hi[a] = 0u;
                 }
             }
-            const SceneBox q = P.part[b];
+            const SceneBox q = P.part[w];
             for (int a = 0; a < 3; ++a) {
                 lo[a] = min(lo[a], q.lo[a]);
                 hi[a] = max(hi[a], q.hi[a]);
@@ -367,18 +368,21 @@ __global__ __launch_bounds__(kBlock) void k_keys(Pool P) {
 }
 
 // --- exclusive scan of counts[0, ctot) into start[0, ctot]; counts zeroed ---
-constexpr uint32_t kScanItems = 8;
-constexpr uint32_t kScanTile = kBlock * kScanItems;
+// A tile of kBlock * ITEMS consecutive cells per block (ITEMS = 4, 8, 16:
+// the cells-per-tile knob of the autotuner); tile sums, their scan, then the
+// tiles again with their offsets.
 
+template <uint32_t ITEMS>
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(Pool P, uint32_t *tile_sums) {
+    constexpr uint32_t kTile = kBlock * ITEMS;
     if (P.ctl->err) return;
     const uint32_t ctot = P.ctl->ctot;
-    const uint32_t ntiles = (ctot + kScanTile - 1) / kScanTile;
+    const uint32_t ntiles = (ctot + kTile - 1) / kTile;
     __shared__ uint32_t sh[kBlock / 64];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         uint32_t acc = 0;
-        const uint32_t base = t * kScanTile;
-        for (uint32_t k = 0; k < kScanItems; ++k) {
+        const uint32_t base = t * kTile;
+        for (uint32_t k = 0; k < ITEMS; ++k) {
             const uint32_t c = base + k * kBlock + threadIdx.x;
             if (c < ctot) acc += P.counts[c];
         }
@@ -386,18 +390,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(Pool P, uint32_t *tile_s
         if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t s = 0;
-            for (uint32_t w = 0; w < kBlock / 64; ++w) s += sh[w];
-            tile_sums[t] = s;
+            uint32_t sum = 0;
+            for (uint32_t w = 0; w < kBlock / 64; ++w) sum += sh[w];
+            tile_sums[t] = sum;
         }
         __syncthreads();
     }
 }
 
+template <uint32_t ITEMS>
 __global__ __launch_bounds__(1024) void k_scan_tiles(Pool P, uint32_t *tile_sums) {
+    constexpr uint32_t kTile = kBlock * ITEMS;
     if (P.ctl->err) return;
     const uint32_t ctot = P.ctl->ctot;
-    const uint32_t ntiles = (ctot + kScanTile - 1) / kScanTile;
+    const uint32_t ntiles = (ctot + kTile - 1) / kTile;
     __shared__ uint32_t sh[1024];
     __shared__ uint32_t carry;
     if (threadIdx.x == 0) carry = 0;
@@ -421,26 +427,32 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(Pool P, uint32_t *tile_sums
     if (threadIdx.x == 0) P.start[ctot] = carry;
 }
 
+template <uint32_t ITEMS>
 __global__ __launch_bounds__(kBlock) void k_scan_apply(Pool P, const uint32_t *tile_sums) {
+    static_assert(ITEMS % 4 == 0, "cells move in 16-byte groups");
+    constexpr uint32_t kTile = kBlock * ITEMS;
     if (P.ctl->err) return;
     const uint32_t ctot = P.ctl->ctot;
-    const uint32_t ntiles = (ctot + kScanTile - 1) / kScanTile;
+    const uint32_t ntiles = (ctot + kTile - 1) / kTile;
     __shared__ uint32_t sh[kBlock];
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        // Each thread owns kScanItems consecutive cells (two 16-byte loads
-        // when all of them are in range: base is a multiple of 8).
-        const uint32_t base = t * kScanTile + threadIdx.x * kScanItems;
-        uint32_t v[kScanItems];
+        // Each thread owns ITEMS consecutive cells (16-byte loads when all of
+        // them are in range: base is a multiple of 4 and the tables are
+        // 16-byte aligned).
+        const uint32_t base = t * kTile + threadIdx.x * ITEMS;
+        uint32_t v[ITEMS];
         uint32_t local = 0;
-        const bool whole = base + kScanItems <= ctot;
+        const bool whole = base + ITEMS <= ctot;
         if (whole) {
-            const uint4 a = *(const uint4 *)(P.counts + base), b4 = *(const uint4 *)(P.counts + base + 4u);
-            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-            v[4] = b4.x; v[5] = b4.y; v[6] = b4.z; v[7] = b4.w;
+#pragma unroll
+            for (uint32_t q = 0; q < ITEMS / 4; ++q) {
+                const uint4 a = *(const uint4 *)(P.counts + base + 4u * q);
+                v[4 * q] = a.x; v[4 * q + 1] = a.y; v[4 * q + 2] = a.z; v[4 * q + 3] = a.w;
+            }
         } else {
-            for (uint32_t k = 0; k < kScanItems; ++k) v[k] = base + k < ctot ? P.counts[base + k] : 0u;
+            for (uint32_t k = 0; k < ITEMS; ++k) v[k] = base + k < ctot ? P.counts[base + k] : 0u;
         }
-        for (uint32_t k = 0; k < kScanItems; ++k) local += v[k];
+        for (uint32_t k = 0; k < ITEMS; ++k) local += v[k];
         sh[threadIdx.x] = local;
         __syncthreads();
         for (uint32_t o = 1; o < kBlock; o <<= 1) {
@@ -451,18 +463,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(Pool P, const uint32_t *t
         }
         uint32_t run = tile_sums[t] + sh[threadIdx.x] - local;
         if (whole) {
-            uint32_t st[kScanItems];
-            for (uint32_t k = 0; k < kScanItems; ++k) {
+            uint32_t st[ITEMS];
+            for (uint32_t k = 0; k < ITEMS; ++k) {
                 st[k] = run;
                 run += v[k];
             }
-            // start is 16-byte aligned and base a multiple of 8
-            *(uint4 *)(P.start + base) = make_uint4(st[0], st[1], st[2], st[3]);
-            *(uint4 *)(P.start + base + 4u) = make_uint4(st[4], st[5], st[6], st[7]);
-            *(uint4 *)(P.counts + base) = make_uint4(0u, 0u, 0u, 0u);
-            *(uint4 *)(P.counts + base + 4u) = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+            for (uint32_t q = 0; q < ITEMS / 4; ++q) {
+                *(uint4 *)(P.start + base + 4u * q) = make_uint4(st[4 * q], st[4 * q + 1], st[4 * q + 2], st[4 * q + 3]);
+                *(uint4 *)(P.counts + base + 4u * q) = make_uint4(0u, 0u, 0u, 0u);
+            }
         } else {
-            for (uint32_t k = 0; k < kScanItems; ++k) {
+            for (uint32_t k = 0; k < ITEMS; ++k) {
                 const uint32_t c = base + k;
                 if (c < ctot) {
                     P.start[c] = run;
@@ -474,6 +486,16 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(Pool P, const uint32_t *t
         __syncthreads();
     }
 }
+
+template __global__ void k_scan_reduce<4>(Pool, uint32_t *);
+template __global__ void k_scan_reduce<8>(Pool, uint32_t *);
+template __global__ void k_scan_reduce<16>(Pool, uint32_t *);
+template __global__ void k_scan_tiles<4>(Pool, uint32_t *);
+template __global__ void k_scan_tiles<8>(Pool, uint32_t *);
+template __global__ void k_scan_tiles<16>(Pool, uint32_t *);
+template __global__ void k_scan_apply<4>(Pool, const uint32_t *);
+template __global__ void k_scan_apply<8>(Pool, const uint32_t *);
+template __global__ void k_scan_apply<16>(Pool, const uint32_t *);
 
 // Scatter every cur slot to its (not yet canonical) place inside its cell.
 __global__ __launch_bounds__(kBlock) void k_place(Pool P) {
@@ -671,8 +693,8 @@ __device__ __forceinline__ uint32_t dens_row(const Src &src, const float4 ai, ui
 }
 
 // Records one wave may stage per neighbour row (the union of its lanes' row
-// windows); larger unions read global memory.
-constexpr uint32_t kStageRecords = 256;
+// windows; STAGE = 128, 256 or 512, the LDS staging depth the autotuner
+// picks); larger unions read global memory.
 
 // Density (fluid_density.rs:17-70) + Tait (fluid_force.rs:11-29, gamma = 7)
 // + the per-particle pressure term c2 = p / (rho * rho) used by the force,
@@ -682,16 +704,20 @@ constexpr uint32_t kStageRecords = 256;
 // accepted neighbours (nbm, nbs) for the force pass. WALLS: the pool has
 // scenes with half-space walls, whose band particles add the mirror ghosts of
 // their neighbours (fluid_density.rs:24-51, :76-121).
-template <bool WALLS>
+// BS = threads per workgroup (64, 128 or 256: a workgroup stays inside one
+// kBlock-slot pool block, so inside one scene).
+template <bool WALLS, uint32_t BS, uint32_t STAGE>
 __device__ __forceinline__ void density_pass(const Pool &P) {
-    __shared__ float4 dens_stage[kBlock / 64][kStageRecords];
+    __shared__ float4 dens_stage[BS / 64][STAGE];
     if (P.ctl->err) return;
     const uint32_t stride = P.nblocks * kBlock;
-    for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
-        const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
-        const uint32_t s = P.block_scene[b];
+    constexpr uint32_t kSub = kBlock / BS;
+    const uint32_t nsub = P.nblocks * kSub;
+    for (uint32_t hb = blockIdx.x; hb < nsub; hb += gridDim.x) {
+        const uint32_t b = gridDim.x == nsub ? xcd_block(hb, nsub) : hb;
+        const uint32_t s = P.block_scene[b / kSub];
         const SceneParams &sp = P.params[s];
-        const uint32_t i = b * kBlock + threadIdx.x;
+        const uint32_t i = b * BS + threadIdx.x;
         if (i >= sp.pbase + sp.n) continue;
         const float4 ai = P.sA[i];
         if (sp.nf == 0u) {
@@ -781,7 +807,7 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
                             j0 = P.start[cf];
                             j1 = P.start[cl + 1u];
                         }
-                        if (n + 4u > kStageRecords) {
+                        if (n + 4u > STAGE) {
                             if (has) scan_row(da, gsrc, r, xl < cur.cx, j0, j1);
                             continue;
                         }
@@ -855,13 +881,28 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
 
 // Six waves per SIMD (at most 80 VGPRs): the kernel is bound by vector
 // memory and LDS traffic, so occupancy pays more than the registers the
-// compiler would otherwise spend on scheduling (it fits without spills).
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_density(Pool P) {
-    density_pass<false>(P);
+// compiler would otherwise spend on scheduling (it fits without spills). A
+// 512-record stage holds 8 KB of LDS per wave, which allows fewer waves.
+template <uint32_t BS, uint32_t STAGE>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(STAGE > 256 ? 4 : 6, 6))) void k_density(Pool P) {
+    density_pass<false, BS, STAGE>(P);
 }
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_density_walls(Pool P) {
-    density_pass<true>(P);
+template <uint32_t BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_density_walls(Pool P) {
+    density_pass<true, BS, 256>(P);
 }
+template __global__ void k_density<64, 128>(Pool);
+template __global__ void k_density<64, 256>(Pool);
+template __global__ void k_density<64, 512>(Pool);
+template __global__ void k_density<128, 128>(Pool);
+template __global__ void k_density<128, 256>(Pool);
+template __global__ void k_density<128, 512>(Pool);
+template __global__ void k_density<256, 128>(Pool);
+template __global__ void k_density<256, 256>(Pool);
+template __global__ void k_density<256, 512>(Pool);
+template __global__ void k_density_walls<64>(Pool);
+template __global__ void k_density_walls<128>(Pool);
+template __global__ void k_density_walls<256>(Pool);
 
 
 
@@ -921,6 +962,227 @@ __device__ __forceinline__ void nb_record(const float4 r0, const float4 r1, uint
     cj = r0.w;
 }
 
+// The walk of the recorded neighbours with each pair's record loaded one
+// pair ahead of its evaluation (same pairs, same order as a plain walk).
+__device__ __forceinline__ void walk_ahead(ForceAcc &f, const Pool &P, const SceneGrid &g, const SceneParams &sp,
+                                           uint32_t i, bool fluid, const float4 ai, const float4 vi, float ci2,
+                                           const RowTab *rowtab, uint32_t stride) {
+    const float4 *__restrict__ sA = P.sA;
+    const float4 *__restrict__ fr = P.fr;
+    const bool own_mass = sp.fmass_uniform == 0u;
+    const float h2 = sp.h2;
+    Cursor cur{};
+    if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+    NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur, rowtab);
+    const ForceConst fc = force_const(sp);
+    uint32_t jn = 0;
+    int stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
+    float4 rn0 = ai, rn1 = vi;
+    if (stn == kWalkPair) {
+        rn0 = *at(fr, 2u * jn);
+        rn1 = *at(fr, 2u * jn + 1u);
+    }
+    for (;;) {
+        const int st = stn;
+        if (!__any(st != kWalkDone)) break;
+        const float4 r0 = rn0, r1 = rn1;
+        const uint32_t j = jn;
+        if (st != kWalkDone) {
+            stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
+            if (stn == kWalkPair) {
+                rn0 = *at(fr, 2u * jn);
+                rn1 = *at(fr, 2u * jn + 1u);
+            }
+        }
+        if (st == kWalkPair) {
+            float4 aj, vj;
+            float cj;
+            nb_record(r0, r1, j, sA, sp.mass, own_mass, aj, vj, cj);
+            force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
+        }
+    }
+}
+
+// The three terms one neighbour adds to a particle's force sums, as
+// force_pair forms them (fluid_force.rs:165-217), kept apart so that another
+// lane can add them in the particle's order: pressure d * (coef / denom),
+// viscosity d * (cv / denom) when approaching (av), XSPH (v_j - v_i) * fx.
+struct PairTerms {
+    float px, py, pz, vx, vy, vz, xx, xy, xz;
+    uint32_t av;
+};
+
+__device__ __forceinline__ PairTerms pair_terms(const float4 ai, const float4 vi, const float ci2, const float4 aj,
+                                                const float4 vj, const float cj2, const ForceConst &fc) {
+    PairTerms o;
+    const float h = fc.h, h2 = fc.h2, k6 = fc.k6, ks = fc.ks, alpha_c = fc.alpha_c;
+    const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+    const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    const float rho_i = vi.w, rho_j = vj.w;
+    const float r = sqrtf(r2);
+    const float t = h - r;
+    const float mj = fabsf(aj.w);
+    const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
+    const float denom = fmaxf(r, 1e-9f);
+    const float qp = coef / denom;
+    o.px = ddx * qp;
+    o.py = ddy * qp;
+    o.pz = ddz * qp;
+    const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
+    const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
+    o.av = vdn > 0.0f ? 1u : 0u;
+    o.vx = o.vy = o.vz = 0.0f;
+    if (vdn > 0.0f) {
+        const float mu = (vdn * h) / (r2 + fc.h2c);
+        const float cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+        const float qv = cv / denom;
+        o.vx = ddx * qv;
+        o.vy = ddy * qv;
+        o.vz = ddz * qv;
+    }
+    const float tt = h2 - r2;
+    const float w = ((k6 * tt) * tt) * tt;
+    const float fx = ((mj * 2.0f) / (rho_i + rho_j)) * w;
+    o.xx = (vj.x - vi.x) * fx;
+    o.xy = (vj.y - vi.y) * fx;
+    o.xz = (vj.z - vi.z) * fx;
+    return o;
+}
+
+// LDS of one wave in the pair-parallel force pass.
+constexpr uint32_t kPairCap = 256;    // pairs listed at once (groups of ranks)
+constexpr uint32_t kPairRanks = 33;   // ranks of one row (32 mask bits) + 1
+constexpr uint32_t kPairStage = 128;  // neighbour records of one row staged per wave
+struct PairLds {
+    float4 stage[kPairStage][2];    // the row's neighbour records (density record halves)
+    float4 term[64][3];             // a chunk's terms: pressure (w: viscosity flag), viscosity, XSPH
+    float4 own[64][2];              // the wave's particles: (x, y, z, c2), (v, rho)
+    uint16_t list[kPairCap];        // (staged index << 6) | owner lane
+    uint16_t link[kPairCap];        // position of the owner's next pair in the group (0xffff: none)
+    uint64_t act[kPairRanks];       // lanes holding a pair of rank k
+};
+
+// Pair-parallel force sums of one wave (every lane without a row longer than
+// 32 candidates). Row by row in the reference order: the records of the
+// row's accepted neighbours are copied to LDS once for the wave with
+// coalesced loads, the lanes' neighbours (the density pass's masks) are
+// listed rank by rank (rank k = every lane's k-th neighbour in the row), all
+// 64 lanes evaluate the listed pairs' terms evenly, a chunk of 64 at a time,
+// and each particle's lane then adds its own terms in rank order, which is
+// its neighbour order: the reference's sums, bit for bit. A row whose
+// neighbours span more than the stage is walked lane by lane from global
+// memory.
+__device__ __forceinline__ void pair_parallel(ForceAcc &f, const Pool &P, const SceneGrid &g, const SceneParams &sp,
+                                              uint32_t i, bool fluid, const float4 ai, const float4 vi, float ci2,
+                                              const RowTab *rowtab, PairLds &L) {
+    const float4 *__restrict__ sA = P.sA;
+    const float4 *__restrict__ fr = P.fr;
+    const bool own_mass = sp.fmass_uniform == 0u;
+    const ForceConst fc = force_const(sp);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+    L.own[lane][0] = make_float4(ai.x, ai.y, ai.z, ci2);
+    L.own[lane][1] = vi;
+    Cursor cur{};
+    if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+    const uint32_t summ = fluid ? P.nbs[i] : 0u;
+    const uint32_t rowb =
+        g.cbase + ((uint32_t)(cur.cz - 1) * g.dims[1] + (uint32_t)(cur.cy - 1)) * g.dims[0] + (uint32_t)cur.cx;
+    for (uint32_t r = 0; r < 9u; ++r) {
+        const bool has = ((summ >> r) & 1u) != 0u;
+        if (!__any(has)) continue;
+        uint32_t m = 0, j0 = 0;
+        if (has) {
+            const RowTab t = rowtab[r];
+            const uint32_t row = rowb + t.cell;
+            j0 = *at(P.start, row - ((summ >> (9u + r)) & 1u));
+            m = P.nbm[t.mask + i];
+        }
+        // the wave's neighbours of this row lie in [jb, je)
+        const uint32_t jb = dev::wave_min_u32(m ? j0 : 0xffffffffu);
+        const uint32_t je = dev::wave_max_u32(m ? j0 + 32u - (uint32_t)__builtin_clz(m) : 0u);
+        if (jb == 0xffffffffu) continue;
+        if (je - jb > kPairStage) {
+            for (uint32_t mm = m; mm != 0u; mm &= mm - 1u) {
+                const uint32_t j = j0 + (uint32_t)__builtin_ctz(mm);
+                float4 aj, vj;
+                float cj;
+                nb_record(*at(fr, 2u * j), *at(fr, 2u * j + 1u), j, sA, sp.mass, own_mass, aj, vj, cj);
+                force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
+            }
+            continue;
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t k = lane; k < 2u * (je - jb); k += 64u) (&L.stage[0][0])[k] = *at(fr, 2u * jb + k);
+        const uint32_t k_mine = (uint32_t)__builtin_popcount(m);
+        uint32_t k_fold = 0;  // this lane's next rank to add
+        uint32_t rank = 0;    // ranks listed so far in this row
+        for (;;) {
+            // List ranks while a whole rank still fits; link each lane's pairs.
+            const uint32_t rank0 = rank;
+            uint32_t n = 0, prev = 0xffffu, first = 0xffffu;
+            while (n + 64u <= kPairCap) {
+                const uint64_t act = __ballot(m != 0u);
+                if (act == 0ull) break;
+                if (m != 0u) {
+                    const uint32_t pos = n + (uint32_t)__popcll(act & below);
+                    L.list[pos] = (uint16_t)(((j0 + (uint32_t)__builtin_ctz(m) - jb) << 6) | lane);
+                    L.link[pos] = 0xffffu;
+                    if (prev != 0xffffu) L.link[prev] = (uint16_t)pos;
+                    if (first == 0xffffu) first = pos;
+                    prev = pos;
+                    m &= m - 1u;
+                }
+                if (lane == 0u) L.act[rank - rank0] = act;
+                n += (uint32_t)__popcll(act);
+                ++rank;
+            }
+            if (n == 0u) break;
+            __builtin_amdgcn_wave_barrier();
+            uint32_t pos = first;
+            for (uint32_t c0 = 0; c0 < n; c0 += 64u) {
+                const uint32_t p = c0 + lane;
+                if (p < n) {
+                    const uint32_t e = L.list[p];
+                    const uint32_t il = e & 63u, k = e >> 6, j = jb + k;
+                    const float4 r0 = L.stage[k][0], r1 = L.stage[k][1];
+                    float4 aj, vj;
+                    float cj;
+                    nb_record(r0, r1, j, sA, sp.mass, own_mass, aj, vj, cj);
+                    const float4 oi = L.own[il][0];
+                    const PairTerms t =
+                        pair_terms(make_float4(oi.x, oi.y, oi.z, 0.0f), L.own[il][1], oi.w, aj, vj, cj, fc);
+                    L.term[lane][0] = make_float4(t.px, t.py, t.pz, __uint_as_float(t.av));
+                    L.term[lane][1] = make_float4(t.vx, t.vy, t.vz, 0.0f);
+                    L.term[lane][2] = make_float4(t.xx, t.xy, t.xz, 0.0f);
+                }
+                __builtin_amdgcn_wave_barrier();
+                while (pos < c0 + 64u) {
+                    const uint32_t q = pos - c0;
+                    const float4 tp = L.term[q][0], tx = L.term[q][2];
+                    f.ax = f.ax + tp.x;
+                    f.ay = f.ay + tp.y;
+                    f.az = f.az + tp.z;
+                    if (__float_as_uint(tp.w) != 0u) {
+                        const float4 tv = L.term[q][1];
+                        f.ax = f.ax + tv.x;
+                        f.ay = f.ay + tv.y;
+                        f.az = f.az + tv.z;
+                    }
+                    f.xx = f.xx + tx.x;
+                    f.xy = f.xy + tx.y;
+                    f.xz = f.xz + tx.z;
+                    ++k_fold;
+                    pos = L.link[pos];
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        (void)k_mine;
+        (void)k_fold;
+    }
+}
+
 // Force (fluid_force.rs:165-217: gravity + spiky pressure + Monaghan AV, XSPH)
 // fused with the semi-implicit Euler + CFL clamp of fluid_step.rs:110-123.
 // Writes the next state into the cur arrays (this substep's grid order) and
@@ -933,21 +1195,25 @@ __device__ __forceinline__ void nb_record(const float4 r0, const float4 r1, uint
 // positions, same test), so the candidate scan is not repeated; the wave
 // evaluates one pair per lane per iteration.
 // WALK = 2: the same walk with the next pair's loads issued one pair ahead.
-template <uint32_t WALK>
-__global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
+//
+// BS = threads per workgroup (64, 128 or 256, inside one pool block).
+template <uint32_t WALK, uint32_t BS>
+__global__ __launch_bounds__(BS) void k_force_t(Pool P) {
     __shared__ RowTab rowtab[9];
     if (P.ctl->err) return;
     const uint32_t stride = P.nblocks * kBlock;
-    for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
-        const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
-        const uint32_t s = P.block_scene[b];
+    constexpr uint32_t kSub = kBlock / BS;
+    const uint32_t nsub = P.nblocks * kSub;
+    for (uint32_t hb = blockIdx.x; hb < nsub; hb += gridDim.x) {
+        const uint32_t b = gridDim.x == nsub ? xcd_block(hb, nsub) : hb;
+        const uint32_t s = P.block_scene[b / kSub];
         const SceneParams &sp = P.params[s];
         if constexpr (WALK != 0) {
             __syncthreads();  // the previous block's walks are done with the table
             row_table(rowtab, P.grid[s], stride);
             __syncthreads();
         }
-        const uint32_t i = b * kBlock + threadIdx.x;
+        const uint32_t i = b * BS + threadIdx.x;
         const bool live = i < sp.pbase + sp.n;
         // Own state: A (position, signed mass) and the density record
         // (velocity, rho, c2).
@@ -1001,38 +1267,19 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
                     force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
                 }
             }
+        } else if constexpr (WALK == 2) {
+            walk_ahead(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride);
         } else {
-            // The walk of WALK = 1 with the next pair's record loaded before
-            // the current pair is evaluated (same pairs, same order).
-            Cursor cur{};
-            if (fluid) cur = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
-            NbWalk w = nb_begin(fluid ? P.nbs[i] : 0u, g, cur, rowtab);
-            const ForceConst fc = force_const(sp);
-            uint32_t jn = 0;
-            int stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
-            float4 rn0 = ai, rn1 = vi;
-            if (stn == kWalkPair) {
-                rn0 = *at(fr, 2u * jn);
-                rn1 = *at(fr, 2u * jn + 1u);
-            }
-            for (;;) {
-                const int st = stn;
-                if (!__any(st != kWalkDone)) break;
-                const float4 r0 = rn0, r1 = rn1;
-                const uint32_t j = jn;
-                if (st != kWalkDone) {
-                    stn = nb_step(w, P, g, cur, i, ai, h2, stride, jn);
-                    if (stn == kWalkPair) {
-                        rn0 = *at(fr, 2u * jn);
-                        rn1 = *at(fr, 2u * jn + 1u);
-                    }
-                }
-                if (st == kWalkPair) {
-                    float4 aj, vj;
-                    float cj;
-                    nb_record(r0, r1, j, sA, sp.mass, own_mass, aj, vj, cj);
-                    force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
-                }
+            // Pair-parallel rows (pair_parallel) for waves whose rows all fit a
+            // 32-bit mask and whose neighbours lie within 2^26 slots of the
+            // row's first candidate; the walk otherwise.
+            const uint32_t summ = fluid ? P.nbs[i] : 0u;
+            const bool tail = ((summ >> 18) & kRowsPresent) != 0u;
+            __shared__ PairLds pair_lds[BS / 64];
+            if (__any(tail)) {
+                walk_ahead(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride);
+            } else {
+                pair_parallel(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, pair_lds[threadIdx.x >> 6]);
             }
         }
         float ox = ai.x, oy = ai.y, oz = ai.z;
@@ -1062,13 +1309,22 @@ __global__ __launch_bounds__(kBlock) void k_force_t(Pool P) {
             P.cV[i] = vo;
             P.cg[i] = P.sg[i];
         }
-        block_box(live, ox, oy, oz, &P.part[b]);
+        wave_box(live, ox, oy, oz, &P.part[i / kPartSlots]);
     }
 }
 
-template __global__ void k_force_t<0>(Pool P);
-template __global__ void k_force_t<1>(Pool P);
-template __global__ void k_force_t<2>(Pool P);
+template __global__ void k_force_t<0, 64>(Pool P);
+template __global__ void k_force_t<1, 64>(Pool P);
+template __global__ void k_force_t<2, 64>(Pool P);
+template __global__ void k_force_t<0, 128>(Pool P);
+template __global__ void k_force_t<1, 128>(Pool P);
+template __global__ void k_force_t<2, 128>(Pool P);
+template __global__ void k_force_t<0, 256>(Pool P);
+template __global__ void k_force_t<1, 256>(Pool P);
+template __global__ void k_force_t<2, 256>(Pool P);
+template __global__ void k_force_t<3, 64>(Pool P);
+template __global__ void k_force_t<3, 128>(Pool P);
+template __global__ void k_force_t<3, 256>(Pool P);
 
 // Reaction on every boundary particle of a scene with reactions:
 // bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending

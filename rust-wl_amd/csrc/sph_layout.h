@@ -19,6 +19,7 @@
 namespace vxl {
 
 constexpr uint32_t kBlock = 256;        // slots per block (4 wave64)
+constexpr uint32_t kPartSlots = 64;     // slots per partial AABB (one wave)
 constexpr uint32_t kGridDimMax = 1u << 14;  // core/grid.rs:8
 // Slots of one pool. The neighbour loops address per-slot arrays through
 // 32-bit byte offsets from a scalar base (dev::at); the widest array read

@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_body_refresh(Pool P) {
             const float4 v = P.cV[m];
             P.cV[m] = make_float4(wv.x, wv.y, wv.z, v.w);
         }
-        dev::block_box(live, a.x, a.y, a.z, &P.part[b]);
+        dev::wave_box(live, a.x, a.y, a.z, &P.part[m / kPartSlots]);
     }
 }
 

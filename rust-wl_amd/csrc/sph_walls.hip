@@ -21,7 +21,7 @@ namespace vxl {
 namespace {
 
 using dev::at;
-using dev::block_box;
+using dev::wave_box;
 using dev::Cursor;
 using dev::for_rows;
 using dev::make_cursor;
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_box(Pool P) {
         const bool live = i < sp.pbase + sp.n;
         float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (live) a = P.cA[i];
-        block_box(live, a.x, a.y, a.z, &P.part[b]);
+        wave_box(live, a.x, a.y, a.z, &P.part[i / kPartSlots]);
     }
 }
 

@@ -281,11 +281,14 @@ VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t) {
     if (!e || !t) return fail(VXL_SPH_NULL_POINTER, "null argument");
     VxlSphStatus st = check_header(t->hdr, sizeof(VxlSphTuning));
     if (st != VXL_SPH_OK) return st;
-    if (t->force_kernel > VXL_SPH_FORCE_WALK_AHEAD) return fail(VXL_SPH_INVALID_ARGUMENT, "unknown kernel variant");
+    if (t->force_kernel > VXL_SPH_FORCE_PAIRS) return fail(VXL_SPH_INVALID_ARGUMENT, "unknown kernel variant");
     return guard([&] {
         Engine::Tuning tu;
         tu.persist_blocks = t->persist_blocks;
         tu.force_kernel = t->force_kernel;
+        tu.block_threads = t->block_threads;
+        tu.stage_records = t->stage_records;
+        tu.scan_tile_cells = t->scan_tile_cells;
         e->eng.set_tuning(tu);
     });
 }
@@ -298,6 +301,9 @@ VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out) {
     out->hdr.version = VXL_SPH_ABI_VERSION;
     out->persist_blocks = tu.persist_blocks;
     out->force_kernel = tu.force_kernel;
+    out->block_threads = tu.block_threads;
+    out->stage_records = tu.stage_records;
+    out->scan_tile_cells = tu.scan_tile_cells;
     return VXL_SPH_OK;
 }
 

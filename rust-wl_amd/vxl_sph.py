@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvxl_sph.so")
 ABI_VERSION = 1
 # Force kernel variants of VxlSphTuning (vxl_sph.h VXL_SPH_FORCE_*).
-FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK, FORCE_WALK_AHEAD = 0, 1, 2, 3
+FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK, FORCE_WALK_AHEAD, FORCE_PAIRS = 0, 1, 2, 3, 4
 
 STATUS = {
     0: "OK", 1: "NULL_POINTER", 2: "VERSION_MISMATCH", 3: "SIZE_MISMATCH", 4: "NOT_READY",
@@ -66,7 +66,11 @@ class FluidConfig(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("hdr", Header), ("persist_blocks", C.c_uint32), ("force_kernel", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("block_threads", C.c_uint32), ("stage_records", C.c_uint32), ("scan_tile_cells", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+TUNING_FIELDS = ("persist_blocks", "force_kernel", "block_threads", "stage_records", "scan_tile_cells")
 
 
 class KernelTime(C.Structure):
@@ -534,28 +538,34 @@ class Engine:
         _check(lib().vxl_sph_engine_device_bytes(self.h, C.byref(v)))
         return v.value
 
-    def set_tuning(self, persist_blocks=0, force_kernel=FORCE_DEFAULT):
-        """force_kernel: FORCE_SCAN | FORCE_WALK | FORCE_WALK_AHEAD (0 = default); persist_blocks:
-        grid size cap of the per-slot kernels (0 = one block per 256 slots).
-        Every choice gives identical results."""
+    def set_tuning(self, persist_blocks=0, force_kernel=FORCE_DEFAULT, block_threads=0, stage_records=0,
+                   scan_tile_cells=0):
+        """force_kernel: FORCE_SCAN | FORCE_WALK | FORCE_WALK_AHEAD | FORCE_PAIRS; persist_blocks: grid size cap of the
+        per-slot kernels (0 = one block per 256 slots); block_threads: workgroup size of the density and
+        force kernels (64, 128, 256); stage_records: density LDS staging depth (128, 256, 512);
+        scan_tile_cells: cells per scan tile (1024, 2048, 4096). 0 keeps a field's default. Every
+        choice gives identical results."""
         t = Tuning()
         t.hdr.size = C.sizeof(Tuning)
         t.hdr.version = ABI_VERSION
         t.persist_blocks = persist_blocks
         t.force_kernel = force_kernel
+        t.block_threads = block_threads
+        t.stage_records = stage_records
+        t.scan_tile_cells = scan_tile_cells
         _check(lib().vxl_sph_engine_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, substeps=2):
         """Pick the fastest launch variant on the current pool (advances it by
-        substeps x variants substeps; results are identical for every variant)."""
+        real substeps; results are identical for every variant)."""
         t = Tuning()
         _check(lib().vxl_sph_engine_autotune(self.h, substeps, C.byref(t)))
-        return dict(persist_blocks=t.persist_blocks, force_kernel=t.force_kernel)
+        return {f: getattr(t, f) for f in TUNING_FIELDS}
 
     def get_tuning(self):
         t = Tuning()
         _check(lib().vxl_sph_engine_get_tuning(self.h, C.byref(t)))
-        return dict(persist_blocks=t.persist_blocks, force_kernel=t.force_kernel)
+        return {f: getattr(t, f) for f in TUNING_FIELDS}
 
     def profile(self, substeps):
         """{kernel name: total ms} over `substeps` eager substeps (HIP events)."""

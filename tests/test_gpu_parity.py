@@ -279,12 +279,13 @@ def test_reference_akinci_canary_on_gpu():
     np.testing.assert_array_equal(bits(gft), bits(ft))
 
 
-@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD])
+@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS])
 @pytest.mark.parametrize("persist", [0, 7])
 @pytest.mark.parametrize("budget", [1 << 21, 1 << 9])
 def test_kernel_variants_bit_exact(force_kernel, persist, budget):
-    """Every autotuner choice (force: candidate scan or walk of the recorded
-    neighbours, directly or with loads one pair ahead; one block per
+    """Every autotuner choice (force: candidate scan, walk of the recorded
+    neighbours directly or with loads one pair ahead, or the pairs of each row
+    spread over the wave's lanes; one block per
     256 slots or a grid-stride loop over a few blocks) gives the oracle's state
     bit for bit. The 2^9 budget doubles the bin: cells of ~64 particles, rows
     longer than 32 candidates (tail tests in the walk) and row unions longer than
@@ -303,13 +304,66 @@ def test_kernel_variants_bit_exact(force_kernel, persist, budget):
     assert eng.state_hash(sid) == orc.state_hash()
 
 
+@pytest.mark.parametrize("block_threads", [64, 128, 256])
+@pytest.mark.parametrize("budget", [1 << 21, 1 << 9])
+def test_tuning_knobs_bit_exact(block_threads, budget):
+    """Every value of every autotuner knob — workgroup size of the density and
+    force kernels, LDS staging depth of the density pass, cells per scan tile —
+    with every force kernel gives the oracle's acceleration, XSPH and state
+    bit for bit (at the coarse 2^9 budget unions outgrow even the 512-record
+    stage and rows run past 32 candidates)."""
+    cfg, data, tank, d = small_scene(seed=19, block=(16, 12, 10), tank=(0.6, 0.5, 0.4), max_bins=budget)
+    orc = oracle_dam_break(cfg, data, tank, d)
+    for _ in range(4):
+        orc.step(cfg.dt_tick)
+    want_acc, want_xs, want_hash = bits(orc.acc()), bits(orc.xsph()), orc.state_hash()
+    for stage in (128, 256, 512):
+        for scan in (1024, 2048, 4096):
+            for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS):
+                eng = V.Engine(diagnostics=True)
+                sid = eng.add(cfg, data, reactions=False)
+                eng.set_tuning(force_kernel=fk, block_threads=block_threads, stage_records=stage,
+                               scan_tile_cells=scan)
+                assert eng.get_tuning()["block_threads"] == block_threads
+                eng.step(4)
+                f = eng.read_fields(sid, diagnostics=True)
+                tag = f"stage {stage} scan {scan} force {fk}"
+                np.testing.assert_array_equal(bits(f["acc"]), want_acc, err_msg=tag)
+                np.testing.assert_array_equal(bits(f["xsph"]), want_xs, err_msg=tag)
+                assert eng.state_hash(sid) == want_hash, tag
+                eng.close()
+
+
+def test_autotuner_picks_allowed_values_and_keeps_results():
+    """The autotuner's choice lies in each knob's value set, and the state it
+    leaves (it tunes on real substeps) equals the oracle after as many steps."""
+    cfg, data, tank, d = small_scene(seed=21, block=(16, 12, 10), tank=(0.6, 0.5, 0.4))
+    eng = V.Engine()
+    sid = eng.add(cfg, data, reactions=False)
+    t = eng.autotune(1)
+    assert t["block_threads"] in (64, 128, 256) and t["stage_records"] in (128, 256, 512)
+    assert t["scan_tile_cells"] in (1024, 2048, 4096) and t["force_kernel"] in (1, 2, 3, 4)
+    assert eng.get_tuning() == t
+    # the tuning substeps advanced the scene; count them by replaying on the oracle
+    orc = oracle_dam_break(cfg, data, tank, d)
+    steps = 0
+    while orc.state_hash() != eng.state_hash(sid) and steps < 200:
+        orc.step(cfg.dt_tick)
+        steps += 1
+    assert orc.state_hash() == eng.state_hash(sid)  # (no steps when the choice came from the cache)
+    eng.step(3)
+    for _ in range(3):
+        orc.step(cfg.dt_tick)
+    assert eng.state_hash(sid) == orc.state_hash()
+
+
 def test_unclamped_pressure_bit_exact():
     """tensile_instability_suppression off (config.rs:16): negative pressures
     and negative c2 in the density record, every force kernel, 6 steps against
     the oracle bit for bit."""
     cfg, data, tank, d = small_scene(seed=17, block=(14, 10, 9), tank=(0.5, 0.4, 0.4))
     cfg.tensile_clamp = 0
-    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD):
+    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS):
         orc = oracle_dam_break(cfg, data, tank, d)
         eng = V.Engine(diagnostics=True)
         sid = eng.add(cfg, data, reactions=False)
@@ -339,7 +393,7 @@ def test_coincident_particles_bit_exact():
     near_b = nf + int(np.argmin(np.linalg.norm(data.pos[nf:] - pos[40], axis=1)))
     pos[40] = pos[near_b]
     data = V.SceneData(pos, data.vel, data.pmass, nf, data.spans)
-    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD):
+    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS):
         orc = oracle_dam_break(cfg, data, tank, d)
         eng = V.Engine(diagnostics=True)
         sid = eng.add(cfg, data, reactions=False)
