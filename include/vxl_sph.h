@@ -432,20 +432,28 @@ typedef struct VxlSphServeConfig {
     uint64_t max_bins;
     float dt;
     int32_t device;
+    uint64_t max_particles;  /* resident particles (fluid + boundary) per pool step at most; 0 = no limit */
+    uint32_t admit_every;    /* pool steps between admissions, batched into one layout (0 = 128) */
+    uint32_t reserved;
 } VxlSphServeConfig;
 
 typedef struct VxlSphServeStats {
     uint32_t requests, completed;
     uint64_t pool_steps, fluid_updates;
     double wall_s, updates_per_s;
-    double step_ms_p50, step_ms_p99, step_ms_mean;
-    double loop_ms_p50, loop_ms_p99;
+    double step_ms_p50, step_ms_p99, step_ms_mean; /* device time of one pool step (HIP events) */
+    double loop_ms_p50, loop_ms_p99;  /* interval between consecutive pool-step completions on the
+                                         device clock: the step plus any gap admissions, layouts or
+                                         retirements left on the device (idle pools excluded) */
     double latency_s_p50, latency_s_p99;
     double wait_s_p50, wait_s_p99;
     uint32_t peak_resident;
     uint32_t rejected;        /* requests whose scene alone exceeds budget_bytes (not served) */
     uint64_t peak_bytes;      /* highest device bytes held (never above budget_bytes) */
     uint64_t layouts;
+    double layout_ms_p50, layout_ms_p99; /* host time of one pool layout (admissions) */
+    double layout_phase_ms[5];           /* summed over the run: waiting for the device, moving resident
+                                            scenes, uploading new ones, scene tables, building the graph */
 } VxlSphServeStats;
 
 typedef struct VxlSphServeResult {

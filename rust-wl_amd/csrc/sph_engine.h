@@ -32,8 +32,9 @@ struct SceneSlot {
     std::vector<Span> spans;
     // host staging for a scene that is not on the device yet
     bool resident = false;
+    bool frozen = false;    // retired in place (no particles stepped); dropped by the next layout
     uint32_t dev_base = 0;  // slot the scene's state starts at on the device
-    std::vector<float> pos, vel, pmass;  // AoS 3n / 3n / n
+    std::vector<float4> stage_a, stage_v;  // packed records (A, V) until the layout uploads them
     uint32_t cells_hint = 0;             // grid cells of the initial box
     // reactions of the last tick (tick average), per span: F[3], tau[3]
     std::vector<float> reactions;
@@ -93,6 +94,17 @@ public:
     ~Engine();
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
+
+    // A scene built on the host (records packed, grid size measured, bodies
+    // and their lattices ready), so that adding it to the pool costs no more
+    // than moving it in; any thread may prepare scenes.
+    static SceneSlot prepare_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos,
+                                   const float *vel, const float *pmass, const Span *spans, uint32_t nspans,
+                                   bool reactions = true);
+    static SceneSlot prepare_coupled(const SceneConfig &cfg, uint32_t nf, const float *fpos, const float *fvel,
+                                     const std::vector<BoxBody> &bodies, const std::vector<float> &density,
+                                     const RigidConfig &rc);
+    uint32_t add_prepared(SceneSlot &&sc);
 
     // Scene management (takes effect at the next step or readback).
     uint32_t add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos,
@@ -174,6 +186,12 @@ public:
     const Tuning &tuning() const { return tuning_; }
     uint64_t device_bytes() const { return bytes_; }
     uint64_t layouts() const { return layouts_; }
+    // Host time of every layout so far (ms), and the summed time of its phases:
+    // 0 waiting for the device, 1 moving resident scenes (and growing the
+    // pool), 2 uploading new scenes, 3 scene tables and boxes; 4 building the
+    // step's graph afterwards.
+    const std::vector<float> &layout_ms() const { return layout_ms_; }
+    const double *layout_phase_ms() const { return layout_phase_ms_; }
     uint32_t capacity_slots() const { return slots_; }
     uint32_t used_slots() const { return used_slots_; }
     // Device bytes a scene of n particles (cells_hint grid cells) adds to the
@@ -187,21 +205,40 @@ public:
     // scenes move), including the readback staging its read_state needs.
     uint64_t projected_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions) const;
     // projected_peak for an engine that holds no scene.
-    static uint64_t solo_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions);
+    static uint64_t solo_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions,
+                              bool full_cells = false);
+    // Reserve cell tables for each scene's whole grid budget (max_bins + 1
+    // cells) instead of a multiple of its initial grid: no capacity replay can
+    // ever happen, so readbacks may be enqueued without a check.
+    void set_full_cell_reserve(bool on) { full_cells_ = on; }
+    // Retire a scene in place: it is not stepped any more (stream-ordered after
+    // the steps enqueued so far) and leaves the pool at the next layout, with
+    // no relayout now. Its state can no longer be read.
+    void freeze_scene(uint32_t id);
+    // Enqueue the readback of a scene's state (index order) into host memory
+    // that stays valid until the stream has passed this point (pinned memory
+    // for an asynchronous copy); no synchronisation.
+    void read_state_async(uint32_t id, float *pos, float *vel);
+    // Apply pending scene changes (the layout) now.
+    void prepare() {
+        if (dirty_) layout();
+        if (use_graph_ && !gexec_ && P_.nblocks) build_graph();
+    }
     // Scene-table bytes a layout allocates (bounded above).
     static uint64_t table_allowance(uint64_t nscenes, uint64_t slots, bool reactions);
     // Pool growth stays within `bytes` (0 = no limit): the x1.5 headroom is
     // dropped when it would not fit beside the old pool.
     void set_memory_limit(uint64_t bytes) { mem_limit_ = bytes; }
     uint64_t peak_bytes() const { return peak_bytes_; }
-    // Free every device array when no scene is left (the pool never shrinks
-    // otherwise).
+    // Free every device array when no scene is left but retired ones (the
+    // pool never shrinks otherwise).
     void release_if_empty();
 
 private:
     void layout();            // assign slot ranges, (re)allocate, move + upload
     uint32_t box_reduce_grid() const;
     uint32_t grown_capacity(uint32_t need, uint64_t cells, bool reactions) const;
+    uint32_t grown_cells(uint64_t cells) const;
     uint32_t pool_substeps() const;  // substeps per tick (shared by every scene of the pool)
     // The launch sequence of one substep; `mark` (if set) is called after each
     // kernel launch with the kernel's name.
@@ -278,6 +315,9 @@ private:
     uint64_t bytes_ = 0;
     uint64_t peak_bytes_ = 0;
     uint64_t mem_limit_ = 0;
+    bool full_cells_ = false;
+    std::vector<float> layout_ms_;
+    double layout_phase_ms_[5] = {0, 0, 0, 0, 0};
 };
 
 }  // namespace vxl

@@ -2,6 +2,7 @@
 #include "sph_engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -20,7 +21,8 @@ uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 // Cell-table entries to reserve for a scene: the grid of its initial box
 // with room for the box to grow (splashes raise the free surface), capped by
 // the scene's own budget.
-uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins) {
+uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins, bool full = false) {
+    if (full) return (uint32_t)std::min<uint64_t>(max_bins, (uint64_t)kMaxCells - 1u) + 1u;
     uint64_t want = std::max<uint64_t>((uint64_t)cells_hint * 4ull, 4096ull);
     want = std::min<uint64_t>(want, max_bins);
     return (uint32_t)want + 1u;  // + pad cell
@@ -129,34 +131,29 @@ void Engine::invalidate_graph() {
     graph_ = nullptr;
 }
 
-uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos, const float *vel,
-                           const float *pmass, const Span *spans, uint32_t nspans, bool reactions) {
+SceneSlot Engine::prepare_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos, const float *vel,
+                                const float *pmass, const Span *spans, uint32_t nspans, bool reactions) {
     if (nf > n) throw std::invalid_argument("n_fluid exceeds particle count");
     if (n == 0) throw std::invalid_argument("empty scene");
-    uint64_t slots = round_up(n, kBlock);
-    for (const auto &s : scenes_) slots += round_up(s.params.n, kBlock);
-    if (n > kMaxPoolSlots || slots > kMaxPoolSlots - kBlock)
-        throw CapacityError("a pool holds at most 2^27 particle slots");
+    if (n > kMaxPoolSlots - kBlock) throw CapacityError("a pool holds at most 2^27 particle slots");
     if (fabsf(cfg.gamma_tait - 7.0f) >= 1e-6f)
         throw std::domain_error("only the gamma = 7 Tait branch (fluid_force.rs:17-21) is supported");
     SceneSlot sc;
-    sc.id = next_id_++;
     sc.params = scene_params(cfg, n, nf);
     sc.substeps = std::max<uint32_t>(cfg.substeps, 1);
-    sc.pos.assign(pos, pos + 3ull * n);
-    if (vel)
-        sc.vel.assign(vel, vel + 3ull * n);
-    else
-        sc.vel.assign(3ull * n, 0.0f);
-    if (pmass)
-        sc.pmass.assign(pmass, pmass + n);
-    else
-        sc.pmass.assign(n, sc.params.mass);
-    for (float m : sc.pmass)
-        if (!(m > 0.0f) || !std::isfinite(m)) throw std::invalid_argument("particle masses must be positive and finite");
+    // Pack the records now (the layout only copies them): A = (x, y, z, m)
+    // with m = +pmass for fluid and -pmass for boundary particles, V = (v, 0).
+    sc.stage_a.resize(n);
+    sc.stage_v.resize(n);
     sc.params.fmass_uniform = 1u;
-    for (uint32_t i = 0; i < nf; ++i)
-        if (sc.pmass[i] != sc.params.mass) sc.params.fmass_uniform = 0u;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float m = pmass ? pmass[i] : sc.params.mass;
+        if (!(m > 0.0f) || !std::isfinite(m)) throw std::invalid_argument("particle masses must be positive and finite");
+        if (i < nf && m != sc.params.mass) sc.params.fmass_uniform = 0u;
+        sc.stage_a[i] = make_float4(pos[3ull * i], pos[3ull * i + 1], pos[3ull * i + 2], i < nf ? m : -m);
+        sc.stage_v[i] = vel ? make_float4(vel[3ull * i], vel[3ull * i + 1], vel[3ull * i + 2], 0.0f)
+                            : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     for (uint32_t k = 0; k < nspans; ++k) {
         if (spans[k].start < nf || spans[k].end > n || spans[k].start > spans[k].end)
             throw std::invalid_argument("span outside the boundary segment");
@@ -175,14 +172,30 @@ uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, cons
     uint32_t dims[3];
     grid_box(lo, hi, sc.params.h, sc.params.max_bins, gmin, bin, dims);
     sc.cells_hint = dims[0] * dims[1] * dims[2];
+    return sc;
+}
+
+uint32_t Engine::add_prepared(SceneSlot &&sc) {
+    uint64_t slots = round_up(sc.params.n, kBlock);
+    for (const auto &s : scenes_)
+        if (!s.frozen) slots += round_up(s.params.n, kBlock);  // retired scenes leave at the next layout
+    if (slots > kMaxPoolSlots - kBlock) throw CapacityError("a pool holds at most 2^27 particle slots");
+    sc.id = next_id_++;
+    sc.resident = false;
+    sc.frozen = false;
     scenes_.push_back(std::move(sc));
     dirty_ = true;
     return scenes_.back().id;
 }
 
-uint32_t Engine::add_coupled_scene(const SceneConfig &cfg, uint32_t nf, const float *fpos, const float *fvel,
-                                   const std::vector<BoxBody> &bodies, const std::vector<float> &density,
-                                   const RigidConfig &rc) {
+uint32_t Engine::add_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos, const float *vel,
+                           const float *pmass, const Span *spans, uint32_t nspans, bool reactions) {
+    return add_prepared(prepare_scene(cfg, n, nf, pos, vel, pmass, spans, nspans, reactions));
+}
+
+SceneSlot Engine::prepare_coupled(const SceneConfig &cfg, uint32_t nf, const float *fpos, const float *fvel,
+                                  const std::vector<BoxBody> &bodies, const std::vector<float> &density,
+                                  const RigidConfig &rc) {
     if (density.size() != bodies.size()) throw std::invalid_argument("one density per body");
     std::vector<float> pos(fpos, fpos + 3ull * nf), vel, pmass(nf, 0.0f);
     if (fvel)
@@ -236,14 +249,20 @@ uint32_t Engine::add_coupled_scene(const SceneConfig &cfg, uint32_t nf, const fl
         rigid.push_back(std::move(r));
     }
     const uint32_t n = (uint32_t)pmass.size();
-    const uint32_t id = add_scene(cfg, n, nf, pos.data(), vel.data(), pmass.data(), spans.data(),
-                                  (uint32_t)spans.size(), !rigid.empty());
+    SceneSlot sc = prepare_scene(cfg, n, nf, pos.data(), vel.data(), pmass.data(), spans.data(),
+                                 (uint32_t)spans.size(), !rigid.empty());
     // Only the dynamic bodies consume a reaction; static bodies' spans are skipped.
-    if (!rigid.empty()) scenes_.back().params.react = kReactDynamic;
-    scenes_.back().rigid = std::move(rigid);
+    if (!rigid.empty()) sc.params.react = kReactDynamic;
+    sc.rigid = std::move(rigid);
     for (size_t b = 0; b < bodies.size(); ++b)
-        if (!(density[b] > 0.0f)) scenes_.back().statics.push_back(bodies[b]);
-    return id;
+        if (!(density[b] > 0.0f)) sc.statics.push_back(bodies[b]);
+    return sc;
+}
+
+uint32_t Engine::add_coupled_scene(const SceneConfig &cfg, uint32_t nf, const float *fpos, const float *fvel,
+                                   const std::vector<BoxBody> &bodies, const std::vector<float> &density,
+                                   const RigidConfig &rc) {
+    return add_prepared(prepare_coupled(cfg, nf, fpos, fvel, bodies, density, rc));
 }
 
 std::vector<RigidDev> Engine::read_bodies(uint32_t id) {
@@ -359,12 +378,30 @@ void *Engine::staging(size_t bytes) {
 // resident scenes device-to-device (into the spare state arrays, which then
 // become current); upload new scenes; rebuild the small scene tables.
 void Engine::layout() {
+    using LClock = std::chrono::steady_clock;
+    const auto t_begin = LClock::now();
+    auto lap = [&](double &acc, LClock::time_point &t) {
+        const auto now = LClock::now();
+        acc += std::chrono::duration<double, std::milli>(now - t).count();
+        t = now;
+    };
+    auto t_phase = t_begin;
     hip_check(hipSetDevice(device_), "hipSetDevice");
     hip_check(hipStreamSynchronize(stream_), "sync before layout");
+    lap(layout_phase_ms_[0], t_phase);
     // Finish (replay) any abandoned substep of the old layout first: the reset
     // of the control block below would otherwise drop it.
     if (P_.nblocks) check_error_state();
     invalidate_graph();
+    // Retired scenes leave the pool (their readbacks completed with the sync above).
+    for (size_t k = 0; k < scenes_.size();) {
+        if (scenes_[k].frozen) {
+            free_wall_tables(scenes_[k]);
+            scenes_.erase(scenes_.begin() + (long)k);
+        } else {
+            ++k;
+        }
+    }
     // Dynamic bodies of resident scenes continue from their device state.
     if (P_.nrigid) {
         std::vector<RigidDev> cur(P_.nrigid);
@@ -393,7 +430,7 @@ void Engine::layout() {
         p.alloc = round_up(p.n, kBlock);
         used += p.alloc;
         for (uint32_t b = 0; b < p.alloc / kBlock; ++b) block_scene.push_back(k);
-        cells += cell_reserve(scenes_[k].cells_hint, p.max_bins);
+        cells += cell_reserve(scenes_[k].cells_hint, p.max_bins, full_cells_);
         reactions = reactions || scenes_[k].want_reactions;
     }
     used_slots_ = used;
@@ -418,48 +455,70 @@ void Engine::layout() {
         d_tile_sums_ = nullptr;
         alloc_cells((uint32_t)new_cells);
     } else if (cells > ccap_) {
+        const uint32_t c = grown_cells(cells);
         free_cells();
-        alloc_cells((uint32_t)std::min<uint64_t>(cells, (uint64_t)kMaxCells));
+        alloc_cells(c);
     }
-    // Move resident scenes: old position (src arrays) -> new position (dst).
+    // Move resident scenes: old position (src arrays) -> new position (dst),
+    // all of them in one launch.
     float4 *dstA = grow ? P_.cA : spare_cA_, *dstV = grow ? P_.cV : spare_cV_;
     uint32_t *dstG = grow ? P_.cg : spare_cg_;
-    for (uint32_t k = 0; k < scenes_.size(); ++k) {
-        SceneSlot &sc = scenes_[k];
-        if (!sc.resident) continue;
-        const size_t n = sc.params.n, from = old_base[k], to = sc.params.pbase;
-        hip_check(hipMemcpyAsync(dstA + to, srcA + from, 16 * n, hipMemcpyDeviceToDevice, stream_), "move A");
-        hip_check(hipMemcpyAsync(dstV + to, srcV + from, 16 * n, hipMemcpyDeviceToDevice, stream_), "move V");
-        hip_check(hipMemcpyAsync(dstG + to, srcG + from, 4 * n, hipMemcpyDeviceToDevice, stream_), "move gid");
+    {
+        std::vector<uint32_t> moves;  // (from, to, first index of this move) per resident scene
+        uint32_t total_moved = 0;
+        for (uint32_t k = 0; k < scenes_.size(); ++k) {
+            const SceneSlot &sc = scenes_[k];
+            if (!sc.resident) continue;
+            moves.push_back(old_base[k]);
+            moves.push_back(sc.params.pbase);
+            moves.push_back(total_moved);
+            total_moved += sc.params.n;
+        }
+        if (total_moved) {
+            uint32_t *d_moves = dalloc<uint32_t>(moves.size() + 3);
+            moves.push_back(0);
+            moves.push_back(0);
+            moves.push_back(total_moved);  // sentinel: the end of the last move
+            hip_check(hipMemcpy(d_moves, moves.data(), 4 * moves.size(), hipMemcpyHostToDevice), "moves");
+            const uint32_t nmoves = (uint32_t)(moves.size() / 3 - 1);
+            const uint32_t grid = std::min<uint32_t>((total_moved + kBlock - 1) / kBlock, 16384u);
+            hipLaunchKernelGGL(k_move_records, dim3(grid), dim3(kBlock), 0, stream_, (const float4 *)srcA,
+                               (const float4 *)srcV, (const uint32_t *)srcG, dstA, dstV, dstG,
+                               (const uint32_t *)d_moves, nmoves, total_moved);
+            hip_check(hipGetLastError(), "k_move_records");
+            hip_check(hipStreamSynchronize(stream_), "move sync");
+            dfree(d_moves);
+        }
     }
-    hip_check(hipStreamSynchronize(stream_), "move sync");
+    lap(layout_phase_ms_[1], t_phase);
     if (!grow) {
         std::swap(P_.cA, spare_cA_);
         std::swap(P_.cV, spare_cV_);
         std::swap(P_.cg, spare_cg_);
     }
     for (void *p : old_allocs) dfree(p);
-    // New scenes: index order; the first grid build sorts them.
+    // New scenes: index order (records packed when the scene was added); the
+    // first grid build sorts them.
     for (auto &sc : scenes_) {
         if (sc.resident) continue;
-        const uint32_t n = sc.params.n, nf = sc.params.nf;
-        // A = (x, y, z, m) with m = +pmass (fluid) / -pmass (boundary); V = (v, 0).
-        std::vector<float4> a(n), v(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            const float m = sc.pmass[i];
-            a[i] = make_float4(sc.pos[3 * i], sc.pos[3 * i + 1], sc.pos[3 * i + 2], i < nf ? m : -m);
-            v[i] = make_float4(sc.vel[3 * i], sc.vel[3 * i + 1], sc.vel[3 * i + 2], 0.0f);
-        }
-        hip_check(hipMemcpy(P_.cA + sc.params.pbase, a.data(), 16ull * n, hipMemcpyHostToDevice), "A");
-        hip_check(hipMemcpy(P_.cV + sc.params.pbase, v.data(), 16ull * n, hipMemcpyHostToDevice), "V");
-        std::vector<uint32_t> g(n);
-        for (uint32_t i = 0; i < n; ++i) g[i] = i;
-        hip_check(hipMemcpy(P_.cg + sc.params.pbase, g.data(), 4ull * n, hipMemcpyHostToDevice), "gid");
+        const uint32_t n = sc.params.n;
+        hip_check(hipMemcpyAsync(P_.cA + sc.params.pbase, sc.stage_a.data(), 16ull * n, hipMemcpyHostToDevice, stream_),
+                  "A");
+        hip_check(hipMemcpyAsync(P_.cV + sc.params.pbase, sc.stage_v.data(), 16ull * n, hipMemcpyHostToDevice, stream_),
+                  "V");
+        hipLaunchKernelGGL(k_iota, dim3(std::min<uint32_t>((n + kBlock - 1) / kBlock, 4096u)), dim3(kBlock), 0, stream_,
+                           P_.cg + sc.params.pbase, n);
+        hip_check(hipGetLastError(), "k_iota");
+    }
+    hip_check(hipStreamSynchronize(stream_), "upload sync");
+    for (auto &sc : scenes_) {
+        if (sc.resident) continue;
         sc.resident = true;
-        sc.pos.clear(); sc.pos.shrink_to_fit();
-        sc.vel.clear(); sc.vel.shrink_to_fit();
+        std::vector<float4>().swap(sc.stage_a);
+        std::vector<float4>().swap(sc.stage_v);
     }
     for (auto &sc : scenes_) sc.dev_base = sc.params.pbase;
+    lap(layout_phase_ms_[2], t_phase);
     // Scene tables.
     dfree(d_params_);
     dfree(d_grid_);
@@ -605,6 +664,8 @@ void Engine::layout() {
             index_wall_table(k, 1);
         }
     hip_check(hipStreamSynchronize(stream_), "layout sync");
+    lap(layout_phase_ms_[3], t_phase);
+    layout_ms_.push_back((float)std::chrono::duration<double, std::milli>(LClock::now() - t_begin).count());
     substeps_done_ = 0;  // substep index of the new layout (tick phase, replay)
     ++layouts_;
     dirty_ = false;
@@ -622,13 +683,26 @@ uint32_t Engine::grown_capacity(uint32_t need, uint64_t cells, bool reactions) c
     return cap;
 }
 
+uint32_t Engine::grown_cells(uint64_t cells) const {
+    // x1.5 headroom (fewer reallocations as scenes arrive) unless it would not
+    // fit under the memory limit.
+    const uint64_t exact = std::min<uint64_t>(cells, (uint64_t)kMaxCells);
+    const uint64_t grown = std::min<uint64_t>(std::max<uint64_t>(cells, (uint64_t)ccap_ + ccap_ / 2), (uint64_t)kMaxCells);
+    if (mem_limit_ && grown > exact) {
+        const uint64_t cells_now = ccap_ ? 12ull * ccap_ : 0ull;
+        if (bytes_ - cells_now + 12ull * grown > mem_limit_) return (uint32_t)exact;
+    }
+    return (uint32_t)grown;
+}
+
 uint64_t Engine::projected_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions) const {
-    uint64_t need = round_up(n, kBlock), cells = cell_reserve(cells_hint, max_bins);
+    uint64_t need = round_up(n, kBlock), cells = cell_reserve(cells_hint, max_bins, full_cells_);
     bool react = reactions || want_reaction_buffers_;
     uint64_t maxn = n, nscenes = 1;
     for (const auto &sc : scenes_) {
+        if (sc.frozen) continue;  // dropped by the next layout
         need += round_up(sc.params.n, kBlock);
-        cells += cell_reserve(sc.cells_hint, sc.params.max_bins);
+        cells += cell_reserve(sc.cells_hint, sc.params.max_bins, full_cells_);
         react = react || sc.want_reactions;
         maxn = std::max<uint64_t>(maxn, sc.params.n);
         ++nscenes;
@@ -651,8 +725,8 @@ uint64_t Engine::projected_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bi
         return base + pool_bytes(cap, new_cells, diagnostics_, react) + stage + tables;
     }
     if (cells > ccap_) {
-        const uint64_t c = std::min<uint64_t>(cells, (uint64_t)kMaxCells);
-        return base - cells_now + 12ull * c + 4ull * ((c + 2047) / 2048 + 1) + 4ull + stage + tables;
+        const uint64_t c = grown_cells(cells);
+        return base - cells_now + 12ull * c + 4ull * ((c + kScanMinTile - 1) / kScanMinTile + 1) + 4ull + stage + tables;
     }
     return base + stage + tables;
 }
@@ -661,17 +735,21 @@ uint64_t Engine::table_allowance(uint64_t nscenes, uint64_t slots, bool reaction
     return 256ull * nscenes + slots / 32ull + 4096ull + (reactions ? 16ull * slots + 4096ull : 0ull);
 }
 
-uint64_t Engine::solo_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions) {
+uint64_t Engine::solo_peak(uint32_t n, uint32_t cells_hint, uint64_t max_bins, bool reactions, bool full_cells) {
     // projected_peak of an engine holding nothing but its control block
     const uint64_t need = round_up(n, kBlock);
-    const uint64_t cells =
-        std::min<uint64_t>(std::max<uint64_t>(cell_reserve(cells_hint, max_bins), 4096ull), (uint64_t)kMaxCells);
+    const uint64_t cells = std::min<uint64_t>(std::max<uint64_t>(cell_reserve(cells_hint, max_bins, full_cells), 4096ull),
+                                              (uint64_t)kMaxCells);
     return sizeof(Control) + pool_bytes(need, cells, false, reactions) + 24ull * n +
            table_allowance(1, need, reactions);
 }
 
 void Engine::release_if_empty() {
-    if (!scenes_.empty()) return;
+    for (const auto &sc : scenes_)
+        if (!sc.frozen) return;
+    hip_check(hipStreamSynchronize(stream_), "sync before release");
+    for (auto &sc : scenes_) free_wall_tables(sc);
+    scenes_.clear();
     hip_check(hipStreamSynchronize(stream_), "sync before release");
     invalidate_graph();
     free_all();
@@ -1174,11 +1252,13 @@ void Engine::set_tuning(const Tuning &t) {
 
 // One whole tick (rigid leg, substeps) as a graph; launched at tick starts.
 void Engine::build_graph() {
+    const auto t0 = std::chrono::steady_clock::now();
     invalidate_graph();
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
     for (uint32_t i = 0; i < pool_substeps(); ++i) enqueue_at(i);
     hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
     hip_check(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+    layout_phase_ms_[4] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 void Engine::step(uint32_t ticks) {
@@ -1302,6 +1382,40 @@ void Engine::set_span_origins(uint32_t id, const Span *spans, uint32_t nspans) {
               "spans");
 }
 
+void Engine::freeze_scene(uint32_t id) {
+    if (dirty_) layout();
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    if (scenes_[k].frozen) return;
+    scenes_[k].frozen = true;
+    // Stream-ordered after every step enqueued so far; the captured tick reads
+    // the parameters at run time, so it stays valid.
+    hipLaunchKernelGGL(k_freeze, dim3(1), dim3(64), 0, stream_, d_params_, k);
+    hip_check(hipGetLastError(), "k_freeze");
+}
+
+void Engine::read_state_async(uint32_t id, float *pos, float *vel) {
+    if (dirty_) layout();
+    uint32_t k = 0;
+    for (; k < scenes_.size(); ++k)
+        if (scenes_[k].id == id) break;
+    if (k == scenes_.size()) throw std::out_of_range("unknown scene");
+    const SceneSlot &sc = scenes_[k];
+    if (sc.frozen) throw std::logic_error("the scene is retired");
+    const uint32_t n = sc.params.n;
+    // Stream order keeps one staging buffer safe for consecutive readbacks;
+    // growing it frees the old one, which waits for the device.
+    float *op = (float *)staging(24ull * n), *ov = op + 3ull * n;
+    const uint32_t nblk = sc.params.alloc / kBlock;
+    hipLaunchKernelGGL(k_unsort, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(nblk, 8192u))), dim3(kBlock), 0,
+                       stream_, P_, k, op, ov);
+    hip_check(hipGetLastError(), "k_unsort");
+    hip_check(hipMemcpyAsync(pos, op, 12ull * n, hipMemcpyDeviceToHost, stream_), "pos");
+    hip_check(hipMemcpyAsync(vel, ov, 12ull * n, hipMemcpyDeviceToHost, stream_), "vel");
+}
+
 void Engine::read_state(uint32_t id, float *pos, float *vel) {
     sync();
     uint32_t k = 0;
@@ -1309,6 +1423,7 @@ void Engine::read_state(uint32_t id, float *pos, float *vel) {
         if (scenes_[k].id == id) break;
     if (k == scenes_.size()) throw std::out_of_range("unknown scene");
     const SceneSlot &sc = scenes_[k];
+    if (sc.frozen) throw std::logic_error("the scene is retired");
     const uint32_t n = sc.params.n;
     float *op = (float *)staging(24ull * n), *ov = op + 3ull * n;
     const uint32_t nblk = sc.params.alloc / kBlock;

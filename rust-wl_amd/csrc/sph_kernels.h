@@ -138,6 +138,10 @@ __global__ void k_rigid(Pool P);
 __global__ void k_body_refresh(Pool P);
 __global__ void k_rigid_collect(Pool P);
 __global__ void k_unsort(Pool P, uint32_t s, float *opos, float *ovel);  // scene s, index order
+__global__ void k_freeze(SceneParams *params, uint32_t s);               // scene s: no particles from now on
+__global__ void k_move_records(const float4 *sA, const float4 *sV, const uint32_t *sG, float4 *dA, float4 *dV,
+                               uint32_t *dG, const uint32_t *moves, uint32_t nmoves, uint32_t total);
+__global__ void k_iota(uint32_t *p, uint32_t n);
 __global__ void k_unsort_fields(Pool P, float *odens, float *opress, uint32_t *oncount, float *oacc, float *oxs);
 
 }  // namespace vxl

@@ -264,6 +264,9 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P) {
                 nx->lo[a] = 0xffffffffu;
                 nx->hi[a] = 0u;
             }
+            // A retired scene (no particles left to step) keeps one cell.
+            if (sp.n == 0u)
+                for (int a = 0; a < 3; ++a) lo[a] = hi[a] = 0.0f;
             float bin = fmaxf(sp.h, 1e-6f);
             const float ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
             uint32_t dims[3] = {1, 1, 1};
@@ -1489,6 +1492,48 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
     if (sub > 1 && phase == sub - 1) {
         const float inv = 1.0f / (float)sub;
         for (int a = 0; a < 6; ++a) o[a] = o[a] * inv;
+    }
+}
+
+// Resident scenes move to their new slots at a layout: move k copies the
+// records of slots [from_k, from_k + n_k) to [to_k, to_k + n_k); moves are
+// listed as (from, to, first) with first the running count, and a sentinel
+// entry whose first is the total.
+__global__ __launch_bounds__(kBlock) void k_move_records(const float4 *__restrict__ sA, const float4 *__restrict__ sV,
+                                                         const uint32_t *__restrict__ sG, float4 *dA, float4 *dV,
+                                                         uint32_t *dG, const uint32_t *__restrict__ moves,
+                                                         uint32_t nmoves, uint32_t total) {
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        uint32_t lo = 0, hi = nmoves;  // the move whose range holds t
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (moves[3u * mid + 2u] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint32_t k = t - moves[3u * lo + 2u];
+        const uint32_t from = moves[3u * lo] + k, to = moves[3u * lo + 1u] + k;
+        dA[to] = sA[from];
+        dV[to] = sV[from];
+        dG[to] = sG[from];
+    }
+}
+
+// p[k] = k for k < n (original indices of a new scene).
+__global__ __launch_bounds__(kBlock) void k_iota(uint32_t *p, uint32_t n) {
+    for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) p[k] = k;
+}
+
+// Retire scene s in place: it keeps its slots and cells until the next
+// layout, but every kernel now treats it as empty (no particle, no fluid, no
+// reactions, no bodies), so it costs nothing and changes nothing.
+__global__ void k_freeze(SceneParams *params, uint32_t s) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        params[s].n = 0u;
+        params[s].nf = 0u;
+        params[s].react = 0u;
+        params[s].nrbody = 0u;
     }
 }
 

@@ -26,7 +26,7 @@ double pct(std::vector<double> v, double q) {
     return v[lo] + (v[hi] - v[lo]) * (r - (double)lo);
 }
 
-uint64_t fnv_fluid(const std::vector<float> &pos, const std::vector<float> &vel, uint32_t nf) {
+uint64_t fnv_fluid(const float *pos, const float *vel, uint32_t nf) {
     uint64_t h = 0xcbf29ce484222325ull;
     auto mix = [&](float f) {
         uint32_t u;
@@ -41,17 +41,13 @@ uint64_t fnv_fluid(const std::vector<float> &pos, const std::vector<float> &vel,
     return h;
 }
 
+// A request's scene, prepared by a builder thread (records packed, bodies
+// and lattices built), so that admitting it only moves it into the pool.
 struct Built {
-    uint32_t index;
-    SceneConfig sc;
-    std::vector<float> pos, vel, pmass;
-    std::vector<Span> spans;
-    uint32_t nf = 0;
-    uint32_t cells_hint = 0;
-    // coupled requests (dynamic boxes)
-    std::vector<BoxBody> bodies;
-    std::vector<float> density;
-    RigidConfig rc;
+    uint32_t index = 0;
+    SceneSlot scene;
+    uint32_t n = 0, nf = 0;
+    bool coupled = false;
 };
 
 }  // namespace
@@ -118,7 +114,7 @@ uint64_t request_solo_bytes(const SceneSample &req, const ServeConfig &cfg) {
     build_request(req, cfg, sc, pos, vel, pmass, spans, nf, &bodies, &density, &rc);
     const uint32_t n = (uint32_t)pmass.size();
     return Engine::solo_peak(n, scene_cells_hint(pos, n, sc.smoothing_radius, sc.max_bins), sc.max_bins,
-                             !bodies.empty());
+                             !bodies.empty(), true);
 }
 
 ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeConfig &cfg,
@@ -156,9 +152,28 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
             Built b;
             b.index = order[pos_k];
             try {
-                build_request(reqs[b.index], cfg, b.sc, b.pos, b.vel, b.pmass, b.spans, b.nf, &b.bodies, &b.density,
-                              &b.rc);
-                b.cells_hint = scene_cells_hint(b.pos, b.pmass.size(), b.sc.smoothing_radius, b.sc.max_bins);
+                const SceneSample &rq = reqs[b.index];
+                SceneConfig sc;
+                std::vector<float> pos, vel, pmass;
+                std::vector<Span> spans;
+                if (rq.nboxes) {
+                    // fluid only here: prepare_coupled samples the bodies
+                    SceneSample fluid_only = rq;
+                    fluid_only.nboxes = 0;
+                    fluid_only.dam.walls = false;
+                    build_request(fluid_only, cfg, sc, pos, vel, pmass, spans, b.nf);
+                    std::vector<BoxBody> bodies;
+                    std::vector<float> density;
+                    RigidConfig rc;
+                    request_bodies(rq, bodies, density, rc);
+                    b.scene = Engine::prepare_coupled(sc, b.nf, pos.data(), vel.data(), bodies, density, rc);
+                    b.coupled = true;
+                } else {
+                    build_request(rq, cfg, sc, pos, vel, pmass, spans, b.nf);
+                    b.scene = Engine::prepare_scene(sc, (uint32_t)pmass.size(), b.nf, pos.data(), vel.data(),
+                                                    pmass.data(), spans.data(), (uint32_t)spans.size(), false);
+                }
+                b.n = b.scene.params.n;
             } catch (const std::exception &e) {
                 std::lock_guard<std::mutex> lk(mu);
                 build_error = e.what();
@@ -187,24 +202,168 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
     struct Resident {
         uint32_t index, scene, steps_left, nf, n;
     };
+    // A pool step in flight: device events around its tick.
+    struct Step {
+        hipEvent_t e0, e1;
+    };
+    // A retired request whose state is on its way to pinned host memory.
+    struct Pinned {
+        float *p = nullptr;
+        size_t floats = 0;
+    };
+    struct Retire {
+        uint32_t index, nf;
+        Pinned buf;
+        hipEvent_t done;
+    };
     std::vector<Resident> resident;
-    size_t next_admit = 0;  // position in `order`
+    std::deque<Step> steps;
+    std::deque<Retire> retiring;
+    std::vector<hipEvent_t> spare_events;
+    std::vector<Pinned> spare_pinned;
+    hipEvent_t last_end = nullptr;  // end event of the previous completed step (interval clock)
+    size_t next_admit = 0;          // position in `order`
+    uint32_t steps_since_layout = 0;
+    uint64_t resident_particles = 0;
     std::vector<double> step_ms, loop_ms, lat, wait;
     std::string error;
+    auto ok = [](hipError_t e, const char *what) {
+        if (e != hipSuccess) throw DeviceError(e, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    auto event = [&]() {
+        if (!spare_events.empty()) {
+            hipEvent_t e = spare_events.back();
+            spare_events.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        ok(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    };
+    auto pinned = [&](size_t floats) {
+        for (size_t k = 0; k < spare_pinned.size(); ++k)
+            if (spare_pinned[k].floats >= floats) {
+                Pinned b = spare_pinned[k];
+                spare_pinned.erase(spare_pinned.begin() + (long)k);
+                return b;
+            }
+        Pinned b;
+        b.floats = floats + floats / 2;
+        ok(hipHostMalloc((void **)&b.p, 4 * b.floats, hipHostMallocDefault), "hipHostMalloc");
+        return b;
+    };
+    // Completed steps give their device time and the interval since the
+    // previous step's end (the step plus any gap the loop left on the device).
+    auto harvest_steps = [&](bool wait_oldest) {
+        while (!steps.empty()) {
+            Step &q = steps.front();
+            if (wait_oldest) {
+                ok(hipEventSynchronize(q.e1), "hipEventSynchronize");
+                wait_oldest = false;
+            } else if (hipEventQuery(q.e1) != hipSuccess) {
+                break;
+            }
+            float ms = 0.0f;
+            ok(hipEventElapsedTime(&ms, q.e0, q.e1), "hipEventElapsedTime");
+            step_ms.push_back(ms);
+            if (last_end) {
+                float iv = 0.0f;
+                ok(hipEventElapsedTime(&iv, last_end, q.e1), "hipEventElapsedTime");
+                loop_ms.push_back(iv);
+                spare_events.push_back(last_end);
+            }
+            last_end = q.e1;
+            spare_events.push_back(q.e0);
+            steps.pop_front();
+        }
+    };
+    // A worker hashes the state of retired requests off the loop thread (a
+    // large request's hash takes milliseconds, which would leave the device
+    // idle); the pinned buffer comes back to the loop afterwards.
+    std::mutex hmu;
+    std::condition_variable hcv;
+    std::deque<Retire> to_hash;
+    std::vector<Pinned> hashed_bufs;
+    uint32_t hashed = 0, handed = 0;  // states hashed / handed to the worker
+    bool hstop = false;
+    std::thread hasher([&] {
+        for (;;) {
+            Retire r;
+            {
+                std::unique_lock<std::mutex> lk(hmu);
+                hcv.wait(lk, [&] { return hstop || !to_hash.empty(); });
+                if (to_hash.empty()) return;
+                r = to_hash.front();
+                to_hash.pop_front();
+            }
+            ServeResult &res = results[r.index];
+            res.state_hash = fnv_fluid(r.buf.p, r.buf.p + 3ull * res.n, r.nf);
+            res.finish_s = sched_now();
+            std::lock_guard<std::mutex> lk(hmu);
+            lat.push_back(res.finish_s - res.arrival_s);
+            hashed_bufs.push_back(r.buf);
+            ++hashed;
+        }
+    });
+    auto harvest_retired = [&](bool all) {
+        while (!retiring.empty()) {
+            Retire &r = retiring.front();
+            if (all)
+                ok(hipEventSynchronize(r.done), "hipEventSynchronize");
+            else if (hipEventQuery(r.done) != hipSuccess)
+                break;
+            spare_events.push_back(r.done);
+            {
+                std::lock_guard<std::mutex> lk(hmu);
+                to_hash.push_back(r);
+                ++handed;
+            }
+            hcv.notify_one();
+            retiring.pop_front();
+        }
+        std::lock_guard<std::mutex> lk(hmu);
+        st.completed = hashed;
+        for (const Pinned &b : hashed_bufs) spare_pinned.push_back(b);
+        hashed_bufs.clear();
+    };
+    auto wait_hashed = [&] {
+        for (;;) {
+            {
+                std::lock_guard<std::mutex> lk(hmu);
+                if (retiring.empty() && hashed == handed) {
+                    st.completed = hashed;
+                    for (const Pinned &b : hashed_bufs) spare_pinned.push_back(b);
+                    hashed_bufs.clear();
+                    return;
+                }
+            }
+            harvest_retired(true);
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+        }
+    };
     // Pool growth stays inside the budget; admission checks the engine's
     // projected peak (old and new pool alive together while scenes move,
     // readback staging included), so device bytes never exceed the budget.
+    // Every scene reserves cells for its whole grid budget, so no substep is
+    // ever abandoned for capacity and readbacks need no check.
     eng.set_memory_limit(cfg.budget_bytes);
+    eng.set_full_cell_reserve(true);
+    constexpr uint32_t kInFlight = 4;  // pool steps enqueued ahead of the device
+    const uint32_t layout_every = std::max<uint32_t>(cfg.admit_every, 1u);
     try {
-        while (st.completed + st.rejected < st.requests) {
+        while (next_admit < order.size() || !resident.empty()) {
             if (now_s() > cfg.max_wall_s) break;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 if (!build_error.empty()) throw std::runtime_error("scene build failed: " + build_error);
             }
-            const auto loop_t0 = Clock::now();
-            // Admit built requests in arrival order while the budget allows.
-            for (;;) {
+            harvest_steps(false);
+            harvest_retired(false);
+            // Admit built requests in arrival order while the budget allows,
+            // at most once every admit_every pool steps (one layout for all).
+            const bool may_admit = resident.empty() || steps_since_layout >= layout_every;
+            bool admitted = false;
+            while (may_admit) {
                 if (next_admit >= order.size()) break;
                 if (cfg.max_resident && resident.size() >= cfg.max_resident) break;
                 Built b;
@@ -215,11 +374,17 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
                     b = std::move(it->second);
                     ready.erase(it);
                 }
-                const uint32_t n = (uint32_t)b.pmass.size();
-                if (resident.empty()) eng.release_if_empty();
-                const uint64_t peak = eng.projected_peak(n, b.cells_hint, cfg.max_bins, !b.bodies.empty());
+                const uint32_t n = b.n;
+                if (cfg.max_particles && !resident.empty() && resident_particles + n > cfg.max_particles) {
+                    // the pool step is as large as allowed: wait for retirements
+                    std::lock_guard<std::mutex> lk(mu);
+                    ready.emplace((uint32_t)next_admit, std::move(b));
+                    break;
+                }
+                if (resident.empty() && steps.empty() && retiring.empty()) eng.release_if_empty();
+                const uint64_t peak = eng.projected_peak(n, b.scene.cells_hint, cfg.max_bins, b.coupled);
                 if (peak > cfg.budget_bytes) {
-                    if (!resident.empty()) {
+                    if (!resident.empty() || !retiring.empty()) {
                         // wait for retirements; keep the built scene for the next try
                         std::lock_guard<std::mutex> lk(mu);
                         ready.emplace((uint32_t)next_admit, std::move(b));
@@ -237,13 +402,10 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
                     ++next_admit;
                     continue;
                 }
-                const uint32_t sid =
-                    b.bodies.empty()
-                        ? eng.add_scene(b.sc, n, b.nf, b.pos.data(), b.vel.data(), b.pmass.data(), b.spans.data(),
-                                        (uint32_t)b.spans.size(), false)
-                        : eng.add_coupled_scene(b.sc, b.nf, b.pos.data(), b.vel.data(), b.bodies, b.density, b.rc);
+                const uint32_t sid = eng.add_prepared(std::move(b.scene));
                 const SceneSample &rq = reqs[b.index];
                 resident.push_back(Resident{b.index, sid, std::max<uint32_t>(rq.steps, 1), b.nf, n});
+                resident_particles += n;
                 ServeResult &res = results[b.index];
                 res.index = b.index;
                 res.n = n;
@@ -253,40 +415,67 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
                 res.admit_s = sched_now();
                 wait.push_back(res.admit_s - rq.arrival_s);
                 ++next_admit;
+                admitted = true;
             }
             if (resident.empty()) {
                 std::this_thread::sleep_for(std::chrono::microseconds(200));
                 continue;
             }
+            if (admitted) {
+                eng.prepare();  // one layout for every admission and retirement since the last
+                steps_since_layout = 0;
+            }
             st.peak_resident = std::max<uint32_t>(st.peak_resident, (uint32_t)resident.size());
-            // One pool step for every resident scene.
-            const std::vector<float> ms = eng.step_timed(1);
-            step_ms.push_back(ms.empty() ? 0.0 : ms[0]);
+            // One pool step for every resident scene, enqueued with its events;
+            // the host stays at most kInFlight steps ahead of the device.
+            if (steps.size() >= kInFlight) harvest_steps(true);
+            Step q{event(), event()};
+            ok(hipEventRecord(q.e0, eng.stream()), "hipEventRecord");
+            eng.step(1);
+            ok(hipEventRecord(q.e1, eng.stream()), "hipEventRecord");
+            steps.push_back(q);
             ++st.pool_steps;
-            // Retire finished scenes.
+            ++steps_since_layout;
+            // Requests that just had their last step: read back asynchronously
+            // and retired in place (no relayout until the next admission).
             for (size_t k = 0; k < resident.size();) {
                 Resident &r = resident[k];
                 st.fluid_updates += r.nf;
                 if (--r.steps_left == 0) {
-                    std::vector<float> p(3ull * r.n), v(3ull * r.n);
-                    eng.read_state(r.scene, p.data(), v.data());
-                    ServeResult &res = results[r.index];
-                    res.state_hash = fnv_fluid(p, v, r.nf);
-                    res.finish_s = sched_now();
-                    lat.push_back(res.finish_s - res.arrival_s);
-                    eng.remove_scene(r.scene);
-                    ++st.completed;
+                    Retire t{r.index, r.nf, pinned(6ull * r.n), event()};
+                    eng.read_state_async(r.scene, t.buf.p, t.buf.p + 3ull * r.n);
+                    ok(hipEventRecord(t.done, eng.stream()), "hipEventRecord");
+                    eng.freeze_scene(r.scene);
+                    resident_particles -= r.n;
+                    retiring.push_back(t);
                     resident[k] = resident.back();
                     resident.pop_back();
                 } else {
                     ++k;
                 }
             }
-            loop_ms.push_back(std::chrono::duration<double, std::milli>(Clock::now() - loop_t0).count());
+            if (resident.empty()) {
+                // Nothing left to step: let the readbacks finish, and do not
+                // count the idle time before the next step as an interval.
+                ok(hipStreamSynchronize(eng.stream()), "drain");
+                harvest_steps(false);
+                harvest_retired(true);
+                if (last_end) spare_events.push_back(last_end);
+                last_end = nullptr;
+            }
         }
+        ok(hipStreamSynchronize(eng.stream()), "drain");
+        harvest_steps(false);
+        wait_hashed();
     } catch (const std::exception &e) {
         error = e.what();
     }
+    {
+        std::lock_guard<std::mutex> lk(hmu);
+        hstop = true;
+    }
+    hcv.notify_all();
+    hasher.join();
     {
         std::lock_guard<std::mutex> lk(mu);
         stop = true;
@@ -294,6 +483,18 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
     cv.notify_all();
     for (auto &t : pool) t.join();
     ticker.join();
+    (void)hipStreamSynchronize(eng.stream());
+    for (auto &q : steps) {
+        (void)hipEventDestroy(q.e0);
+        (void)hipEventDestroy(q.e1);
+    }
+    for (auto &r : retiring) {
+        (void)hipEventDestroy(r.done);
+        (void)hipHostFree(r.buf.p);
+    }
+    for (auto e : spare_events) (void)hipEventDestroy(e);
+    if (last_end) (void)hipEventDestroy(last_end);
+    for (auto &b : spare_pinned) (void)hipHostFree(b.p);
     for (auto &r : resident) eng.remove_scene(r.scene);
     if (!error.empty()) throw std::runtime_error(error);
 
@@ -312,6 +513,12 @@ ServeStats serve(Engine &eng, const std::vector<SceneSample> &reqs, const ServeC
     st.wait_s_p99 = pct(wait, 0.99);
     st.layouts = eng.layouts();
     st.peak_bytes = eng.peak_bytes();
+    {
+        std::vector<double> lm(eng.layout_ms().begin(), eng.layout_ms().end());
+        st.layout_ms_p50 = pct(lm, 0.50);
+        st.layout_ms_p99 = pct(lm, 0.99);
+        for (int k = 0; k < 5; ++k) st.layout_phase_ms[k] = eng.layout_phase_ms()[k];
+    }
     return st;
 }
 

@@ -20,6 +20,8 @@ struct ServeConfig {
     double time_scale = 1.0;               // arrival schedule speed-up (1 = real time)
     double max_wall_s = 3600.0;            // stop admitting/stepping after this
     uint32_t max_resident = 0;             // 0 = limited by the budget only
+    uint64_t max_particles = 0;            // resident particles (fluid + boundary) at most; 0 = no limit
+    uint32_t admit_every = 128;            // pool steps between admissions (one layout for all of them)
     uint32_t builders = 4;                 // host threads building arrived scenes
     uint64_t max_bins = 1ull << 21;        // grid budget of every scene
     float dt = 5e-4f;
@@ -49,6 +51,8 @@ struct ServeStats {
     uint32_t rejected = 0;    // requests larger than the budget on their own (not served)
     uint64_t peak_bytes = 0;  // highest device bytes the engine held
     uint64_t layouts = 0;
+    double layout_ms_p50 = 0.0, layout_ms_p99 = 0.0;  // host time of a layout (admissions)
+    double layout_phase_ms[5] = {0, 0, 0, 0, 0};      // summed: wait, move, upload, tables, graph
 };
 
 // Builds the scene of one request: the dam-break fluid and, with its tank,

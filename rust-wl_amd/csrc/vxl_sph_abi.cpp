@@ -1073,6 +1073,8 @@ VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSp
         sc.builders = cfg->builders ? cfg->builders : 4;
         sc.max_bins = cfg->max_bins ? cfg->max_bins : (1ull << 21);
         sc.dt = cfg->dt > 0 ? cfg->dt : 5e-4f;
+        sc.max_particles = cfg->max_particles;
+        sc.admit_every = cfg->admit_every ? cfg->admit_every : 128u;
         std::vector<SceneSample> v(n);
         for (uint32_t k = 0; k < n; ++k) {
             v[k] = from_request(reqs[k]);
@@ -1101,6 +1103,9 @@ VxlSphStatus vxl_sph_serve(const VxlSphServeConfig *cfg, uint32_t n, const VxlSp
         rejected = s.rejected;
         stats->peak_bytes = s.peak_bytes;
         stats->layouts = s.layouts;
+        stats->layout_ms_p50 = s.layout_ms_p50;
+        stats->layout_ms_p99 = s.layout_ms_p99;
+        for (int k = 0; k < 5; ++k) stats->layout_phase_ms[k] = s.layout_phase_ms[k];
         if (results)
             for (uint32_t k = 0; k < n; ++k) {
                 results[k].index = res[k].index;

@@ -126,7 +126,8 @@ class Request(C.Structure):
 class ServeConfig(C.Structure):
     _fields_ = [("hdr", Header), ("budget_bytes", C.c_uint64), ("time_scale", C.c_double),
                 ("max_wall_s", C.c_double), ("max_resident", C.c_uint32), ("builders", C.c_uint32),
-                ("max_bins", C.c_uint64), ("dt", C.c_float), ("device", C.c_int32)]
+                ("max_bins", C.c_uint64), ("dt", C.c_float), ("device", C.c_int32), ("max_particles", C.c_uint64),
+                ("admit_every", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ServeStats(C.Structure):
@@ -136,7 +137,8 @@ class ServeStats(C.Structure):
                 ("loop_ms_p50", C.c_double), ("loop_ms_p99", C.c_double), ("latency_s_p50", C.c_double),
                 ("latency_s_p99", C.c_double), ("wait_s_p50", C.c_double), ("wait_s_p99", C.c_double),
                 ("peak_resident", C.c_uint32), ("rejected", C.c_uint32), ("peak_bytes", C.c_uint64),
-                ("layouts", C.c_uint64)]
+                ("layouts", C.c_uint64), ("layout_ms_p50", C.c_double), ("layout_ms_p99", C.c_double),
+                ("layout_phase_ms", C.c_double * 5)]
 
 
 class ServeResult(C.Structure):
@@ -1019,7 +1021,7 @@ def pool_limits():
 
 
 def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_resident=0, builders=4,
-          max_bins=1 << 21, dt=5e-4, device=0):
+          max_bins=1 << 21, dt=5e-4, device=0, max_particles=0, admit_every=0):
     """Continuous-batching run over a request list; returns (stats dict, results list)."""
     cfg = ServeConfig()
     cfg.hdr.size = C.sizeof(ServeConfig)
@@ -1032,6 +1034,8 @@ def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_r
     cfg.max_bins = max_bins
     cfg.dt = dt
     cfg.device = device
+    cfg.max_particles = max_particles
+    cfg.admit_every = admit_every
     n = len(reqs)
     arr = (Request * max(n, 1))(*reqs)
     st = ServeStats()
@@ -1042,6 +1046,7 @@ def serve(reqs, budget_bytes=200 << 30, time_scale=1.0, max_wall_s=3600.0, max_r
     # CAPACITY: some request alone exceeds the budget; it is not served (steps
     # = 0 in its result) and the other requests' results are complete.
     stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
+    stats["layout_phase_ms"] = dict(zip(("wait", "move", "upload", "tables", "graph"), st.layout_phase_ms[:]))
     stats["status"] = STATUS[code]
     results = [dict(index=r.index, n=r.n, n_fluid=r.n_fluid, steps=r.steps, arrival_s=r.arrival_s,
                     admit_s=r.admit_s, finish_s=r.finish_s, state_hash=r.state_hash) for r in res[:n]]

@@ -81,9 +81,9 @@ def test_serve_budget_binds_and_oversized_request_is_rejected():
     big, second = order[-1], order[-2]
     assert sizes[big] > sizes[second]
     budget = (sizes[big] + sizes[second]) // 2  # every request but the largest fits alone
-    free, _ = V.serve([r for k, r in enumerate(reqs) if k != big], budget_bytes=1 << 40, builders=2)
+    free, _ = V.serve([r for k, r in enumerate(reqs) if k != big], budget_bytes=1 << 40, builders=2, admit_every=2)
     assert free["peak_bytes"] > budget  # unconstrained, the stream would hold more
-    stats, results = V.serve(reqs, budget_bytes=budget, builders=2)
+    stats, results = V.serve(reqs, budget_bytes=budget, builders=2, admit_every=2)
     assert stats["status"] == "CAPACITY"
     assert stats["rejected"] == 1 and results[big]["steps"] == 0
     assert stats["completed"] == len(reqs) - 1

@@ -522,7 +522,7 @@ def test_serve_results_equal_alone():
 
     reqs = V.sample_requests(5, 8, rate=200.0, nmin=1500, nmax=12000, smin=3, smax=12, max_boxes=3)
     assert any(r.nboxes for r in reqs) and any(not r.nboxes for r in reqs)
-    stats, results = V.serve(reqs, budget_bytes=1 << 30, time_scale=1.0, builders=2)
+    stats, results = V.serve(reqs, budget_bytes=1 << 30, time_scale=1.0, builders=2, admit_every=2)
     assert stats["completed"] == len(reqs) and stats["status"] == "OK"
     assert stats["peak_resident"] >= 2  # requests overlapped on the device
     for r, res in zip(reqs, results):
@@ -554,7 +554,7 @@ def test_serve_residency_limit_and_fifo_admission():
     # Requests long enough (150-200 steps) that a second one is built and
     # waiting while the first still runs, whatever the host's speed.
     reqs = V.sample_requests(6, 6, rate=1000.0, nmin=1500, nmax=4000, smin=150, smax=200)
-    stats, results = V.serve(reqs, max_resident=2, builders=2)
+    stats, results = V.serve(reqs, max_resident=2, builders=2, admit_every=4)
     assert stats["completed"] == len(reqs)
     assert stats["peak_resident"] == 2
     admits = [r["admit_s"] for r in sorted(results, key=lambda r: r["arrival_s"])]
