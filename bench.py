@@ -64,8 +64,28 @@ def build_scene(V, block, d, tank, max_bins, seed=1):
     return cfg, data
 
 
-def cpu_baseline(cfg, data, steps, threads):
-    """Oracle (test infrastructure) on the same scene, bounded sample."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The host cores this job may use: the machine's count, capped by the
+    share the GPU box allots one GPU (its OMP_NUM_THREADS, 16)."""
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(os.cpu_count() or 1, share))
+
+
+def cpu_baseline(cfg, data, steps, threads, reactions=False):
+    """Oracle (test infrastructure) on the same scene, bounded sample. With
+    reactions=False the boundary-force pass is skipped, as the GPU headline
+    line (a static tank, nothing consumes its reaction) skips it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -73,7 +93,8 @@ def cpu_baseline(cfg, data, steps, threads):
         rest_density=cfg.rest_density, smoothing_radius=cfg.smoothing_radius, xsph_viscosity=cfg.xsph_viscosity,
         sound_speed=cfg.sound_speed, gamma_tait=cfg.gamma_tait, substeps=cfg.substeps, threads=threads,
         tensile_clamp=cfg.tensile_clamp, boundary_layers=cfg.boundary_layers, gravity=tuple(cfg.gravity),
-        max_speed_frac=cfg.max_speed_frac, artificial_viscosity=cfg.artificial_viscosity, max_bins=cfg.max_bins)
+        max_speed_frac=cfg.max_speed_frac, artificial_viscosity=cfg.artificial_viscosity, max_bins=cfg.max_bins,
+        skip_reactions=0 if reactions else 1)
     nf = data.n_fluid
     s = O.System.particles(oc, data.pos[:nf], data.vel[:nf], np.float32(cfg.particle_mass), np.float32(cfg.spacing))
     s.set_boundary_raw(data.pos[nf:], data.vel[nf:], data.pmass[nf:], data.spans)
@@ -82,6 +103,25 @@ def cpu_baseline(cfg, data, steps, threads):
         s.step(cfg.dt_tick)
     dt = time.perf_counter() - t0
     return nf * steps / dt, dt
+
+
+def bench_stepper(V, cfg, data, ticks):
+    """The drop-in path: FluidStepper (GpuFluidStepper semantics) with
+    reactions on every span and the boundary segment uploaded every tick, as
+    World::fluid_stepper_pass drives it; host-to-host time per tick."""
+    st = V.FluidStepper(cfg)
+    nf = data.n_fluid
+    pos, vel = data.pos.copy(), data.vel.copy()  # the stepper writes the state back in place
+    st.step(cfg.dt_tick, nf, pos, vel, data.pmass, data.spans)  # uploads the scene
+    st.step(cfg.dt_tick, nf, pos, vel, data.pmass, data.spans)
+    t0 = time.perf_counter()
+    for _ in range(ticks):
+        st.step(cfg.dt_tick, nf, pos, vel, data.pmass, data.spans)
+        st.reactions()
+    dt = time.perf_counter() - t0
+    st.close()
+    return {"ms_per_tick": 1e3 * dt / ticks, "value": nf * ticks / dt, "unit": "particle-updates/s", "ticks": ticks,
+            "fluid_particles": nf, "boundary_particles": data.n - nf, "spans": len(data.spans)}
 
 
 def bench_batched(V, args, one_at_a_time=True):
@@ -246,7 +286,8 @@ def main():
     ap.add_argument("--block", type=int, default=215, help="fluid lattice edge (215 -> 9.94M particles)")
     ap.add_argument("--budget-log2", type=int, default=22,
                     help="grid budget GRID_MAX_BINS = 2^k (22 keeps bin = h at 10M; the reference rule is 21)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=12,
+                    help="oracle steps timed at the host-core share (about 10 s on 16 cores)")
     ap.add_argument("--cpu-single-steps", type=int, default=1,
                     help="steps of the single-thread CPU baseline (the reference's default threads = 1); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
@@ -269,6 +310,11 @@ def main():
                     help="resident particles per pool step at most (the step's size; the stream overloads one GPU, "
                          "so the excess waits in the queue)")
     ap.add_argument("--no-walls", action="store_true", help="skip the provider-wall measurement")
+    ap.add_argument("--late-steps", type=int, default=480,
+                    help="step the headline scene on to this step and time 20 more (0 skips)")
+    ap.add_argument("--no-ref-rule", action="store_true", help="skip the 2^21 grid-budget figure")
+    ap.add_argument("--no-stepper", action="store_true", help="skip the FluidStepper drop-in timings")
+    ap.add_argument("--stepper-ticks", type=int, default=20)
     ap.add_argument("--walls-steps", type=int, default=200)
     ap.add_argument("--walls-table-ticks", type=int, default=5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -353,6 +399,21 @@ def main():
     prof = eng.profile(args.profile_steps)
     tuning = eng.get_tuning()
     g = eng.read_grid(sid)
+    # Late in the run: the same scene stepped on to step 480, then 20 steps
+    # timed (the flow is far from the initial lattice by then).
+    late = None
+    if args.late_steps:
+        done = args.warmup + 2 * args.steps + args.profile_steps + 4
+        if done < args.late_steps:
+            eng.step(args.late_steps - done)
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.step(20)
+        eng.sync()
+        lw = time.perf_counter() - t0
+        late = {"steps": f"{max(done, args.late_steps)}-{max(done, args.late_steps) + 20}",
+                "ms_per_step": 1e3 * lw / 20, "value": NF * 20 / lw, "unit": "particle-updates/s"}
+        print(f"# late run (steps {late['steps']}): {late['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     C = g["total"] + 1
     per_kernel = {k: v / args.profile_steps for k, v in prof.items()}
     scan_ms = sum(per_kernel.get(k, 0.0) for k in ("scan_reduce", "scan_tiles", "scan_apply"))
@@ -368,18 +429,49 @@ def main():
           file=sys.stderr, flush=True)
 
     eng.close()
+    # The reference's own grid rule at this size (GRID_MAX_BINS = 2^21, core/grid.rs:20): the bin doubles to 2h.
+    ref_rule = None
+    if args.budget_log2 != 21 and not args.no_ref_rule:
+        cfg21, data21 = build_scene(V, block, d, tank, 1 << 21)
+        e21 = V.Engine()
+        s21 = e21.add(cfg21, data21, reactions=False)
+        t21 = e21.autotune(2)
+        e21.step(args.warmup)
+        e21.sync()
+        t0 = time.perf_counter()
+        e21.step(args.steps)
+        e21.sync()
+        w21 = time.perf_counter() - t0
+        g21 = e21.read_grid(s21)
+        e21.close()
+        ref_rule = {"grid_budget": "2^21", "bin_m": float(g21["bin"]), "dims": g21["dims"],
+                    "ms_per_step": 1e3 * w21 / args.steps, "value": NF * args.steps / w21,
+                    "unit": "particle-updates/s", "tuning": t21}
+        print(f"# reference grid rule 2^21 (bin {float(g21['bin']):.3g} m): {ref_rule['ms_per_step']:.3f} ms/step",
+              file=sys.stderr, flush=True)
+    stepper = None
+    if not args.no_stepper:
+        stepper = {}
+        cfg2 = V.dam_break_config(0.02)
+        data2 = V.dam_break(cfg2, (4.0, 2.0, 2.0), (100, 100, 100), 0.02, seed=2)
+        stepper["dam_break_1M"] = bench_stepper(V, cfg2, data2, args.stepper_ticks)
+        stepper["dam_break_10M"] = bench_stepper(V, cfg, data, max(args.stepper_ticks // 2, 1))
+        print("# FluidStepper drop-in: " + ", ".join(f"{k} {v['ms_per_tick']:.3f} ms/tick"
+                                                    for k, v in stepper.items()), file=sys.stderr, flush=True)
     batched = None if args.no_batch else bench_batched(V, args)
     served = None if args.no_serve else bench_served(V, args)
     walls = None if args.no_walls else bench_walls(V, args)
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         print(f"# cpu baseline: oracle, {threads} threads, {args.cpu_steps} steps", file=sys.stderr, flush=True)
         cv, cdt = cpu_baseline(cfg, data, args.cpu_steps, threads)
         cpu = {"value": cv, "unit": "particle-updates/s", "cores": threads, "kind": "port",
+               "nproc": os.cpu_count(), "cpu_model": cpu_model(),
                "sample": f"{args.cpu_steps} steps of the same {NF}-fluid scene ({cdt:.1f} s), strict-f32 C++ oracle, "
-                         f"phase-parallel over {threads} threads"}
+                         f"phase-parallel over {threads} threads (the host share of one GPU; {os.cpu_count()} "
+                         "cores visible), no boundary-force pass (as the GPU line)"}
         if args.cpu_single_steps > 0:
             print(f"# cpu baseline: oracle, 1 thread, {args.cpu_single_steps} step(s)", file=sys.stderr, flush=True)
             c1, c1dt = cpu_baseline(cfg, data, args.cpu_single_steps, 1)
@@ -416,6 +508,9 @@ def main():
         "kernel_algorithmic_GBs": stage_gbs,
         "tuning": tuning,
         "tuning_sweep": sweep,
+        "late_run": late,
+        "reference_grid_rule": ref_rule,
+        "stepper": stepper,
         "batched": batched,
         "served": served,
         "walls": walls,

@@ -28,6 +28,8 @@ class OrConfig(C.Structure):
         ("max_speed_frac", C.c_float),
         ("artificial_viscosity", C.c_float),
         ("max_bins", C.c_uint64),
+        ("skip_reactions", C.c_uint32),
+        ("pad", C.c_uint32),
     ]
 
 

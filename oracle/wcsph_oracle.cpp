@@ -470,7 +470,7 @@ void force_pass(OrSystem *s) {
         }
     };
     parallel_chunks(0, nf, s->cfg.threads, fluid);
-    if (nt > nf) {
+    if (nt > nf && !s->cfg.skip_reactions) {
         // Reaction on boundary particles: bforce[j] -= d * ((m_i*(coef+cv))/denom),
         // accumulated over fluid i ascending (fluid_force.rs:196-206, :111-145).
         for (size_t i = 0; i < nf; ++i) {
@@ -786,6 +786,8 @@ extern "C" void or_default_config(OrConfig *c) {  // config.rs:51-69
     c->max_speed_frac = 0.4f;
     c->artificial_viscosity = 1.0f;
     c->max_bins = 1ull << 21;
+    c->skip_reactions = 0;
+    c->pad = 0;
 }
 
 // fluid_access.rs:7-76

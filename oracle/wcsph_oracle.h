@@ -40,6 +40,8 @@ typedef struct OrConfig {
     float max_speed_frac;
     float artificial_viscosity;
     uint64_t max_bins;           /* GRID_MAX_BINS */
+    uint32_t skip_reactions;     /* 1: no boundary-force pass (timing a GPU line that computes none) */
+    uint32_t pad;
 } OrConfig;
 
 /* A Box body for set_boundary_particles (fluid_step.rs:13-53). */

@@ -337,3 +337,21 @@ def test_time_weighted_reactions_over_substeps():
     mean = (acc * np.float32(0.25)).astype(np.float32)
     s4.step(1.0 / 60.0)
     np.testing.assert_array_equal(bits(s4.reactions()[1][0]), bits(mean))
+
+
+def test_skipped_reactions_leave_the_fluid_bit_identical():
+    """The boundary-force pass only feeds the bodies' reactions: skipping it
+    (the benchmark's static-tank line) changes no fluid bit and zeroes the
+    reaction, while the full pass gives a nonzero one."""
+    def run(skip):
+        cfg = O.default_config(substeps=2, skip_reactions=skip)
+        s = O.System.lattice(cfg, (-0.1, 0.02, -0.1), (4, 4, 4), 0.05)
+        s.set_boundary_boxes([dict(id=3, half=(0.25, 0.05, 0.25), pos=(0.0, -0.05, 0.0))])
+        for _ in range(3):
+            s.step(1.0 / 240.0)
+        return s
+    full, skipped = run(0), run(1)
+    for k in ("pos", "vel", "dens", "press"):
+        assert np.array_equal(bits(getattr(full, k)()), bits(getattr(skipped, k)())), k
+    assert np.any(full.reactions()[1][0] != 0)
+    assert not np.any(skipped.reactions()[1][0])
