@@ -18,7 +18,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libvxl_sph.so")
+# VXL_SPH_LIB names another build of the same library (the host-sanitizer
+# builds the tests load); by default the one beside this file.
+LIB_PATH = os.environ.get("VXL_SPH_LIB") or os.path.join(_HERE, "libvxl_sph.so")
 ABI_VERSION = 1
 # Force kernel variants of VxlSphTuning (vxl_sph.h VXL_SPH_FORCE_*).
 FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK, FORCE_WALK_AHEAD, FORCE_PAIRS = 0, 1, 2, 3, 4
