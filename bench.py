@@ -144,7 +144,7 @@ def bench_batched(V, args, one_at_a_time=True):
     eng.sync()
     wall = time.perf_counter() - t0
     lat = eng.step_timed(args.batch_steps)
-    mix_hash = eng.state_hash(ids[0])
+    prof = {k: round(v / 6, 4) for k, v in eng.profile(6).items()}
     eng.close()
     batched_ups = nf_total * args.batch_steps / wall
     pooled = {"scenes": len(built), "fluid_particles": nf_total,
@@ -153,11 +153,13 @@ def bench_batched(V, args, one_at_a_time=True):
               "ms_per_step": 1e3 * wall / args.batch_steps,
               "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
               "tuning": tuned,
+              "kernel_ms_per_step": prof,
               "workload": "seeded set: fluid counts log-uniform [8192, 524288], tank x,z ~ U[1,4] m, y = 1 m, "
                           "fill ~ U[0.1,0.5], c0 ~ U[15,30], alpha ~ U[0.01,0.05], d = 0.02 m, dt = 5e-4 s"}
     if not one_at_a_time:
         print(f"# batched: {len(built)} scenes, {nf_total} fluid, {1e3 * wall / args.batch_steps:.3f} ms/step "
-              f"(p50 {pooled['step_latency_ms']['p50']:.3f} / p99 {pooled['step_latency_ms']['p99']:.3f} ms)",
+              f"(p50 {pooled['step_latency_ms']['p50']:.3f} / p99 {pooled['step_latency_ms']['p99']:.3f} ms); "
+              "kernels " + ", ".join(f"{k} {v:.3f}" for k, v in sorted(prof.items(), key=lambda x: -x[1])),
               file=sys.stderr, flush=True)
         return pooled
     # One scene at a time through the same engine code (plain single-scene path).

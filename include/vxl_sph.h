@@ -138,7 +138,10 @@ VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out);
 VxlSphStatus vxl_sph_engine_autotune(VxlSphEngine *e, uint32_t substeps, VxlSphTuning *out);
 /* Replay each substep as one hipGraph (default on). */
 VxlSphStatus vxl_sph_engine_set_graphs(VxlSphEngine *e, int32_t on);
-/* Advance every scene by `ticks` ticks (enqueue only; see vxl_sph_engine_sync). */
+/* Advance every scene by `ticks` ticks (enqueue only; see vxl_sph_engine_sync).
+ * Scenes may differ in substeps per tick: a tick is as many launches as the
+ * scene with the most substeps takes, and a scene with s substeps steps in
+ * the last s of them (results as if it were stepped alone). */
 VxlSphStatus vxl_sph_engine_step(VxlSphEngine *e, uint32_t ticks);
 VxlSphStatus vxl_sph_engine_sync(VxlSphEngine *e);
 VxlSphStatus vxl_sph_engine_device_bytes(VxlSphEngine *e, uint64_t *out);

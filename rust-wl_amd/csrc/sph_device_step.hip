@@ -128,7 +128,7 @@ void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, fl
     // neighbours with loads one pair ahead.
     const uint64_t substeps = (uint64_t)ticks * std::max<uint32_t>(cfg.substeps, 1u);
     for (uint64_t k = 0; k < substeps; ++k) {
-        hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream, P);
+        hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream, P, 0u, 0u);
         hipLaunchKernelGGL(k_keys, dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_scan_reduce<8>, dim3(grid_scan), dim3(kBlock), 0, stream, P, tile_sums);
         hipLaunchKernelGGL(k_scan_tiles<8>, dim3(1), dim3(1024), 0, stream, P, tile_sums);

@@ -239,10 +239,11 @@ private:
     uint32_t box_reduce_grid() const;
     uint32_t grown_capacity(uint32_t need, uint64_t cells, bool reactions) const;
     uint32_t grown_cells(uint64_t cells) const;
-    uint32_t pool_substeps() const;  // substeps per tick (shared by every scene of the pool)
-    // The launch sequence of one substep; `mark` (if set) is called after each
-    // kernel launch with the kernel's name.
-    void enqueue_substep(const std::function<void(const char *)> *mark = nullptr);
+    uint32_t pool_substeps() const;  // launches per pool tick: the most substeps of any scene
+    // The launch sequence of one substep, launch tick_sub of a pool tick of
+    // tick_len launches; `mark` (if set) is called after each kernel launch
+    // with the kernel's name.
+    void enqueue_substep(uint32_t tick_sub, uint32_t tick_len, const std::function<void(const char *)> *mark = nullptr);
     // Substep i (counted from the last layout) with the rigid leg's launches at
     // the tick's start and end when the pool has dynamic bodies.
     // `head` = false skips the tick's rigid leg (a replay of a substep whose

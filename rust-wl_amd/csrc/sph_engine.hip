@@ -988,7 +988,7 @@ KernelFn force_kernel(uint32_t variant, uint32_t bs) {
 
 }  // namespace
 
-void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
+void Engine::enqueue_substep(uint32_t tick_sub, uint32_t tick_len, const std::function<void(const char *)> *mark) {
     const uint32_t nb = P_.nblocks;
     const uint32_t grid_slots = tuning_.persist_blocks ? std::min(nb, tuning_.persist_blocks) : nb;
     // density and force: workgroups of block_threads slots
@@ -999,7 +999,7 @@ void Engine::enqueue_substep(const std::function<void(const char *)> *mark) {
     auto M = [&](const char *name) {
         if (mark) (*mark)(name);
     };
-    hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream_, P_);
+    hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream_, P_, tick_sub, tick_len);
     M("grid_box");
     hipLaunchKernelGGL(k_keys, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("keys");
@@ -1089,7 +1089,7 @@ void Engine::enqueue_at(uint64_t i, const std::function<void(const char *)> *mar
         hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
         M("body_refresh");
     }
-    enqueue_substep(mark);
+    enqueue_substep((uint32_t)(i % sub), sub, mark);
     if (P_.nrigid && i % sub == sub - 1) {
         hipLaunchKernelGGL(k_rigid_collect, dim3((P_.nrigid + 63) / 64), dim3(64), 0, stream_, P_);
         M("rigid");
@@ -1265,11 +1265,9 @@ void Engine::step(uint32_t ticks) {
     if (dirty_) layout();
     if (scenes_.empty() || P_.nblocks == 0) return;
     hip_check(hipSetDevice(device_), "hipSetDevice");
-    // All scenes share one substep launch; a tick is `substeps` of them.
+    // A pool tick is as many launches as the scene with the most substeps
+    // takes; every scene steps in the last of them (substep_active).
     const uint32_t sub = pool_substeps();
-    for (auto &s : scenes_)
-        if (s.substeps != sub)
-            throw std::invalid_argument("scenes of one pool must share the substep count");
     uint64_t n = (uint64_t)ticks * sub;
     // A tick left open by profile() is finished launch by launch.
     while (n && substeps_done_ % sub) {

@@ -86,7 +86,7 @@ __global__ void k_box(Pool P);
 __global__ void k_boundary_scatter(Pool P, uint32_t s, const float *bpos, const float *bvel);
 constexpr uint32_t kBoxReduceSpan = kBlock * 8;  // partial boxes per k_box_reduce block
 __global__ void k_box_reduce(Pool P);
-__global__ void k_grid_box(Pool P);
+__global__ void k_grid_box(Pool P, uint32_t tick_sub, uint32_t tick_len);
 __global__ void k_keys(Pool P);
 // ITEMS cells per thread: a scan tile holds kBlock * ITEMS cells (4, 8, 16).
 template <uint32_t ITEMS>

@@ -238,8 +238,9 @@ __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
 }
 
 // grid_box rule per scene (core/grid.rs:34-51), cell bases, pad-cell counts,
-// AABB ping-pong reset, capacity check. One block.
-__global__ __launch_bounds__(1024) void k_grid_box(Pool P) {
+// AABB ping-pong reset, capacity check, and the launch's place in the pool
+// tick (tick_sub of tick_len launches). One block.
+__global__ __launch_bounds__(1024) void k_grid_box(Pool P, uint32_t tick_sub, uint32_t tick_len) {
     __shared__ uint32_t scan[1024];
     __shared__ uint32_t carry;
     __shared__ uint32_t bad;
@@ -329,6 +330,8 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P) {
             P.ctl->err |= kErrCellCapacity;
         else
             P.ctl->step += 1u;
+        P.ctl->tick_sub = tick_sub;
+        P.ctl->tick_len = tick_len;
     }
 }
 
@@ -1204,6 +1207,7 @@ template <uint32_t WALK, uint32_t BS>
 __global__ __launch_bounds__(BS) void k_force_t(Pool P) {
     __shared__ RowTab rowtab[9];
     if (P.ctl->err) return;
+    const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
     const uint32_t stride = P.nblocks * kBlock;
     constexpr uint32_t kSub = kBlock / BS;
     const uint32_t nsub = P.nblocks * kSub;
@@ -1211,6 +1215,9 @@ __global__ __launch_bounds__(BS) void k_force_t(Pool P) {
         const uint32_t b = gridDim.x == nsub ? xcd_block(hb, nsub) : hb;
         const uint32_t s = P.block_scene[b / kSub];
         const SceneParams &sp = P.params[s];
+        // A scene that does not step in this launch keeps its state (every
+        // particle is carried over like a boundary one).
+        const bool active = substep_active(tick_sub, tick_len, sp.substeps);
         if constexpr (WALK != 0) {
             __syncthreads();  // the previous block's walks are done with the table
             row_table(rowtab, P.grid[s], stride);
@@ -1227,7 +1234,7 @@ __global__ __launch_bounds__(BS) void k_force_t(Pool P) {
             vi = P.fr[2u * i + 1u];
             ci2 = P.fr[2u * i].w;
         }
-        const bool fluid = live && ai.w > 0.0f;
+        const bool fluid = live && ai.w > 0.0f && active;
         const SceneGrid &g = P.grid[s];
         const float4 *__restrict__ sA = P.sA;
         const float4 *__restrict__ fr = P.fr;
@@ -1341,11 +1348,12 @@ template __global__ void k_force_t<3, 256>(Pool P);
 // covers the usual case (a wall particle sees about half a neighbourhood).
 __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
     if (P.ctl->err) return;
+    const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
     for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
         const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
-        if (!sp.react) continue;
+        if (!sp.react || !substep_active(tick_sub, tick_len, sp.substeps)) continue;
         const uint32_t j = b * kBlock + threadIdx.x;
         if (j >= sp.pbase + sp.n) continue;
         const uint32_t gj = P.sg[j];
@@ -1449,6 +1457,10 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
     if (k >= P.nspans) return;
     const SpanDev sd = P.spans[k];
     if (!sd.active) return;
+    const uint32_t sub = sd.substeps;
+    const uint32_t tick_sub = P.ctl->tick_sub, tick_len = max(P.ctl->tick_len, sub);
+    if (!substep_active(tick_sub, tick_len, sub)) return;  // the scene does not step in this launch
+    const uint32_t phase = tick_sub + sub - tick_len;      // the scene's substep index inside its tick
     const uint32_t lane = threadIdx.x;
     float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
     float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p = f;
@@ -1480,8 +1492,6 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
         __syncthreads();  // the terms are read before the next chunk's are written
     }
     if (lane != 0u) return;
-    const uint32_t sub = sd.substeps;
-    const uint32_t phase = (P.ctl->step - 1u) % sub;  // substep index inside the tick
     float *o = P.react + 6ull * k;
     if (phase == 0) {
         o[0] = Fx; o[1] = Fy; o[2] = Fz; o[3] = Tx; o[4] = Ty; o[5] = Tz;

@@ -59,6 +59,8 @@ struct SceneParams {
     uint32_t rbody0;     // first dynamic rigid body of the scene (pool rigid table)
     uint32_t nrbody;     // dynamic rigid bodies of the scene (0: none)
     uint32_t fmass_uniform;  // every fluid particle's mass is `mass`, bit for bit
+    uint32_t substeps;   // substeps per tick of this scene (dt = dt_tick / substeps)
+    uint32_t pad[3];
 };
 
 // A dynamic Box body coupled to its scene's fluid (the rigid leg of
@@ -179,8 +181,20 @@ struct Control {
     uint32_t step;       // substep counter: AABB ping-pong parity
     uint32_t ctot;       // cells used this substep (sum of total + 1)
     uint32_t err;        // nonzero: the substep was abandoned (see kErr*)
-    uint32_t pad;
+    uint32_t tick_sub;   // launch index inside the pool tick (set by k_grid_box)
+    uint32_t tick_len;   // launches per pool tick: the most substeps of any scene (0: every launch steps every scene)
+    uint32_t pad[3];
 };
+
+// Scenes of one pool may differ in substeps per tick. A pool tick runs
+// tick_len launches and a scene with s substeps steps in the last s of them;
+// in the others it is carried over unchanged (sorted and its densities taken
+// like the others, not integrated, no reactions). Ending every scene's tick
+// on the pool tick's last launch leaves each scene's fields, reactions and
+// rigid coupling as they are after its own last substep.
+__host__ __device__ inline bool substep_active(uint32_t tick_sub, uint32_t tick_len, uint32_t substeps) {
+    return tick_sub + substeps >= tick_len;
+}
 
 constexpr uint32_t kErrCellCapacity = 1u;
 constexpr uint32_t kErrGrid = 2u;  // non-finite particle box

@@ -115,6 +115,7 @@ SceneParams scene_params(const SceneConfig &cfg, uint32_t n, uint32_t nf) {
     p.vmax = (cfg.max_speed_frac * c.h) / p.dt;      // fluid_step.rs:110
     p.vmax2 = p.vmax * p.vmax;
     p.clamp_neg = cfg.tensile_clamp ? 1u : 0u;
+    p.substeps = sub;
     return p;
 }
 

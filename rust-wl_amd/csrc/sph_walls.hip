@@ -119,11 +119,13 @@ __global__ __launch_bounds__(kBlock) void k_wall_density(Pool P) {
 // order against the position as pushed so far.
 __global__ __launch_bounds__(kBlock) void k_wall_project(Pool P) {
     if (P.ctl->err) return;
+    const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
     for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
         const uint32_t s = P.block_scene[b];
         const WallSceneDev &W = P.walls[s];
         if (W.mode == kWallsOff) continue;
         const SceneParams &sp = P.params[s];
+        if (!substep_active(tick_sub, tick_len, sp.substeps)) continue;  // not integrated in this launch
         const uint32_t i = b * kBlock + threadIdx.x;
         if (i >= sp.pbase + sp.n) continue;
         const float4 a = P.cA[i];
