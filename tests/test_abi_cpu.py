@@ -75,6 +75,42 @@ def test_constants_match_oracle():
             assert bits([c[k]])[0] == bits([oc[k]])[0], k
 
 
+def _powi_squaring(x, n):
+    """x^n by binary exponentiation in f32: the order of LLVM's run-time
+    expansion of llvm.powi.f32 (and of compiler-rt's __powisf2)."""
+    x, r = np.float32(x), None
+    while n:
+        if n & 1:
+            r = x if r is None else np.float32(r * x)
+        n >>= 1
+        if n:
+            x = np.float32(x * x)
+    return r
+
+
+def test_powi_evaluation_order_is_moot_at_the_benchmarked_radii():
+    """h.powi(9) and h.powi(6) (fluid_access.rs:10-11) may be folded through
+    double pow() or expanded as f32 squarings, depending on how the reference
+    binary was built. For h = 2d with d = 0.02 m and 0.01 m (every dam-break
+    workload: h = 0.04, 0.02) both forms give the same f32 bits, so k6 and ks
+    are pinned whatever the build did; at h = 0.1 (the Akinci canary) they
+    differ by an ulp and the canary's recorded readout decides for the folded
+    form."""
+    F = np.float32
+    for d in (0.02, 0.01):
+        cfg = V.dam_break_config(d)
+        h = F(cfg.smoothing_radius)
+        for n in (9, 6):
+            assert F(np.float64(h) ** n) == _powi_squaring(h, n), (d, n)
+        c = V.fluid_constants(cfg)
+        pi = F(np.pi)
+        k6 = F(F(315.0) / F(F(F(64.0) * pi) * _powi_squaring(h, 9)))
+        ks = F(F(45.0) / F(pi * _powi_squaring(h, 6)))
+        assert bits([c["k6"]])[0] == bits([k6])[0] and bits([c["ks"]])[0] == bits([ks])[0], d
+    h = F(0.1)
+    assert F(np.float64(h) ** 9) != _powi_squaring(h, 9)
+
+
 @pytest.mark.parametrize("half,s,h,layers", [((0.06,) * 3, 0.05, 0.1, 2), ((1.04, 0.02, 0.54), 0.02, 0.04, 1),
                                              ((0.3, 0.05, 0.3), 0.05, 0.1, 2), ((0.12, 0.2, 0.05), 0.02, 0.04, 3)])
 def test_box_lattice_matches_oracle(half, s, h, layers):
