@@ -132,7 +132,10 @@ def bench_batched(V, args, one_at_a_time=True):
     nf_total = sum(d.n_fluid for _, d in built)
     eng = V.Engine()
     ids = [eng.add(c, d, reactions=False) for c, d in built]
-    if args.force_kernel is not None:
+    if args.tuning:
+        eng.set_tuning(**json.loads(args.tuning))
+        tuned = eng.get_tuning()
+    elif args.force_kernel is not None:
         eng.set_tuning(force_kernel=args.force_kernel)
         tuned = eng.get_tuning()
     else:
@@ -312,6 +315,9 @@ def main():
                     help="resident particles per pool step at most (the step's size; the stream overloads one GPU, "
                          "so the excess waits in the queue)")
     ap.add_argument("--no-walls", action="store_true", help="skip the provider-wall measurement")
+    ap.add_argument("--tuning", default=None,
+                    help='launch geometry as JSON instead of the autotuner (e.g. \'{"force_kernel": 3, '
+                         '"block_threads": 128}\'), for profiles of a fixed kernel set')
     ap.add_argument("--late-steps", type=int, default=480,
                     help="step the headline scene on to this step and time 20 more (0 skips)")
     ap.add_argument("--no-ref-rule", action="store_true", help="skip the 2^21 grid-budget figure")
@@ -360,7 +366,9 @@ def main():
 
     eng = V.Engine()
     sid = eng.add(cfg, data, reactions=False)
-    if args.force_kernel is not None:
+    if args.tuning:
+        eng.set_tuning(**json.loads(args.tuning))
+    elif args.force_kernel is not None:
         eng.set_tuning(force_kernel=args.force_kernel)
     else:
         tuned = eng.autotune(2)
@@ -372,7 +380,7 @@ def main():
         eng.step(2)
         saved = eng.get_tuning()
         sweep = []
-        for field, values in (("force_kernel", (1, 2, 3)), ("block_threads", (64, 128, 256)),
+        for field, values in (("force_kernel", (1, 2, 3, 4, 5)), ("block_threads", (64, 128, 256)),
                               ("stage_records", (128, 256, 512)), ("scan_tile_cells", (1024, 2048, 4096)),
                               ("persist_blocks", (0, 2048))):
             for v in values:

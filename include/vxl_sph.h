@@ -116,19 +116,20 @@ VxlSphStatus vxl_sph_engine_set_diagnostics(VxlSphEngine *e, int32_t on);
 typedef struct VxlSphTuning {
     VxlSphHeader hdr;
     uint32_t persist_blocks;  /* grid size cap of the per-slot kernels (0 = one block per 256 slots) */
-    uint32_t force_kernel;    /* VXL_SPH_FORCE_* */
+    uint32_t force_kernel;    /* VXL_SPH_FORCE_* (0 = VXL_SPH_FORCE_QUEUE) */
     uint32_t block_threads;   /* workgroup size of the density and force kernels: 64, 128, 256 (0 = 256) */
     uint32_t stage_records;   /* LDS staging depth of the density pass, records per wave and row: 128, 256, 512 (0 = 256) */
     uint32_t scan_tile_cells; /* cells per tile of the cell-count scan: 1024, 2048, 4096 (0 = 2048) */
     uint32_t reserved;
 } VxlSphTuning;
 /* Force pass: every particle tests its neighbour candidates again (SCAN), or
- * walks the accepted neighbours the density pass recorded (WALK). */
+ * walks the accepted neighbours the density pass recorded (the others). */
 #define VXL_SPH_FORCE_DEFAULT 0u
 #define VXL_SPH_FORCE_SCAN 1u
 #define VXL_SPH_FORCE_WALK 2u
 #define VXL_SPH_FORCE_WALK_AHEAD 3u  /* the walk with each pair's loads issued one pair ahead */
 #define VXL_SPH_FORCE_PAIRS 4u       /* row by row, the wave's recorded pairs spread evenly over its lanes */
+#define VXL_SPH_FORCE_QUEUE 5u       /* the walk from a per-lane queue of row chunks opened before the pair loop */
 VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t);
 VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out);
 /* Autotuner: coordinate descent over every field above, each candidate timed

@@ -125,7 +125,7 @@ void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, fl
     hipLaunchKernelGGL(k_box_reduce, dim3(grid_reduce), dim3(kBlock), 0, stream, P);
     // The engine's substep for a pool without walls or reactions
     // (Engine::enqueue_substep), the force pass walking the recorded
-    // neighbours with loads one pair ahead.
+    // neighbours from its per-lane queue of row chunks.
     const uint64_t substeps = (uint64_t)ticks * std::max<uint32_t>(cfg.substeps, 1u);
     for (uint64_t k = 0; k < substeps; ++k) {
         hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream, P, 0u, 0u);
@@ -136,7 +136,7 @@ void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, fl
         hipLaunchKernelGGL(k_place, dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_canon_gather, dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL((k_density<256, 256>), dim3(nb), dim3(kBlock), 0, stream, P);
-        hipLaunchKernelGGL((k_force_t<2, 256>), dim3(nb), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL((k_force_t<4, 256>), dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_box_reduce, dim3(grid_reduce), dim3(kBlock), 0, stream, P);
     }
     hip_ok(hipGetLastError(), "device step launch");

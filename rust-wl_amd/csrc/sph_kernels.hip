@@ -1009,6 +1009,129 @@ __device__ __forceinline__ void walk_ahead(ForceAcc &f, const Pool &P, const Sce
     }
 }
 
+// The walk of the recorded neighbours from a per-lane queue of row chunks.
+// Before the pair loop every lane opens all its rows at once (the masks and
+// row starts load together) and lists, in row order, each chunk that holds a
+// neighbour as (mask, first slot): the density pass's mask of the row's first
+// 32 candidates, then the row's further candidates tested here 32 at a time.
+// The pair loop then only takes the lowest bit of the current chunk and, when
+// the chunk runs out, reads the lane's next chunk from LDS: no row opening and
+// no global load but the neighbour's record. Same pairs, same order as
+// nb_step's walk. Returns false (nothing evaluated) when a lane needs more
+// than kQueueCap chunks; the caller then walks the wave with walk_ahead.
+constexpr uint32_t kQueueCap = 12;
+struct QueueLds {
+    uint2 e[kQueueCap][64];  // chunk k of lane l at e[k][l]
+};
+
+__device__ __forceinline__ bool queue_walk(ForceAcc &f, const Pool &P, const SceneGrid &g, const SceneParams &sp,
+                                           uint32_t i, bool fluid, const float4 ai, const float4 vi, float ci2,
+                                           const RowTab *rowtab, uint32_t stride, QueueLds &q) {
+    const float4 *__restrict__ sA = P.sA;
+    const float4 *__restrict__ fr = P.fr;
+    const bool own_mass = sp.fmass_uniform == 0u;
+    const float h2 = sp.h2;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t n = 0;
+    if (fluid) {
+        const Cursor c = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
+        const uint32_t summ = P.nbs[i];
+        const uint32_t rowb =
+            g.cbase + ((uint32_t)(c.cz - 1) * g.dims[1] + (uint32_t)(c.cy - 1)) * g.dims[0] + (uint32_t)c.cx;
+        uint32_t j0s[9], mks[9];
+#pragma unroll
+        for (uint32_t r = 0; r < 9u; ++r) {
+            j0s[r] = 0u;
+            mks[r] = 0u;
+            if ((summ >> r) & 1u) {
+                const RowTab t = rowtab[r];
+                const uint32_t row = rowb + t.cell;
+                j0s[r] = *at(P.start, row - ((summ >> (9u + r)) & 1u));
+                mks[r] = P.nbm[t.mask + i];
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 9u; ++r) {
+            if (mks[r] != 0u) {
+                if (n < kQueueCap) q.e[n][lane] = make_uint2(mks[r], j0s[r]);
+                ++n;
+            }
+            if ((summ >> (18u + r)) & 1u) {
+                // Candidates past the first 32 of the row, tested 32 at a time.
+                const uint32_t row = rowb + rowtab[r].cell;
+                const uint32_t je = P.start[row - (uint32_t)c.cx + row_last_cell(g, c, r) + 1u];
+                for (uint32_t jb = j0s[r] + 32u; jb < je; jb += 32u) {
+                    uint32_t m = 0;
+                    const uint32_t kn = min(32u, je - jb);
+                    for (uint32_t k = 0; k < kn; ++k) {
+                        const uint32_t j = jb + k;
+                        const float4 aj = *at(sA, j);
+                        const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        if (r2 <= h2 && j != i) m |= 1u << k;
+                    }
+                    if (m != 0u) {
+                        if (n < kQueueCap) q.e[n][lane] = make_uint2(m, jb);
+                        ++n;
+                    }
+                }
+            }
+        }
+    }
+    if (__any(n > kQueueCap)) return false;
+    const ForceConst fc = force_const(sp);
+    uint32_t e = 0, mk = 0, j0 = 0;
+    if (n != 0u) {
+        const uint2 v = q.e[0][lane];
+        mk = v.x;
+        j0 = v.y;
+    }
+    // The lane's next pair, if any (chunks in order, bits in order).
+    auto pop = [&](uint32_t &j) -> bool {
+        if (mk == 0u) return false;
+        j = j0 + (uint32_t)__builtin_ctz(mk);
+        mk &= mk - 1u;
+        if (mk == 0u && ++e < n) {
+            const uint2 v = q.e[e][lane];
+            mk = v.x;
+            j0 = v.y;
+        }
+        return true;
+    };
+    auto eval = [&](const float4 r0, const float4 r1, uint32_t j) {
+        float4 aj, vj;
+        float cj;
+        nb_record(r0, r1, j, sA, sp.mass, own_mass, aj, vj, cj);
+        force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
+    };
+    // Two pairs in flight: each pair's record loads are issued one pair
+    // before it is evaluated, alternating between two register sets.
+    uint32_t ja = 0, jb = 0;
+    float4 a0 = ai, a1 = vi, b0 = ai, b1 = vi;
+    bool ha = pop(ja);
+    if (ha) {
+        a0 = *at(fr, 2u * ja);
+        a1 = *at(fr, 2u * ja + 1u);
+    }
+    for (;;) {
+        if (!__any(ha)) break;
+        const bool hb = pop(jb);
+        if (hb) {
+            b0 = *at(fr, 2u * jb);
+            b1 = *at(fr, 2u * jb + 1u);
+        }
+        if (ha) eval(a0, a1, ja);
+        if (!__any(hb)) break;
+        ha = pop(ja);
+        if (ha) {
+            a0 = *at(fr, 2u * ja);
+            a1 = *at(fr, 2u * ja + 1u);
+        }
+        if (hb) eval(b0, b1, jb);
+    }
+    return true;
+}
+
 // The three terms one neighbour adds to a particle's force sums, as
 // force_pair forms them (fluid_force.rs:165-217), kept apart so that another
 // lane can add them in the particle's order: pressure d * (coef / denom),
@@ -1201,6 +1324,8 @@ __device__ __forceinline__ void pair_parallel(ForceAcc &f, const Pool &P, const 
 // positions, same test), so the candidate scan is not repeated; the wave
 // evaluates one pair per lane per iteration.
 // WALK = 2: the same walk with the next pair's loads issued one pair ahead.
+// WALK = 3: pair-parallel rows (pair_parallel).
+// WALK = 4: the walk from a per-lane queue of row chunks (queue_walk).
 //
 // BS = threads per workgroup (64, 128 or 256, inside one pool block).
 template <uint32_t WALK, uint32_t BS>
@@ -1279,6 +1404,10 @@ __global__ __launch_bounds__(BS) void k_force_t(Pool P) {
             }
         } else if constexpr (WALK == 2) {
             walk_ahead(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride);
+        } else if constexpr (WALK == 4) {
+            __shared__ QueueLds queue_lds[BS / 64];
+            if (!queue_walk(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride, queue_lds[threadIdx.x >> 6]))
+                walk_ahead(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride);
         } else {
             // Pair-parallel rows (pair_parallel) for waves whose rows all fit a
             // 32-bit mask and whose neighbours lie within 2^26 slots of the
@@ -1332,6 +1461,9 @@ template __global__ void k_force_t<2, 128>(Pool P);
 template __global__ void k_force_t<0, 256>(Pool P);
 template __global__ void k_force_t<1, 256>(Pool P);
 template __global__ void k_force_t<2, 256>(Pool P);
+template __global__ void k_force_t<4, 64>(Pool P);
+template __global__ void k_force_t<4, 128>(Pool P);
+template __global__ void k_force_t<4, 256>(Pool P);
 template __global__ void k_force_t<3, 64>(Pool P);
 template __global__ void k_force_t<3, 128>(Pool P);
 template __global__ void k_force_t<3, 256>(Pool P);

@@ -279,7 +279,7 @@ def test_reference_akinci_canary_on_gpu():
     np.testing.assert_array_equal(bits(gft), bits(ft))
 
 
-@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS])
+@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE])
 @pytest.mark.parametrize("persist", [0, 7])
 @pytest.mark.parametrize("budget", [1 << 21, 1 << 9])
 def test_kernel_variants_bit_exact(force_kernel, persist, budget):
@@ -319,7 +319,7 @@ def test_tuning_knobs_bit_exact(block_threads, budget):
     want_acc, want_xs, want_hash = bits(orc.acc()), bits(orc.xsph()), orc.state_hash()
     for stage in (128, 256, 512):
         for scan in (1024, 2048, 4096):
-            for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS):
+            for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE):
                 eng = V.Engine(diagnostics=True)
                 sid = eng.add(cfg, data, reactions=False)
                 eng.set_tuning(force_kernel=fk, block_threads=block_threads, stage_records=stage,
@@ -342,7 +342,7 @@ def test_autotuner_picks_allowed_values_and_keeps_results():
     sid = eng.add(cfg, data, reactions=False)
     t = eng.autotune(1)
     assert t["block_threads"] in (64, 128, 256) and t["stage_records"] in (128, 256, 512)
-    assert t["scan_tile_cells"] in (1024, 2048, 4096) and t["force_kernel"] in (1, 2, 3, 4)
+    assert t["scan_tile_cells"] in (1024, 2048, 4096) and t["force_kernel"] in (1, 2, 3, 4, 5)
     assert eng.get_tuning() == t
     # the tuning substeps advanced the scene; count them by replaying on the oracle
     orc = oracle_dam_break(cfg, data, tank, d)
@@ -363,7 +363,7 @@ def test_unclamped_pressure_bit_exact():
     the oracle bit for bit."""
     cfg, data, tank, d = small_scene(seed=17, block=(14, 10, 9), tank=(0.5, 0.4, 0.4))
     cfg.tensile_clamp = 0
-    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS):
+    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE):
         orc = oracle_dam_break(cfg, data, tank, d)
         eng = V.Engine(diagnostics=True)
         sid = eng.add(cfg, data, reactions=False)
@@ -393,7 +393,7 @@ def test_coincident_particles_bit_exact():
     near_b = nf + int(np.argmin(np.linalg.norm(data.pos[nf:] - pos[40], axis=1)))
     pos[40] = pos[near_b]
     data = V.SceneData(pos, data.vel, data.pmass, nf, data.spans)
-    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS):
+    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE):
         orc = oracle_dam_break(cfg, data, tank, d)
         eng = V.Engine(diagnostics=True)
         sid = eng.add(cfg, data, reactions=False)
