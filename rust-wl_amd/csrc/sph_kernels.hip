@@ -1579,9 +1579,10 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
 //
 // One wave per span. The terms (f_k and the cross products) do not depend on
 // the running sums: the lanes load 64 consecutive particles and form their
-// terms in LDS, and lane 0 adds them in index order while the lanes' loads of
-// the next 64 are in flight. Same operations, same order, same bits as a
-// serial fold.
+// terms in LDS, then lane c (c < 6) adds component c of the 64 terms in index
+// order (the six sums are independent chains, each folded by its own lane)
+// while the lanes' loads of the next 64 are in flight. Same operations, same
+// order, same bits as a serial fold.
 __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
     __shared__ float term[kReactChunk][8];  // six terms per particle, rows padded to 32 bytes
     if (P.ctl->err) return;
@@ -1594,7 +1595,7 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
     if (!substep_active(tick_sub, tick_len, sub)) return;  // the scene does not step in this launch
     const uint32_t phase = tick_sub + sub - tick_len;      // the scene's substep index inside its tick
     const uint32_t lane = threadIdx.x;
-    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
+    float acc = 0.0f;  // component `lane` of (F, tau) for lanes 0..5
     float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p = f;
     if (sd.start + lane < sd.end) {
         f = P.bf[sd.start + lane];
@@ -1613,28 +1614,17 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
             p = P.bp[qn + lane];
         }
         __syncthreads();
-        if (lane == 0u) {
+        if (lane < 6u) {
             const uint32_t nq = min(kReactChunk, sd.end - q);
-            for (uint32_t u = 0; u < nq; ++u) {
-                const float *w = term[u];
-                Fx = Fx + w[0]; Fy = Fy + w[1]; Fz = Fz + w[2];
-                Tx = Tx + w[3]; Ty = Ty + w[4]; Tz = Tz + w[5];
-            }
+#pragma unroll 8
+            for (uint32_t u = 0; u < nq; ++u) acc = acc + term[u][lane];
         }
         __syncthreads();  // the terms are read before the next chunk's are written
     }
-    if (lane != 0u) return;
-    float *o = P.react + 6ull * k;
-    if (phase == 0) {
-        o[0] = Fx; o[1] = Fy; o[2] = Fz; o[3] = Tx; o[4] = Ty; o[5] = Tz;
-    } else {
-        o[0] = o[0] + Fx; o[1] = o[1] + Fy; o[2] = o[2] + Fz;
-        o[3] = o[3] + Tx; o[4] = o[4] + Ty; o[5] = o[5] + Tz;
-    }
-    if (sub > 1 && phase == sub - 1) {
-        const float inv = 1.0f / (float)sub;
-        for (int a = 0; a < 6; ++a) o[a] = o[a] * inv;
-    }
+    if (lane >= 6u) return;
+    float *o = P.react + 6ull * k + lane;
+    *o = phase == 0 ? acc : *o + acc;
+    if (sub > 1 && phase == sub - 1) *o = *o * (1.0f / (float)sub);
 }
 
 // Resident scenes move to their new slots at a layout: move k copies the
