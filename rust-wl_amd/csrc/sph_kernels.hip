@@ -1475,18 +1475,25 @@ template __global__ void k_force_t<3, 256>(Pool P);
 // fluid_force.rs:196-206). The neighbours arrive in cell order, so they are
 // subtracted in batches: each pass over the candidates keeps, sorted by
 // original index, the kReactBatch smallest indices above the last one already
-// subtracted, subtracts them in order, and passes again while any were left
-// out. Any neighbour count gives the reference's sum bit for bit; one pass
-// covers the usual case (a wall particle sees about half a neighbourhood).
-__global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
+// subtracted (insertion into the lane's column of an LDS table of index and
+// slot), then evaluates and subtracts them in that order, and passes again
+// while any were left out. Any neighbour count gives the reference's sum bit
+// for bit; one pass covers the usual case (a wall particle sees about half a
+// neighbourhood). kBforceThreads slots per workgroup, inside one pool block.
+__global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
+    __shared__ uint32_t skey[kReactBatch][kBforceThreads];   // original index, ascending
+    __shared__ uint32_t sslot[kReactBatch][kBforceThreads];  // its slot
     if (P.ctl->err) return;
     const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
-    for (uint32_t hb = blockIdx.x; hb < P.nblocks; hb += gridDim.x) {
-        const uint32_t b = gridDim.x == P.nblocks ? xcd_block(hb, P.nblocks) : hb;
-        const uint32_t s = P.block_scene[b];
+    constexpr uint32_t kSub = kBlock / kBforceThreads;
+    const uint32_t nsub = P.nblocks * kSub;
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t hb = blockIdx.x; hb < nsub; hb += gridDim.x) {
+        const uint32_t b = gridDim.x == nsub ? xcd_block(hb, nsub) : hb;
+        const uint32_t s = P.block_scene[b / kSub];
         const SceneParams &sp = P.params[s];
         if (!sp.react || !substep_active(tick_sub, tick_len, sp.substeps)) continue;
-        const uint32_t j = b * kBlock + threadIdx.x;
+        const uint32_t j = b * kBforceThreads + lane;
         if (j >= sp.pbase + sp.n) continue;
         const uint32_t gj = P.sg[j];
         if (gj < sp.nf) continue;  // a fluid particle (the index prefix)
@@ -1502,8 +1509,6 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
         const float rho_j = -vj.w, cj2 = P.fr[2u * j].w, mj = -aj.w;  // a boundary record: rho negated
         const Cursor cur = make_cursor(g, sp.h, aj.x, aj.y, aj.z);
         const float h = sp.h, h2 = sp.h2, ks = sp.ks, alpha_c = sp.alpha_c;
-        uint32_t kid[kReactBatch];
-        float kx[kReactBatch], ky[kReactBatch], kz[kReactBatch];
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         bool first = true;
         uint32_t last = 0;  // original index of the last neighbour subtracted
@@ -1522,7 +1527,6 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
                         const uint32_t i = ib + u;
                         const float4 ai = a4[u];
                         if (i >= i1 || !(ai.w > 0.0f)) continue;  // fluid neighbours only
-                        // d = p_i - p_j as seen from the fluid particle i.
                         const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
                         const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                         if (!(r2 <= h2)) continue;
@@ -1530,42 +1534,49 @@ __global__ __launch_bounds__(kBlock) void k_bforce(Pool P) {
                         if (!first && gi <= last) continue;  // subtracted by an earlier pass
                         if (cnt == kReactBatch) {
                             more = true;
-                            if (gi > kid[kReactBatch - 1]) continue;  // a later pass takes it
+                            if (gi > skey[kReactBatch - 1][lane]) continue;  // a later pass takes it
                         }
-                        const float4 vi = P.fr[2u * i + 1u];
-                        const float rho_i = vi.w, ci2 = P.fr[2u * i].w, mi = ai.w;
-                        const float r = sqrtf(r2);
-                        const float t = h - r;
-                        const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
-                        const float denom = fmaxf(r, 1e-9f);
-                        const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
-                        const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
-                        float cv = 0.0f;
-                        if (vdn > 0.0f) {
-                            const float mu = (vdn * h) / (r2 + 0.01f * h2);
-                            cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
-                        }
-                        const float q = (mi * (coef + cv)) / denom;
                         // insertion by original index (a full batch drops its largest)
                         uint32_t k = cnt < kReactBatch ? cnt++ : kReactBatch - 1u;
-                        while (k > 0 && kid[k - 1] > gi) {
-                            kid[k] = kid[k - 1];
-                            kx[k] = kx[k - 1]; ky[k] = ky[k - 1]; kz[k] = kz[k - 1];
+                        while (k > 0 && skey[k - 1][lane] > gi) {
+                            skey[k][lane] = skey[k - 1][lane];
+                            sslot[k][lane] = sslot[k - 1][lane];
                             --k;
                         }
-                        kid[k] = gi;
-                        kx[k] = ddx * q; ky[k] = ddy * q; kz[k] = ddz * q;
+                        skey[k][lane] = gi;
+                        sslot[k][lane] = i;
                     }
                 }
             });
+            // The batch in ascending original index: d = p_i - p_j as seen
+            // from the fluid particle i.
             for (uint32_t k = 0; k < cnt; ++k) {
-                fx = fx - kx[k];
-                fy = fy - ky[k];
-                fz = fz - kz[k];
+                const uint32_t i = sslot[k][lane];
+                const float4 ai = *at(P.sA, i);
+                const float4 vi = *at(P.fr, 2u * i + 1u);
+                const float ci2 = at(P.fr, 2u * i)->w;
+                const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                const float rho_i = vi.w, mi = ai.w;
+                const float r = sqrtf(r2);
+                const float t = h - r;
+                const float coef = (mj * ((ks * t) * t)) * (ci2 + cj2);
+                const float denom = fmaxf(r, 1e-9f);
+                const float wx = vi.x - vj.x, wy = vi.y - vj.y, wz = vi.z - vj.z;
+                const float vdn = -(wx * ddx + wy * ddy + wz * ddz);
+                float cv = 0.0f;
+                if (vdn > 0.0f) {
+                    const float mu = (vdn * h) / (r2 + 0.01f * h2);
+                    cv = (mj * ((alpha_c * mu) / (0.5f * (rho_i + rho_j)))) * ((ks * t) * t);
+                }
+                const float q = (mi * (coef + cv)) / denom;
+                fx = fx - ddx * q;
+                fy = fy - ddy * q;
+                fz = fz - ddz * q;
             }
             if (!more) break;
             first = false;
-            last = kid[kReactBatch - 1];
+            last = skey[kReactBatch - 1][lane];
         }
         const uint32_t d = sp.pbase + gj;
         P.bf[d] = make_float4(fx, fy, fz, 0.0f);
