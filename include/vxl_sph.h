@@ -146,6 +146,11 @@ VxlSphStatus vxl_sph_engine_set_graphs(VxlSphEngine *e, int32_t on);
 VxlSphStatus vxl_sph_engine_step(VxlSphEngine *e, uint32_t ticks);
 VxlSphStatus vxl_sph_engine_sync(VxlSphEngine *e);
 VxlSphStatus vxl_sph_engine_device_bytes(VxlSphEngine *e, uint64_t *out);
+/* Tick graphs instantiated and updated in place so far. A layout (scenes
+ * added or removed) captures the tick again; when its launch sequence is
+ * unchanged the executable graph is updated (hipGraphExecUpdate) rather than
+ * instantiated. No counterpart in the reference. */
+VxlSphStatus vxl_sph_engine_graph_stats(VxlSphEngine *e, uint64_t *built, uint64_t *updated);
 
 /* Stage timings. No C-ABI counterpart in the reference; they replace its
  * per-tick stage timer TickMs (crates/vxl-phys-gpu/src/pipeline/gpu_types.rs:48-58)

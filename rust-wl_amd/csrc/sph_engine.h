@@ -192,6 +192,8 @@ public:
     // step's graph afterwards.
     const std::vector<float> &layout_ms() const { return layout_ms_; }
     const double *layout_phase_ms() const { return layout_phase_ms_; }
+    uint64_t graph_builds() const { return graph_builds_; }    // tick graphs instantiated
+    uint64_t graph_updates() const { return graph_updates_; }  // tick graphs updated in place
     uint32_t capacity_slots() const { return slots_; }
     uint32_t used_slots() const { return used_slots_; }
     // Device bytes a scene of n particles (cells_hint grid cells) adds to the
@@ -222,7 +224,7 @@ public:
     // Apply pending scene changes (the layout) now.
     void prepare() {
         if (dirty_) layout();
-        if (use_graph_ && !gexec_ && P_.nblocks) build_graph();
+        if (use_graph_ && (!gexec_ || graph_stale_) && P_.nblocks) build_graph();
     }
     // Scene-table bytes a layout allocates (bounded above).
     static uint64_t table_allowance(uint64_t nscenes, uint64_t slots, bool reactions);
@@ -311,6 +313,9 @@ private:
     std::unordered_map<void *, size_t> alloc_size_;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
+    bool graph_stale_ = false;     // gexec_ holds the previous layout's launches (update it before use)
+    uint64_t graph_builds_ = 0;    // tick graphs instantiated
+    uint64_t graph_updates_ = 0;   // tick graphs updated in place after a layout
     uint64_t substeps_done_ = 0;
     uint32_t ctl_step_base_ = 0;
     uint64_t bytes_ = 0;

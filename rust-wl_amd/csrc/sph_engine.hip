@@ -129,6 +129,7 @@ void Engine::invalidate_graph() {
     if (graph_) (void)hipGraphDestroy(graph_);
     gexec_ = nullptr;
     graph_ = nullptr;
+    graph_stale_ = false;
 }
 
 SceneSlot Engine::prepare_scene(const SceneConfig &cfg, uint32_t n, uint32_t nf, const float *pos, const float *vel,
@@ -392,7 +393,10 @@ void Engine::layout() {
     // Finish (replay) any abandoned substep of the old layout first: the reset
     // of the control block below would otherwise drop it.
     if (P_.nblocks) check_error_state();
-    invalidate_graph();
+    // The tick graph's launches take the pool by value: it is captured again
+    // for the new layout and, when the launch sequence is unchanged, updated
+    // in place instead of instantiated again (build_graph).
+    graph_stale_ = gexec_ != nullptr;
     // Retired scenes leave the pool (their readbacks completed with the sync above).
     for (size_t k = 0; k < scenes_.size();) {
         if (scenes_[k].frozen) {
@@ -1259,11 +1263,33 @@ void Engine::set_tuning(const Tuning &t) {
 // One whole tick (rigid leg, substeps) as a graph; launched at tick starts.
 void Engine::build_graph() {
     const auto t0 = std::chrono::steady_clock::now();
-    invalidate_graph();
+    if (!graph_stale_) invalidate_graph();
+    hipGraph_t g = nullptr;
     hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
     for (uint32_t i = 0; i < pool_substeps(); ++i) enqueue_at(i);
-    hip_check(hipStreamEndCapture(stream_, &graph_), "end capture");
-    hip_check(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+    hip_check(hipStreamEndCapture(stream_, &g), "end capture");
+    bool updated = false;
+    if (graph_stale_ && gexec_) {
+        // Same launch sequence (kernels, order) with new arguments and grid
+        // sizes: update the executable graph. Any other change (a scene with
+        // walls, bodies or reactions joined or left) fails the update, and
+        // the graph is instantiated afresh.
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(gexec_, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        (void)hipGetLastError();  // a refused update is not an error of the next call
+    }
+    if (updated) {
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_ = g;
+        ++graph_updates_;
+    } else {
+        invalidate_graph();
+        graph_ = g;
+        hip_check(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+        ++graph_builds_;
+    }
+    graph_stale_ = false;
     layout_phase_ms_[4] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1281,7 +1307,7 @@ void Engine::step(uint32_t ticks) {
         --n;
     }
     if (use_graph_) {
-        if (!gexec_) build_graph();
+        if (!gexec_ || graph_stale_) build_graph();
         for (uint64_t t = 0; t < n / sub; ++t) hip_check(hipGraphLaunch(gexec_, stream_), "graph launch");
         substeps_done_ += n;
     } else {

@@ -330,6 +330,13 @@ VxlSphStatus vxl_sph_engine_device_bytes(VxlSphEngine *e, uint64_t *out) {
     return VXL_SPH_OK;
 }
 
+VxlSphStatus vxl_sph_engine_graph_stats(VxlSphEngine *e, uint64_t *built, uint64_t *updated) {
+    if (!e || !built || !updated) return fail(VXL_SPH_NULL_POINTER, "null argument");
+    *built = e->eng.graph_builds();
+    *updated = e->eng.graph_updates();
+    return VXL_SPH_OK;
+}
+
 VxlSphStatus vxl_sph_engine_profile(VxlSphEngine *e, uint32_t substeps, uint32_t cap, VxlSphKernelTime *out,
                                     uint32_t *count) {
     if (!e || !count || (cap && !out)) return fail(VXL_SPH_NULL_POINTER, "null argument");

@@ -157,6 +157,7 @@ EXPORTS = (
     "vxl_sph_device_count", "vxl_sph_last_error", "vxl_sph_default_config",
     "vxl_sph_engine_create", "vxl_sph_engine_destroy", "vxl_sph_engine_set_diagnostics",
     "vxl_sph_engine_set_graphs", "vxl_sph_engine_step", "vxl_sph_engine_sync", "vxl_sph_engine_device_bytes",
+    "vxl_sph_engine_graph_stats",
     "vxl_sph_engine_profile", "vxl_sph_engine_step_timed", "vxl_sph_engine_set_tuning",
     "vxl_sph_engine_get_tuning", "vxl_sph_engine_autotune",
     "vxl_sph_scene_add", "vxl_sph_scene_remove", "vxl_sph_scene_upload_boundary", "vxl_sph_scene_read_state",
@@ -209,6 +210,8 @@ def lib():
     L.vxl_sph_engine_step.argtypes = [P, C.c_uint32]
     L.vxl_sph_engine_device_bytes.restype = st
     L.vxl_sph_engine_device_bytes.argtypes = [P, C.POINTER(C.c_uint64)]
+    L.vxl_sph_engine_graph_stats.restype = st
+    L.vxl_sph_engine_graph_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.vxl_sph_engine_set_tuning.restype = st
     L.vxl_sph_engine_set_tuning.argtypes = [P, C.POINTER(Tuning)]
     L.vxl_sph_engine_get_tuning.restype = st
@@ -541,6 +544,12 @@ class Engine:
         v = C.c_uint64()
         _check(lib().vxl_sph_engine_device_bytes(self.h, C.byref(v)))
         return v.value
+
+    def graph_stats(self):
+        """(tick graphs instantiated, tick graphs updated in place after a layout)."""
+        b, u = C.c_uint64(), C.c_uint64()
+        _check(lib().vxl_sph_engine_graph_stats(self.h, C.byref(b), C.byref(u)))
+        return b.value, u.value
 
     def set_tuning(self, persist_blocks=0, force_kernel=FORCE_DEFAULT, block_threads=0, stage_records=0,
                    scan_tile_cells=0):

@@ -624,3 +624,31 @@ def test_grid_box_known_answers_on_device():
         assert g["dims"] == c["dims"], c["what"]
         assert int(bits([g["bin"]])[0]) == c["bin_bits"], c["what"]
         assert bits(g["min"]).tolist() == bits(np.array(c["lo"], np.float32)).tolist(), c["what"]
+
+
+def test_relayout_updates_the_tick_graph_in_place():
+    """Scenes joining and leaving a pool change the tick graph's arguments
+    and grid sizes but not its launch sequence: the executable graph is
+    updated in place (hipGraphExecUpdate) rather than instantiated again,
+    and every scene still equals the same scene stepped alone."""
+    specs = [dict(seed=51, block=(8, 6, 6)), dict(seed=52, block=(12, 9, 7)), dict(seed=53, block=(10, 10, 8))]
+    scenes = []
+    for sp in specs:
+        cfg = V.dam_break_config(0.02)
+        scenes.append((cfg, V.dam_break(cfg, (0.4, 0.3, 0.3), sp["block"], 0.02, seed=sp["seed"])))
+    eng = V.Engine()
+    a = eng.add(*scenes[0], reactions=False)
+    eng.step(2)
+    b = eng.add(*scenes[1], reactions=False)  # relayout: grows the pool
+    eng.step(2)
+    eng.remove(a)                            # relayout: the pool shrinks to one scene
+    eng.step(2)
+    c = eng.add(*scenes[2], reactions=False)
+    eng.step(2)
+    built, updated = eng.graph_stats()
+    assert built >= 1 and updated >= 2, (built, updated)
+    for sid, (cfg, data), ticks in ((b, scenes[1], 6), (c, scenes[2], 2)):
+        alone = V.Engine()
+        s = alone.add(cfg, data, reactions=False)
+        alone.step(ticks)
+        assert eng.state_hash(sid) == alone.state_hash(s)
