@@ -297,8 +297,10 @@ def main():
                     help="steps of the single-thread CPU baseline (the reference's default threads = 1); 0 skips it")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
-    ap.add_argument("--force-kernel", type=int, default=None, choices=(1, 2, 3),
-                    help="force kernel instead of the autotuner's choice: 1 candidate scan, 2 neighbour walk, 3 walk with loads one pair ahead")
+    ap.add_argument("--force-kernel", type=int, default=None, choices=(1, 2, 3, 4, 5, 6, 7),
+                    help="force kernel instead of the autotuner's choice: 1 candidate scan, 2 neighbour walk, "
+                         "3 walk with loads one pair ahead, 4 pair-parallel rows, 5 queue walk, "
+                         "6 queue walk with lane-pair record loads, 7 queue walk with each wave's own row staged in LDS")
     ap.add_argument("--sweep", action="store_true", help="print per-kernel times for every tuning variant")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched measurement")
     ap.add_argument("--only-batch", action="store_true",
@@ -380,7 +382,7 @@ def main():
         eng.step(2)
         saved = eng.get_tuning()
         sweep = []
-        for field, values in (("force_kernel", (1, 2, 3, 4, 5)), ("block_threads", (64, 128, 256)),
+        for field, values in (("force_kernel", (1, 2, 3, 4, 5, 6, 7)), ("block_threads", (64, 128, 256)),
                               ("stage_records", (128, 256, 512)), ("scan_tile_cells", (1024, 2048, 4096)),
                               ("persist_blocks", (0, 2048))):
             for v in values:

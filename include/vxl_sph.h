@@ -130,6 +130,8 @@ typedef struct VxlSphTuning {
 #define VXL_SPH_FORCE_WALK_AHEAD 3u  /* the walk with each pair's loads issued one pair ahead */
 #define VXL_SPH_FORCE_PAIRS 4u       /* row by row, the wave's recorded pairs spread evenly over its lanes */
 #define VXL_SPH_FORCE_QUEUE 5u       /* the walk from a per-lane queue of row chunks opened before the pair loop */
+#define VXL_SPH_FORCE_PAIRED 6u      /* the queue walk with each neighbour record loaded by a lane pair */
+#define VXL_SPH_FORCE_STAGED 7u      /* the queue walk with each wave's own cell row staged in LDS */
 VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t);
 VxlSphStatus vxl_sph_engine_get_tuning(VxlSphEngine *e, VxlSphTuning *out);
 /* Autotuner: coordinate descent over every field above, each candidate timed

@@ -974,8 +974,16 @@ KernelFn force_kernel(uint32_t variant, uint32_t bs) {
                        : variant == Engine::kForceWalkAhead ? 2u
                        : variant == Engine::kForcePairs     ? 3u
                        : variant == Engine::kForceQueue     ? 4u
+                       : variant == Engine::kForcePaired    ? 5u
+                       : variant == Engine::kForceStaged    ? 6u
                                                             : 1u;
     switch (w * 1000u + bs) {
+        case 6064: return k_force_t<6, 64>;
+        case 6128: return k_force_t<6, 128>;
+        case 6256: return k_force_t<6, 256>;
+        case 5064: return k_force_t<5, 64>;
+        case 5128: return k_force_t<5, 128>;
+        case 5256: return k_force_t<5, 256>;
         case 4064: return k_force_t<4, 64>;
         case 4128: return k_force_t<4, 128>;
         case 4256: return k_force_t<4, 256>;
@@ -1190,7 +1198,7 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
     const std::vector<Axis> axes = {
         {&Tuning::force_kernel,
          {(uint32_t)kForceScan, (uint32_t)kForceWalk, (uint32_t)kForceWalkAhead, (uint32_t)kForcePairs,
-          (uint32_t)kForceQueue}},
+          (uint32_t)kForceQueue, (uint32_t)kForcePaired, (uint32_t)kForceStaged}},
         {&Tuning::block_threads, {64u, 128u, 256u}},
         {&Tuning::stage_records, {128u, 256u, 512u}},
         {&Tuning::scan_tile_cells, {1024u, 2048u, 4096u}},
@@ -1249,7 +1257,7 @@ void Engine::set_tuning(const Tuning &t) {
     if (u.block_threads == 0) u.block_threads = d.block_threads;
     if (u.stage_records == 0) u.stage_records = d.stage_records;
     if (u.scan_tile_cells == 0) u.scan_tile_cells = d.scan_tile_cells;
-    if (u.force_kernel > kForceQueue) throw std::invalid_argument("unknown kernel variant");
+    if (u.force_kernel > kForceStaged) throw std::invalid_argument("unknown kernel variant");
     if (u.block_threads != 64 && u.block_threads != 128 && u.block_threads != 256)
         throw std::invalid_argument("block_threads must be 64, 128 or 256");
     if (u.stage_records != 128 && u.stage_records != 256 && u.stage_records != 512)

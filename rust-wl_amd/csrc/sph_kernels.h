@@ -106,7 +106,8 @@ template <uint32_t BS>
 __global__ void k_density_walls(Pool P);  // pools with half-space walls: mirror ghosts in the sums
 template <uint32_t WALK, uint32_t BS>
 __global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks;
-                                      // 2: the walk, loads one pair ahead
+                                      // 2: the walk, loads one pair ahead; 3: pair-parallel rows;
+                                      // 4: queue walk; 5: queue walk, own rows staged in LDS
 __global__ void k_bforce(Pool P);
 // Provider walls (scenes with WallSceneDev::mode != kWallsOff):
 // k_wall_band + k_wall_density after the density pass (table scenes: the

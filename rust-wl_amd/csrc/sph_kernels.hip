@@ -1017,22 +1017,27 @@ __device__ __forceinline__ void walk_ahead(ForceAcc &f, const Pool &P, const Sce
 // The pair loop then only takes the lowest bit of the current chunk and, when
 // the chunk runs out, reads the lane's next chunk from LDS: no row opening and
 // no global load but the neighbour's record. Same pairs, same order as
-// nb_step's walk. Returns false (nothing evaluated) when a lane needs more
-// than kQueueCap chunks; the caller then walks the wave with walk_ahead.
-constexpr uint32_t kQueueCap = 12;
+// nb_step's walk. A lane that needs more than CAP chunks makes its wave walk
+// with walk_ahead instead.
+template <uint32_t CAP>
 struct QueueLds {
-    uint2 e[kQueueCap][64];  // chunk k of lane l at e[k][l]
+    uint2 e[CAP][64];  // chunk k of lane l at e[k][l]
 };
+constexpr uint32_t kQueueCap = 12;
 
-__device__ __forceinline__ bool queue_walk(ForceAcc &f, const Pool &P, const SceneGrid &g, const SceneParams &sp,
-                                           uint32_t i, bool fluid, const float4 ai, const float4 vi, float ci2,
-                                           const RowTab *rowtab, uint32_t stride, QueueLds &q) {
+// Lists the lane's chunks (returns their count; above CAP the queue is
+// incomplete) and the slot range [lo, hi) of its accepted neighbours in its
+// own cell row (row 4: dz = dy = 0), empty (lo >= hi) when it has none there.
+template <uint32_t CAP>
+__device__ __forceinline__ uint32_t queue_build(const Pool &P, const SceneGrid &g, const SceneParams &sp, uint32_t i,
+                                                bool fluid, const float4 ai, const RowTab *rowtab, QueueLds<CAP> &q,
+                                                uint32_t &lo, uint32_t &hi) {
     const float4 *__restrict__ sA = P.sA;
-    const float4 *__restrict__ fr = P.fr;
-    const bool own_mass = sp.fmass_uniform == 0u;
     const float h2 = sp.h2;
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t n = 0;
+    lo = 0xffffffffu;
+    hi = 0u;
     if (fluid) {
         const Cursor c = make_cursor(g, sp.h, ai.x, ai.y, ai.z);
         const uint32_t summ = P.nbs[i];
@@ -1050,12 +1055,17 @@ __device__ __forceinline__ bool queue_walk(ForceAcc &f, const Pool &P, const Sce
                 mks[r] = P.nbm[t.mask + i];
             }
         }
+        auto list = [&](uint32_t r, uint32_t m, uint32_t base) {
+            if (n < CAP) q.e[n][lane] = make_uint2(m, base);
+            ++n;
+            if (r == 4u) {
+                lo = min(lo, base + (uint32_t)__builtin_ctz(m));
+                hi = max(hi, base + 32u - (uint32_t)__builtin_clz(m));
+            }
+        };
 #pragma unroll
         for (uint32_t r = 0; r < 9u; ++r) {
-            if (mks[r] != 0u) {
-                if (n < kQueueCap) q.e[n][lane] = make_uint2(mks[r], j0s[r]);
-                ++n;
-            }
+            if (mks[r] != 0u) list(r, mks[r], j0s[r]);
             if ((summ >> (18u + r)) & 1u) {
                 // Candidates past the first 32 of the row, tested 32 at a time.
                 const uint32_t row = rowb + rowtab[r].cell;
@@ -1070,15 +1080,60 @@ __device__ __forceinline__ bool queue_walk(ForceAcc &f, const Pool &P, const Sce
                         const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                         if (r2 <= h2 && j != i) m |= 1u << k;
                     }
-                    if (m != 0u) {
-                        if (n < kQueueCap) q.e[n][lane] = make_uint2(m, jb);
-                        ++n;
-                    }
+                    if (m != 0u) list(r, m, jb);
                 }
             }
         }
     }
-    if (__any(n > kQueueCap)) return false;
+    return n;
+}
+
+// One wave's copy of the neighbour records of its lanes' own cell rows (row
+// 4): the union of the lanes' row-4 ranges, slots [slo, slo + sn), its first
+// CAP records copied with coalesced loads. The whole wave calls it.
+template <uint32_t CAP>
+__device__ __forceinline__ void stage_own_rows(float4 (*st)[2], const float4 *__restrict__ fr, uint32_t lo,
+                                               uint32_t hi, uint32_t &slo, uint32_t &sn) {
+    slo = __builtin_amdgcn_readfirstlane(wave_min_u32(lo));
+    const uint32_t shi = __builtin_amdgcn_readfirstlane(wave_max_u32(hi));
+    sn = slo < shi ? min(shi - slo, CAP) : 0u;
+    for (uint32_t k = threadIdx.x & 63u; k < 2u * sn; k += 64u) (&st[0][0])[k] = *at(fr, 2u * slo + k);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Exchange with the other lane of the lane pair (2k, 2k + 1): DPP quad_perm
+// [1, 0, 3, 2]; the whole wave must be active.
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ float4 pair_swap(const float4 v) {
+    return make_float4(__uint_as_float(pair_swap(__float_as_uint(v.x))), __uint_as_float(pair_swap(__float_as_uint(v.y))),
+                       __uint_as_float(pair_swap(__float_as_uint(v.z))), __uint_as_float(pair_swap(__float_as_uint(v.w))));
+}
+
+// How the pair loop of the queue walk reads a neighbour's density record.
+enum : uint32_t {
+    kLoadPlain = 0,   // each lane loads its record's two 16-byte halves
+    kLoadPaired = 1,  // lane pairs load both halves of one record per instruction
+    kLoadStaged = 2,  // records of the wave's stage from LDS, the others as kLoadPlain
+};
+
+// The pair loop over the lane's n listed chunks. kLoadPaired: the lanes of a
+// pair (2k, 2k + 1) load the even lane's record (half `odd` each), then the
+// odd lane's, and exchange halves through DPP, so a wave-wide load touches
+// the cache lines of 32 whole records instead of 64 half records (a lane
+// without a pair loads its own record for its partner). kLoadStaged: a
+// neighbour in the wave's stage (slots [slo, slo + sn), stage_own_rows) is
+// read from LDS. Same pairs, same order, same values in every mode.
+template <uint32_t LOAD, uint32_t CAP>
+__device__ __forceinline__ void queue_pairs(ForceAcc &f, const Pool &P, const SceneParams &sp, uint32_t i,
+                                            const float4 ai, const float4 vi, float ci2, uint32_t n,
+                                            const QueueLds<CAP> &q, const float4 (*st)[2], uint32_t slo,
+                                            uint32_t sn) {
+    const float4 *__restrict__ sA = P.sA;
+    const float4 *__restrict__ fr = P.fr;
+    const bool own_mass = sp.fmass_uniform == 0u;
+    const uint32_t lane = threadIdx.x & 63u;
     const ForceConst fc = force_const(sp);
     uint32_t e = 0, mk = 0, j0 = 0;
     if (n != 0u) {
@@ -1104,31 +1159,60 @@ __device__ __forceinline__ bool queue_walk(ForceAcc &f, const Pool &P, const Sce
         nb_record(r0, r1, j, sA, sp.mass, own_mass, aj, vj, cj);
         force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
     };
+    const uint32_t odd = lane & 1u;
+    auto load = [&](bool h, uint32_t j, float4 &r0, float4 &r1) {
+        if constexpr (LOAD == kLoadPaired) {
+            const uint32_t js = h ? j : i;
+            const uint32_t jp = pair_swap(js);
+            const float4 la = *at(fr, 2u * (odd ? jp : js) + odd);  // half `odd` of the even lane's record
+            const float4 lb = *at(fr, 2u * (odd ? js : jp) + odd);  // half `odd` of the odd lane's record
+            const float4 rv = pair_swap(odd ? la : lb);
+            r0 = odd ? rv : la;
+            r1 = odd ? lb : rv;
+        } else if constexpr (LOAD == kLoadStaged) {
+            const uint32_t k = j - slo;
+            if (h && k < sn) {
+                r0 = st[k][0];
+                r1 = st[k][1];
+            } else if (h) {
+                r0 = *at(fr, 2u * j);
+                r1 = *at(fr, 2u * j + 1u);
+            }
+        } else {
+            if (h) {
+                r0 = *at(fr, 2u * j);
+                r1 = *at(fr, 2u * j + 1u);
+            }
+        }
+    };
     // Two pairs in flight: each pair's record loads are issued one pair
     // before it is evaluated, alternating between two register sets.
     uint32_t ja = 0, jb = 0;
     float4 a0 = ai, a1 = vi, b0 = ai, b1 = vi;
     bool ha = pop(ja);
-    if (ha) {
-        a0 = *at(fr, 2u * ja);
-        a1 = *at(fr, 2u * ja + 1u);
-    }
+    load(ha, ja, a0, a1);
     for (;;) {
         if (!__any(ha)) break;
         const bool hb = pop(jb);
-        if (hb) {
-            b0 = *at(fr, 2u * jb);
-            b1 = *at(fr, 2u * jb + 1u);
-        }
+        load(hb, jb, b0, b1);
         if (ha) eval(a0, a1, ja);
         if (!__any(hb)) break;
         ha = pop(ja);
-        if (ha) {
-            a0 = *at(fr, 2u * ja);
-            a1 = *at(fr, 2u * ja + 1u);
-        }
+        load(ha, ja, a0, a1);
         if (hb) eval(b0, b1, jb);
     }
+}
+
+// Returns false (nothing evaluated) when a lane of the wave needs more than
+// kQueueCap chunks; the caller then walks the wave with walk_ahead.
+template <uint32_t LOAD>
+__device__ __forceinline__ bool queue_walk(ForceAcc &f, const Pool &P, const SceneGrid &g, const SceneParams &sp,
+                                           uint32_t i, bool fluid, const float4 ai, const float4 vi, float ci2,
+                                           const RowTab *rowtab, QueueLds<kQueueCap> &q) {
+    uint32_t lo, hi;
+    const uint32_t n = queue_build(P, g, sp, i, fluid, ai, rowtab, q, lo, hi);
+    if (__any(n > kQueueCap)) return false;
+    queue_pairs<LOAD, kQueueCap>(f, P, sp, i, ai, vi, ci2, n, q, nullptr, 0u, 0u);
     return true;
 }
 
@@ -1326,10 +1410,14 @@ __device__ __forceinline__ void pair_parallel(ForceAcc &f, const Pool &P, const 
 // WALK = 2: the same walk with the next pair's loads issued one pair ahead.
 // WALK = 3: pair-parallel rows (pair_parallel).
 // WALK = 4: the walk from a per-lane queue of row chunks (queue_walk).
+// WALK = 5: the queue walk with each record loaded by a lane pair
+// (kLoadPaired).
+// WALK = 6: the queue walk reading each wave's own cell row from LDS
+// (kLoadStaged).
 //
 // BS = threads per workgroup (64, 128 or 256, inside one pool block).
 template <uint32_t WALK, uint32_t BS>
-__global__ __launch_bounds__(BS) void k_force_t(Pool P) {
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WALK == 6 ? 6 : 1, 8))) void k_force_t(Pool P) {
     __shared__ RowTab rowtab[9];
     if (P.ctl->err) return;
     const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
@@ -1404,10 +1492,29 @@ __global__ __launch_bounds__(BS) void k_force_t(Pool P) {
             }
         } else if constexpr (WALK == 2) {
             walk_ahead(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride);
-        } else if constexpr (WALK == 4) {
-            __shared__ QueueLds queue_lds[BS / 64];
-            if (!queue_walk(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride, queue_lds[threadIdx.x >> 6]))
+        } else if constexpr (WALK == 4 || WALK == 5) {
+            __shared__ QueueLds<kQueueCap> queue_lds[BS / 64];
+            if (!queue_walk<WALK == 5 ? kLoadPaired : kLoadPlain>(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab,
+                                                                  queue_lds[threadIdx.x >> 6]))
                 walk_ahead(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride);
+        } else if constexpr (WALK == 6) {
+            // The queue walk with each wave's own cell rows staged in LDS.
+            // Nine chunks per lane and 62 staged records per wave keep a
+            // 256-thread workgroup under 26.6 KB of LDS: six per CU, six waves
+            // per SIMD.
+            constexpr uint32_t kCap = 9, kStage = 62;
+            __shared__ QueueLds<kCap> queue_lds[BS / 64];
+            __shared__ float4 stage_lds[BS / 64][kStage][2];
+            QueueLds<kCap> &q = queue_lds[threadIdx.x >> 6];
+            float4(*st)[2] = stage_lds[threadIdx.x >> 6];
+            uint32_t lo, hi, slo, sn;
+            const uint32_t n = queue_build(P, g, sp, i, fluid, ai, rowtab, q, lo, hi);
+            if (__any(n > kCap)) {
+                walk_ahead(f, P, g, sp, i, fluid, ai, vi, ci2, rowtab, stride);
+            } else {
+                stage_own_rows<kStage>(st, fr, lo, hi, slo, sn);
+                queue_pairs<kLoadStaged, kCap>(f, P, sp, i, ai, vi, ci2, n, q, st, slo, sn);
+            }
         } else {
             // Pair-parallel rows (pair_parallel) for waves whose rows all fit a
             // 32-bit mask and whose neighbours lie within 2^26 slots of the
@@ -1467,6 +1574,12 @@ template __global__ void k_force_t<4, 256>(Pool P);
 template __global__ void k_force_t<3, 64>(Pool P);
 template __global__ void k_force_t<3, 128>(Pool P);
 template __global__ void k_force_t<3, 256>(Pool P);
+template __global__ void k_force_t<5, 64>(Pool P);
+template __global__ void k_force_t<5, 128>(Pool P);
+template __global__ void k_force_t<5, 256>(Pool P);
+template __global__ void k_force_t<6, 64>(Pool P);
+template __global__ void k_force_t<6, 128>(Pool P);
+template __global__ void k_force_t<6, 256>(Pool P);
 
 // Reaction on every boundary particle of a scene with reactions:
 // bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending

@@ -281,7 +281,7 @@ VxlSphStatus vxl_sph_engine_set_tuning(VxlSphEngine *e, const VxlSphTuning *t) {
     if (!e || !t) return fail(VXL_SPH_NULL_POINTER, "null argument");
     VxlSphStatus st = check_header(t->hdr, sizeof(VxlSphTuning));
     if (st != VXL_SPH_OK) return st;
-    if (t->force_kernel > VXL_SPH_FORCE_QUEUE) return fail(VXL_SPH_INVALID_ARGUMENT, "unknown kernel variant");
+    if (t->force_kernel > VXL_SPH_FORCE_STAGED) return fail(VXL_SPH_INVALID_ARGUMENT, "unknown kernel variant");
     return guard([&] {
         Engine::Tuning tu;
         tu.persist_blocks = t->persist_blocks;

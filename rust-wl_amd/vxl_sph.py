@@ -23,7 +23,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("VXL_SPH_LIB") or os.path.join(_HERE, "libvxl_sph.so")
 ABI_VERSION = 1
 # Force kernel variants of VxlSphTuning (vxl_sph.h VXL_SPH_FORCE_*).
-FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK, FORCE_WALK_AHEAD, FORCE_PAIRS, FORCE_QUEUE = 0, 1, 2, 3, 4, 5
+FORCE_DEFAULT, FORCE_SCAN, FORCE_WALK, FORCE_WALK_AHEAD, FORCE_PAIRS, FORCE_QUEUE, FORCE_PAIRED, FORCE_STAGED = \
+    0, 1, 2, 3, 4, 5, 6, 7
 
 STATUS = {
     0: "OK", 1: "NULL_POINTER", 2: "VERSION_MISMATCH", 3: "SIZE_MISMATCH", 4: "NOT_READY",
@@ -553,7 +554,8 @@ class Engine:
 
     def set_tuning(self, persist_blocks=0, force_kernel=FORCE_DEFAULT, block_threads=0, stage_records=0,
                    scan_tile_cells=0):
-        """force_kernel: FORCE_SCAN | FORCE_WALK | FORCE_WALK_AHEAD | FORCE_PAIRS | FORCE_QUEUE; persist_blocks: grid size cap of the
+        """force_kernel: FORCE_SCAN | FORCE_WALK | FORCE_WALK_AHEAD | FORCE_PAIRS | FORCE_QUEUE |
+        FORCE_PAIRED | FORCE_STAGED; persist_blocks: grid size cap of the
         per-slot kernels (0 = one block per 256 slots); block_threads: workgroup size of the density and
         force kernels (64, 128, 256); stage_records: density LDS staging depth (128, 256, 512);
         scan_tile_cells: cells per scan tile (1024, 2048, 4096). 0 keeps a field's default. Every

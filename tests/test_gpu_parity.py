@@ -279,13 +279,16 @@ def test_reference_akinci_canary_on_gpu():
     np.testing.assert_array_equal(bits(gft), bits(ft))
 
 
-@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE])
+@pytest.mark.parametrize("force_kernel", [V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE,
+                                          V.FORCE_PAIRED, V.FORCE_STAGED])
 @pytest.mark.parametrize("persist", [0, 7])
 @pytest.mark.parametrize("budget", [1 << 21, 1 << 9])
 def test_kernel_variants_bit_exact(force_kernel, persist, budget):
     """Every autotuner choice (force: candidate scan, walk of the recorded
-    neighbours directly or with loads one pair ahead, or the pairs of each row
-    spread over the wave's lanes; one block per
+    neighbours directly or with loads one pair ahead, the pairs of each row
+    spread over the wave's lanes, the walk from a queue of row chunks, that
+    walk with records loaded by lane pairs or with each wave's own row staged
+    in LDS; one block per
     256 slots or a grid-stride loop over a few blocks) gives the oracle's state
     bit for bit. The 2^9 budget doubles the bin: cells of ~64 particles, rows
     longer than 32 candidates (tail tests in the walk) and row unions longer than
@@ -319,7 +322,8 @@ def test_tuning_knobs_bit_exact(block_threads, budget):
     want_acc, want_xs, want_hash = bits(orc.acc()), bits(orc.xsph()), orc.state_hash()
     for stage in (128, 256, 512):
         for scan in (1024, 2048, 4096):
-            for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE):
+            for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE, V.FORCE_PAIRED,
+                   V.FORCE_STAGED):
                 eng = V.Engine(diagnostics=True)
                 sid = eng.add(cfg, data, reactions=False)
                 eng.set_tuning(force_kernel=fk, block_threads=block_threads, stage_records=stage,
@@ -363,7 +367,8 @@ def test_unclamped_pressure_bit_exact():
     the oracle bit for bit."""
     cfg, data, tank, d = small_scene(seed=17, block=(14, 10, 9), tank=(0.5, 0.4, 0.4))
     cfg.tensile_clamp = 0
-    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE):
+    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE, V.FORCE_PAIRED,
+                   V.FORCE_STAGED):
         orc = oracle_dam_break(cfg, data, tank, d)
         eng = V.Engine(diagnostics=True)
         sid = eng.add(cfg, data, reactions=False)
@@ -393,7 +398,8 @@ def test_coincident_particles_bit_exact():
     near_b = nf + int(np.argmin(np.linalg.norm(data.pos[nf:] - pos[40], axis=1)))
     pos[40] = pos[near_b]
     data = V.SceneData(pos, data.vel, data.pmass, nf, data.spans)
-    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE):
+    for fk in (V.FORCE_SCAN, V.FORCE_WALK, V.FORCE_WALK_AHEAD, V.FORCE_PAIRS, V.FORCE_QUEUE, V.FORCE_PAIRED,
+                   V.FORCE_STAGED):
         orc = oracle_dam_break(cfg, data, tank, d)
         eng = V.Engine(diagnostics=True)
         sid = eng.add(cfg, data, reactions=False)
