@@ -1153,13 +1153,22 @@ __device__ __forceinline__ void queue_pairs(ForceAcc &f, const Pool &P, const Sc
         }
         return true;
     };
-    auto eval = [&](const float4 r0, const float4 r1, uint32_t j) {
+    auto eval = [&](float4 r0, float4 r1, uint32_t j) {
+        if constexpr (LOAD == kLoadStaged) {
+            // A staged neighbour (its global load read a placeholder).
+            const uint32_t k = j - slo;
+            if (k < sn) {
+                r0 = st[k][0];
+                r1 = st[k][1];
+            }
+        }
         float4 aj, vj;
         float cj;
         nb_record(r0, r1, j, sA, sp.mass, own_mass, aj, vj, cj);
         force_pair(f, ai, vi, ci2, aj, vj, cj, fc);
     };
     const uint32_t odd = lane & 1u;
+    const uint32_t i0 = __builtin_amdgcn_readfirstlane(i);
     auto load = [&](bool h, uint32_t j, float4 &r0, float4 &r1) {
         if constexpr (LOAD == kLoadPaired) {
             const uint32_t js = h ? j : i;
@@ -1170,14 +1179,13 @@ __device__ __forceinline__ void queue_pairs(ForceAcc &f, const Pool &P, const Sc
             r0 = odd ? rv : la;
             r1 = odd ? lb : rv;
         } else if constexpr (LOAD == kLoadStaged) {
-            const uint32_t k = j - slo;
-            if (h && k < sn) {
-                r0 = st[k][0];
-                r1 = st[k][1];
-            } else if (h) {
-                r0 = *at(fr, 2u * j);
-                r1 = *at(fr, 2u * j + 1u);
-            }
+            // Every lane issues the global loads in every iteration (a lane
+            // whose neighbour is staged, or that has none, reads the wave's
+            // first slot); the staged record is read from LDS when the pair
+            // is evaluated.
+            const uint32_t jg = h && j - slo >= sn ? j : i0;
+            r0 = *at(fr, 2u * jg);
+            r1 = *at(fr, 2u * jg + 1u);
         } else {
             if (h) {
                 r0 = *at(fr, 2u * j);

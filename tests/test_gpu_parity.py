@@ -346,7 +346,7 @@ def test_autotuner_picks_allowed_values_and_keeps_results():
     sid = eng.add(cfg, data, reactions=False)
     t = eng.autotune(1)
     assert t["block_threads"] in (64, 128, 256) and t["stage_records"] in (128, 256, 512)
-    assert t["scan_tile_cells"] in (1024, 2048, 4096) and t["force_kernel"] in (1, 2, 3, 4, 5)
+    assert t["scan_tile_cells"] in (1024, 2048, 4096) and t["force_kernel"] in (1, 2, 3, 4, 5, 6, 7)
     assert eng.get_tuning() == t
     # the tuning substeps advanced the scene; count them by replaying on the oracle
     orc = oracle_dam_break(cfg, data, tank, d)
