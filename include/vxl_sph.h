@@ -116,7 +116,7 @@ VxlSphStatus vxl_sph_engine_set_diagnostics(VxlSphEngine *e, int32_t on);
 typedef struct VxlSphTuning {
     VxlSphHeader hdr;
     uint32_t persist_blocks;  /* grid size cap of the per-slot kernels (0 = one block per 256 slots) */
-    uint32_t force_kernel;    /* VXL_SPH_FORCE_* (0 = VXL_SPH_FORCE_QUEUE) */
+    uint32_t force_kernel;    /* VXL_SPH_FORCE_* (0 = VXL_SPH_FORCE_STAGED) */
     uint32_t block_threads;   /* workgroup size of the density and force kernels: 64, 128, 256 (0 = 256) */
     uint32_t stage_records;   /* LDS staging depth of the density pass, records per wave and row: 128, 256, 512 (0 = 256) */
     uint32_t scan_tile_cells; /* cells per tile of the cell-count scan: 1024, 2048, 4096 (0 = 2048) */

@@ -179,7 +179,7 @@ public:
     };
     struct Tuning {
         uint32_t persist_blocks = 0;         // grid-stride grid size cap for slot kernels (0 = none)
-        uint32_t force_kernel = kForceQueue;  // kForceScan ... kForceStaged
+        uint32_t force_kernel = kForceStaged;  // kForceScan ... kForceStaged
         uint32_t block_threads = 256;        // workgroup size of the density and force kernels: 64 | 128 | 256
         uint32_t stage_records = 256;        // density LDS staging depth per wave and row: 128 | 256 | 512
         uint32_t scan_tile_cells = 2048;     // cells per scan tile: 1024 | 2048 | 4096

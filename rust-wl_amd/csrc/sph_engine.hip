@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 namespace vxl {
 
@@ -534,6 +535,8 @@ void Engine::layout() {
     d_react_ = nullptr;
     d_params_ = dalloc<SceneParams>(std::max<size_t>(scenes_.size(), 1));
     d_grid_ = dalloc<SceneGrid>(std::max<size_t>(scenes_.size(), 1));
+    // No grid until the first substep of this layout (bin = 0).
+    hip_check(hipMemset(d_grid_, 0, sizeof(SceneGrid) * std::max<size_t>(scenes_.size(), 1)), "memset grid");
     d_box_ = dalloc<SceneBox>(2 * std::max<size_t>(scenes_.size(), 1));
     d_block_scene_ = dalloc<uint32_t>(std::max<size_t>(block_scene.size(), 1));
     std::vector<SpanDev> spans;
@@ -1162,12 +1165,28 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
     if (dirty_) layout();
     if (scenes_.empty() || P_.nblocks == 0) return tuning_;
     static std::mutex mu;
-    static std::map<std::pair<std::string, uint32_t>, Tuning> cache;
+    static std::map<std::tuple<std::string, uint32_t, uint32_t>, Tuning> cache;
     hipDeviceProp_t prop;
     hip_check(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
     uint32_t bucket = 0;
     for (uint32_t v = used_slots_; v > 1; v >>= 1) ++bucket;
-    const auto key = std::make_pair(std::string(prop.gcnArchName), bucket);
+    // The grid's coarseness is part of the key: cells of many particles (a bin
+    // of several h under a small grid budget) favour other walks. It is read
+    // from the largest scene's last grid (one substep first if none ran yet).
+    uint32_t big = 0;
+    for (uint32_t k = 1; k < scenes_.size(); ++k)
+        if (scenes_[k].params.n > scenes_[big].params.n) big = k;
+    SceneGrid gb{};
+    hip_check(hipMemcpy(&gb, d_grid_ + big, sizeof(SceneGrid), hipMemcpyDeviceToHost), "grid");
+    if (!(gb.bin > 0.0f)) {
+        step(1);
+        sync();
+        hip_check(hipMemcpy(&gb, d_grid_ + big, sizeof(SceneGrid), hipMemcpyDeviceToHost), "grid");
+    }
+    const float ratio = gb.bin / scenes_[big].params.h;
+    const uint32_t coarse =
+        ratio > 1.0f ? std::min<uint32_t>(7u, (uint32_t)std::lround(std::log2((double)ratio))) : 0u;
+    const auto key = std::make_tuple(std::string(prop.gcnArchName), bucket, coarse);
     {
         std::lock_guard<std::mutex> lk(mu);
         auto it = cache.find(key);
