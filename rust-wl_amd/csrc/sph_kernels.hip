@@ -863,14 +863,22 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
             // above counts boundary neighbours too.)
             DensAcc<false> da{sum, sum_b, false};
             for_row_windows(g, cur, [&](uint32_t, uint32_t cf, uint32_t cl, bool) {
-                uint32_t lo = P.start[cf];
-                for (uint32_t c = cf; c <= cl; ++c) {
-                    const uint32_t hi = P.start[c + 1u];
-                    const uint32_t jn = lo + min(P.fcount[c], hi - lo);
+                // The window's (at most three) cell starts and fluid counts
+                // load together; most tank particles have no fluid near them
+                // and add nothing.
+                const uint32_t nc = cl - cf;  // 0..2
+                uint32_t st[4], fc[3];
+#pragma unroll
+                for (uint32_t k = 0; k < 4u; ++k) st[k] = P.start[cf + min(k, nc + 1u)];
+#pragma unroll
+                for (uint32_t k = 0; k < 3u; ++k) fc[k] = P.fcount[cf + min(k, nc)];
+#pragma unroll
+                for (uint32_t k = 0; k < 3u; ++k) {
+                    if (k > nc) break;
+                    const uint32_t lo = st[k], jn = lo + min(fc[k], st[k + 1u] - lo);
                     uint32_t unused = 0;
                     for (uint32_t jb = lo; jb < jn; jb += 4u)
                         dens_group<false, true, false>(gsrc, ai, i, jb, lo, jn, h2, k6, da, unused, cnt);
-                    lo = hi;
                 }
             });
             sum = da.sum;
