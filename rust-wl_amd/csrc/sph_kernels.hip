@@ -1718,13 +1718,16 @@ __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
 // scaled by 1/substeps at its end (fluid_step.rs:75-89).
 //
 // One wave per span. The terms (f_k and the cross products) do not depend on
-// the running sums: the lanes load 64 consecutive particles and form their
-// terms in LDS, then lane c (c < 6) adds component c of the 64 terms in index
-// order (the six sums are independent chains, each folded by its own lane)
-// while the lanes' loads of the next 64 are in flight. Same operations, same
-// order, same bits as a serial fold.
+// the running sums: the lanes load 256 consecutive particles (four each) and
+// write their terms to LDS component by component, then lane c (c < 6) adds
+// component c of the 256 terms in index order (the six sums are independent
+// chains, each folded by its own lane), reading them 64 at a time into
+// registers, while the lanes' loads of the next 256 are in flight. Same
+// operations, same order, same bits as a serial fold; the chain of dependent
+// additions is the kernel's time.
 __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
-    __shared__ float term[kReactChunk][8];  // six terms per particle, rows padded to 32 bytes
+    constexpr uint32_t kPer = 4, kFold = kPer * kReactChunk;  // particles per chunk
+    __shared__ float4 term[6][kFold / 4];                     // term c of particle u at component c, u
     if (P.ctl->err) return;
     const uint32_t k = blockIdx.x;
     if (k >= P.nspans) return;
@@ -1736,28 +1739,51 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
     const uint32_t phase = tick_sub + sub - tick_len;      // the scene's substep index inside its tick
     const uint32_t lane = threadIdx.x;
     float acc = 0.0f;  // component `lane` of (F, tau) for lanes 0..5
-    float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p = f;
-    if (sd.start + lane < sd.end) {
-        f = P.bf[sd.start + lane];
-        p = P.bp[sd.start + lane];
-    }
-    for (uint32_t q = sd.start; q < sd.end; q += kReactChunk) {
-        const float rx = p.x - sd.origin[0], ry = p.y - sd.origin[1], rz = p.z - sd.origin[2];
-        float *t = term[lane];
-        t[0] = f.x; t[1] = f.y; t[2] = f.z;
-        t[3] = ry * f.z - rz * f.y;
-        t[4] = rz * f.x - rx * f.z;
-        t[5] = rx * f.y - ry * f.x;
-        const uint32_t qn = q + kReactChunk;
-        if (qn + lane < sd.end) {
-            f = P.bf[qn + lane];
-            p = P.bp[qn + lane];
+    float4 f[kPer], p[kPer];
+    auto fetch = [&](uint32_t q) {
+#pragma unroll
+        for (uint32_t m = 0; m < kPer; ++m) {
+            const uint32_t j = q + m * kReactChunk + lane;
+            f[m] = p[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (j < sd.end) {
+                f[m] = P.bf[j];
+                p[m] = P.bp[j];
+            }
         }
+    };
+    fetch(sd.start);
+    float *tc = &term[0][0].x;
+    for (uint32_t q = sd.start; q < sd.end; q += kFold) {
+#pragma unroll
+        for (uint32_t m = 0; m < kPer; ++m) {
+            const uint32_t u = m * kReactChunk + lane;
+            const float rx = p[m].x - sd.origin[0], ry = p[m].y - sd.origin[1], rz = p[m].z - sd.origin[2];
+            tc[0 * kFold + u] = f[m].x;
+            tc[1 * kFold + u] = f[m].y;
+            tc[2 * kFold + u] = f[m].z;
+            tc[3 * kFold + u] = ry * f[m].z - rz * f[m].y;
+            tc[4 * kFold + u] = rz * f[m].x - rx * f[m].z;
+            tc[5 * kFold + u] = rx * f[m].y - ry * f[m].x;
+        }
+        fetch(q + kFold);
         __syncthreads();
         if (lane < 6u) {
-            const uint32_t nq = min(kReactChunk, sd.end - q);
-#pragma unroll 8
-            for (uint32_t u = 0; u < nq; ++u) acc = acc + term[u][lane];
+            const uint32_t nq = min(kFold, sd.end - q);
+            const float4 *row = term[lane];
+            uint32_t u = 0;
+            for (; u + 64u <= nq; u += 64u) {
+                float4 t[16];
+#pragma unroll
+                for (uint32_t v = 0; v < 16u; ++v) t[v] = row[u / 4u + v];
+#pragma unroll
+                for (uint32_t v = 0; v < 16u; ++v) {
+                    acc = acc + t[v].x;
+                    acc = acc + t[v].y;
+                    acc = acc + t[v].z;
+                    acc = acc + t[v].w;
+                }
+            }
+            for (; u < nq; ++u) acc = acc + tc[lane * kFold + u];
         }
         __syncthreads();  // the terms are read before the next chunk's are written
     }
