@@ -1608,7 +1608,9 @@ template __global__ void k_force_t<6, 256>(Pool P);
 // slot), then evaluates and subtracts them in that order, and passes again
 // while any were left out. Any neighbour count gives the reference's sum bit
 // for bit; one pass covers the usual case (a wall particle sees about half a
-// neighbourhood). kBforceThreads slots per workgroup, inside one pool block.
+// neighbourhood). Only each cell's fluid head is scanned, so a dry tank
+// particle tests no candidate. kBforceThreads slots per workgroup, inside one
+// pool block.
 __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
     __shared__ uint32_t skey[kReactBatch][kBforceThreads];   // original index, ascending
     __shared__ uint32_t sslot[kReactBatch][kBforceThreads];  // its slot
@@ -1644,36 +1646,41 @@ __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
         for (;;) {
             uint32_t cnt = 0;
             bool more = false;
-            for_rows(g, P.start, cur, [&](uint32_t, bool, uint32_t i0, uint32_t i1) {
-                for (uint32_t ib = i0; ib < i1; ib += 4u) {
-                    // four candidates' records in flight at once (the grid-ordered
-                    // array is padded past its end)
-                    float4 a4[4];
+            // Fluid neighbours only: each cell's fluid head (its members in
+            // ascending original index put the fluid prefix first).
+            for_row_windows(g, cur, [&](uint32_t, uint32_t cf, uint32_t cl, bool) {
+                for (uint32_t c = cf; c <= cl; ++c) {
+                    const uint32_t i0 = P.start[c], i1 = i0 + min(P.fcount[c], P.start[c + 1u] - i0);
+                    for (uint32_t ib = i0; ib < i1; ib += 4u) {
+                        // four candidates' records in flight at once (the grid-ordered
+                        // array is padded past its end)
+                        float4 a4[4];
 #pragma unroll
-                    for (uint32_t u = 0; u < 4u; ++u) a4[u] = *at(P.sA, ib + u);
+                        for (uint32_t u = 0; u < 4u; ++u) a4[u] = *at(P.sA, ib + u);
 #pragma unroll
-                    for (uint32_t u = 0; u < 4u; ++u) {
-                        const uint32_t i = ib + u;
-                        const float4 ai = a4[u];
-                        if (i >= i1 || !(ai.w > 0.0f)) continue;  // fluid neighbours only
-                        const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
-                        const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        if (!(r2 <= h2)) continue;
-                        const uint32_t gi = P.sg[i];
-                        if (!first && gi <= last) continue;  // subtracted by an earlier pass
-                        if (cnt == kReactBatch) {
-                            more = true;
-                            if (gi > skey[kReactBatch - 1][lane]) continue;  // a later pass takes it
+                        for (uint32_t u = 0; u < 4u; ++u) {
+                            const uint32_t i = ib + u;
+                            const float4 ai = a4[u];
+                            if (i >= i1 || !(ai.w > 0.0f)) continue;  // fluid neighbours only
+                            const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
+                            const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            if (!(r2 <= h2)) continue;
+                            const uint32_t gi = P.sg[i];
+                            if (!first && gi <= last) continue;  // subtracted by an earlier pass
+                            if (cnt == kReactBatch) {
+                                more = true;
+                                if (gi > skey[kReactBatch - 1][lane]) continue;  // a later pass takes it
+                            }
+                            // insertion by original index (a full batch drops its largest)
+                            uint32_t k = cnt < kReactBatch ? cnt++ : kReactBatch - 1u;
+                            while (k > 0 && skey[k - 1][lane] > gi) {
+                                skey[k][lane] = skey[k - 1][lane];
+                                sslot[k][lane] = sslot[k - 1][lane];
+                                --k;
+                            }
+                            skey[k][lane] = gi;
+                            sslot[k][lane] = i;
                         }
-                        // insertion by original index (a full batch drops its largest)
-                        uint32_t k = cnt < kReactBatch ? cnt++ : kReactBatch - 1u;
-                        while (k > 0 && skey[k - 1][lane] > gi) {
-                            skey[k][lane] = skey[k - 1][lane];
-                            sslot[k][lane] = sslot[k - 1][lane];
-                            --k;
-                        }
-                        skey[k][lane] = gi;
-                        sslot[k][lane] = i;
                     }
                 }
             });
