@@ -361,6 +361,41 @@ def test_autotuner_picks_allowed_values_and_keeps_results():
     assert eng.state_hash(sid) == orc.state_hash()
 
 
+def test_autotune_cache_keys_the_grid_coarseness():
+    """The autotuner's cache is keyed by the grid's coarseness as well as the
+    pool size: a scene of the same size under a small grid budget (bin 2h,
+    cells of ~64 particles, where other walks win) is tuned afresh rather than
+    handed the bin = h choice, and a second such scene takes the cached
+    choice. Tuning advances a scene by real substeps (one substep when the
+    choice comes from the cache, to read the grid); the substeps are counted
+    by stepping an untuned copy of the scene until its state hash matches."""
+    def steps_taken(eng, sid, cfg, data):
+        ref = V.Engine()
+        sr = ref.add(cfg, data, reactions=False)
+        k = 0
+        while ref.state_hash(sr) != eng.state_hash(sid) and k < 400:
+            ref.step(1)
+            k += 1
+        ref.close()
+        return k
+
+    cfg, data, _, _ = small_scene(seed=23, block=(16, 12, 10), tank=(0.6, 0.5, 0.4))
+    fine = V.Engine()
+    fine.add(cfg, data, reactions=False)
+    fine.autotune(1)
+    cfg2, data2, _, _ = small_scene(seed=23, block=(16, 12, 10), tank=(0.6, 0.5, 0.4), max_bins=1 << 9)
+    coarse = V.Engine()
+    sc = coarse.add(cfg2, data2, reactions=False)
+    coarse.autotune(1)
+    assert steps_taken(coarse, sc, cfg2, data2) > 2
+    again = V.Engine()
+    sa = again.add(cfg2, data2, reactions=False)
+    again.autotune(1)
+    assert steps_taken(again, sa, cfg2, data2) == 1
+    for e in (fine, coarse, again):
+        e.close()
+
+
 def test_unclamped_pressure_bit_exact():
     """tensile_instability_suppression off (config.rs:16): negative pressures
     and negative c2 in the density record, every force kernel, 6 steps against
