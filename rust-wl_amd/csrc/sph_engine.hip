@@ -34,8 +34,8 @@ uint32_t cell_reserve(uint32_t cells_hint, uint64_t max_bins, bool full = false)
 uint64_t Engine::pool_bytes(uint64_t slots, uint64_t ccap, bool diagnostics, bool reactions) {
     // Per slot: cur + spare state 36 + 36, grid-ordered A + gid 20, density
     // record 32, sort 16, neighbour record 40 (alloc_pool); diagnostics 36,
-    // reaction buffers 32. Per cell: counts, start, fluid count (alloc_cells).
-    const uint64_t per = 180ull + (diagnostics ? 36ull : 0ull) + (reactions ? 32ull : 0ull);
+    // reaction buffers and slot list 36. Per cell: counts, start, fluid count (alloc_cells).
+    const uint64_t per = 180ull + (diagnostics ? 36ull : 0ull) + (reactions ? 36ull : 0ull);
     return per * slots + 16ull * kReadPad + sizeof(SceneBox) * (slots / kPartSlots + 1) + 12ull * ccap + 4ull +
            4ull * ((ccap + 2047) / 2048 + 1);
 }
@@ -334,8 +334,10 @@ void Engine::alloc_pool(uint32_t slots) {
     }
     if (want_reaction_buffers_) {
         F4(P_.bf); F4(P_.bp);
+        U(P_.rlist);
     } else {
         P_.bf = P_.bp = nullptr;
+        P_.rlist = nullptr;
     }
 }
 
@@ -1087,8 +1089,9 @@ void Engine::enqueue_substep(uint32_t tick_sub, uint32_t tick_len, const std::fu
     hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
     M("box_reduce");
     if (nspans_) {
-        hipLaunchKernelGGL(k_bforce, dim3(grid_slots * (kBlock / kBforceThreads)), dim3(kBforceThreads), 0, stream_,
-                           P_);
+        hipLaunchKernelGGL(k_bforce_list, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(k_bforce, dim3(std::min<uint32_t>(grid_slots * (kBlock / kBforceThreads), 4096u)),
+                           dim3(kBforceThreads), 0, stream_, P_);
         M("bforce");
         hipLaunchKernelGGL(k_reactions, dim3(nspans_), dim3(kReactChunk), 0, stream_, P_);
         M("reactions");

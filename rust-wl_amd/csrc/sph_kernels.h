@@ -58,6 +58,7 @@ struct Pool {
     uint32_t nspans;   // reaction spans (0 = no reaction pass)
     // reactions (index order, pool slots): boundary force and position
     float4 *bf, *bp;
+    uint32_t *rlist;   // [slots] the grid slots of this substep's reacting boundary particles (ctl->nreact)
     SpanDev *spans;    // torque origins of dynamic bodies follow their pose
     float *react;      // [nspans][6]: F, tau (tick accumulator)
     // dynamic rigid bodies (all scenes) and their body-frame Akinci lattices
@@ -108,6 +109,7 @@ template <uint32_t WALK, uint32_t BS>
 __global__ void k_force_t(Pool P);    // 0: candidate scan; 1: walk of the density pass's masks;
                                       // 2: the walk, loads one pair ahead; 3: pair-parallel rows;
                                       // 4: queue walk; 5: queue walk, own rows staged in LDS
+__global__ void k_bforce_list(Pool P);  // lists the boundary slots the reaction pass works on
 __global__ void k_bforce(Pool P);
 // Provider walls (scenes with WallSceneDev::mode != kWallsOff):
 // k_wall_band + k_wall_density after the density pass (table scenes: the

@@ -332,6 +332,7 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P, uint32_t tick_sub, ui
             P.ctl->step += 1u;
         P.ctl->tick_sub = tick_sub;
         P.ctl->tick_len = tick_len;
+        P.ctl->nreact = 0u;
     }
 }
 
@@ -1611,29 +1612,68 @@ template __global__ void k_force_t<6, 256>(Pool P);
 // neighbourhood). Only each cell's fluid head is scanned, so a dry tank
 // particle tests no candidate. kBforceThreads slots per workgroup, inside one
 // pool block.
+// The boundary particles whose reaction this substep needs (every boundary
+// particle of a scene with reactions on all spans, the dynamic bodies'
+// particles of a coupled scene), listed by grid slot with one atomic per wave,
+// so the reaction pass below runs on dense waves: a dynamic body's particles
+// are scattered over the fluid's slots.
+__global__ __launch_bounds__(kBlock) void k_bforce_list(Pool P) {
+    if (P.ctl->err) return;
+    const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
+    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
+        const uint32_t s = P.block_scene[b];
+        const SceneParams &sp = P.params[s];
+        if (!sp.react || !substep_active(tick_sub, tick_len, sp.substeps)) continue;  // uniform per block
+        const uint32_t j = b * kBlock + threadIdx.x;
+        bool want = false;
+        if (j < sp.pbase + sp.n) {
+            const uint32_t gj = P.sg[j];
+            want = gj >= sp.nf;  // boundary particles follow the fluid prefix
+            if (want && sp.react == kReactDynamic) {  // only dynamic bodies consume a reaction
+                bool dyn = false;
+                for (uint32_t k = sp.rbody0; k < sp.rbody0 + sp.nrbody; ++k)
+                    dyn = dyn || (gj >= P.rigid[k].gstart && gj < P.rigid[k].gend);
+                want = dyn;
+            }
+        }
+        const uint64_t m = __ballot(want);
+        if (m == 0ull) continue;
+        uint32_t base = 0;
+        if (lane == 0u) base = atomicAdd(&P.ctl->nreact, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0, 64);
+        if (want) P.rlist[base + (uint32_t)__popcll(m & below)] = j;
+    }
+}
+
+// Reaction on every listed boundary particle:
+// bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending
+// index, X = d * ((m_i * (coef + cv)) / denom) with d = p_i - p_j (the
+// reference accumulates it from the fluid side, i ascending:
+// fluid_force.rs:196-206). The neighbours arrive in cell order, so they are
+// subtracted in batches: each pass over the candidates keeps, sorted by
+// original index, the kReactBatch smallest indices above the last one already
+// subtracted (insertion into the lane's column of an LDS table of index and
+// slot), then evaluates and subtracts them in that order, and passes again
+// while any were left out. Any neighbour count gives the reference's sum bit
+// for bit; one pass covers the usual case (a wall particle sees about half a
+// neighbourhood). Only each cell's fluid head is scanned, so a dry tank
+// particle tests no candidate. A grid-stride loop over the list
+// (kBforceThreads entries per workgroup and round).
 __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
     __shared__ uint32_t skey[kReactBatch][kBforceThreads];   // original index, ascending
     __shared__ uint32_t sslot[kReactBatch][kBforceThreads];  // its slot
     if (P.ctl->err) return;
-    const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
-    constexpr uint32_t kSub = kBlock / kBforceThreads;
-    const uint32_t nsub = P.nblocks * kSub;
+    const uint32_t nlist = P.ctl->nreact;
     const uint32_t lane = threadIdx.x;
-    for (uint32_t hb = blockIdx.x; hb < nsub; hb += gridDim.x) {
-        const uint32_t b = gridDim.x == nsub ? xcd_block(hb, nsub) : hb;
-        const uint32_t s = P.block_scene[b / kSub];
+    for (uint32_t q0 = blockIdx.x * kBforceThreads; q0 < nlist; q0 += gridDim.x * kBforceThreads) {
+        const uint32_t qi = q0 + lane;
+        if (qi >= nlist) continue;
+        const uint32_t j = P.rlist[qi];
+        const uint32_t s = P.block_scene[j / kBlock];
         const SceneParams &sp = P.params[s];
-        if (!sp.react || !substep_active(tick_sub, tick_len, sp.substeps)) continue;
-        const uint32_t j = b * kBforceThreads + lane;
-        if (j >= sp.pbase + sp.n) continue;
         const uint32_t gj = P.sg[j];
-        if (gj < sp.nf) continue;  // a fluid particle (the index prefix)
-        if (sp.react == kReactDynamic) {  // only dynamic bodies consume a reaction
-            bool dyn = false;
-            for (uint32_t k = sp.rbody0; k < sp.rbody0 + sp.nrbody; ++k)
-                dyn = dyn || (gj >= P.rigid[k].gstart && gj < P.rigid[k].gend);
-            if (!dyn) continue;
-        }
         const float4 aj = P.sA[j];
         const SceneGrid &g = P.grid[s];
         const float4 vj = P.fr[2u * j + 1u];

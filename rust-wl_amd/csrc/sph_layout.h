@@ -183,7 +183,8 @@ struct Control {
     uint32_t err;        // nonzero: the substep was abandoned (see kErr*)
     uint32_t tick_sub;   // launch index inside the pool tick (set by k_grid_box)
     uint32_t tick_len;   // launches per pool tick: the most substeps of any scene (0: every launch steps every scene)
-    uint32_t pad[3];
+    uint32_t nreact;     // boundary slots listed for this substep's reaction pass (k_bforce_list)
+    uint32_t pad[2];
 };
 
 // Scenes of one pool may differ in substeps per tick. A pool tick runs
