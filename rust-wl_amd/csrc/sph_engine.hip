@@ -1186,9 +1186,10 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
         sync();
         hip_check(hipMemcpy(&gb, d_grid_ + big, sizeof(SceneGrid), hipMemcpyDeviceToHost), "grid");
     }
-    const float ratio = gb.bin / scenes_[big].params.h;
+    const float hb = scenes_[big].params.h;
+    const float ratio = hb > 0.0f ? gb.bin / hb : 1.0f;  // (h = 0: the grid rule's 1e-6 bin)
     const uint32_t coarse =
-        ratio > 1.0f ? std::min<uint32_t>(7u, (uint32_t)std::lround(std::log2((double)ratio))) : 0u;
+        ratio > 1.0f && ratio < 1e30f ? std::min<uint32_t>(7u, (uint32_t)std::lround(std::log2((double)ratio))) : 0u;
     const auto key = std::make_tuple(std::string(prop.gcnArchName), bucket, coarse);
     {
         std::lock_guard<std::mutex> lk(mu);
