@@ -1089,11 +1089,11 @@ void Engine::enqueue_substep(uint32_t tick_sub, uint32_t tick_len, const std::fu
     hipLaunchKernelGGL(k_box_reduce, dim3(box_reduce_grid()), dim3(kBlock), 0, stream_, P_);
     M("box_reduce");
     if (nspans_) {
-        hipLaunchKernelGGL(k_bforce_list, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+        hipLaunchKernelGGL(k_bforce_list, dim3((nb + kListBlocks - 1) / kListBlocks), dim3(kBlock), 0, stream_, P_);
         hipLaunchKernelGGL(k_bforce, dim3(std::min<uint32_t>(grid_slots * (kBlock / kBforceThreads), 4096u)),
                            dim3(kBforceThreads), 0, stream_, P_);
         M("bforce");
-        hipLaunchKernelGGL(k_reactions, dim3(nspans_), dim3(kReactChunk), 0, stream_, P_);
+        hipLaunchKernelGGL(k_reactions, dim3(nspans_), dim3(kReactThreads), 0, stream_, P_);
         M("reactions");
     }
     hip_check(hipGetLastError(), "substep launch");

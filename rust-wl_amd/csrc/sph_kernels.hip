@@ -1598,52 +1598,66 @@ template __global__ void k_force_t<6, 64>(Pool P);
 template __global__ void k_force_t<6, 128>(Pool P);
 template __global__ void k_force_t<6, 256>(Pool P);
 
-// Reaction on every boundary particle of a scene with reactions:
-// bforce[j] = 0 - X_1 - X_2 - ... over its fluid neighbours i in ascending
-// index, X = d * ((m_i * (coef + cv)) / denom) with d = p_i - p_j (the
-// reference accumulates it from the fluid side, i ascending:
-// fluid_force.rs:196-206). The neighbours arrive in cell order, so they are
-// subtracted in batches: each pass over the candidates keeps, sorted by
-// original index, the kReactBatch smallest indices above the last one already
-// subtracted (insertion into the lane's column of an LDS table of index and
-// slot), then evaluates and subtracts them in that order, and passes again
-// while any were left out. Any neighbour count gives the reference's sum bit
-// for bit; one pass covers the usual case (a wall particle sees about half a
-// neighbourhood). Only each cell's fluid head is scanned, so a dry tank
-// particle tests no candidate. kBforceThreads slots per workgroup, inside one
-// pool block.
 // The boundary particles whose reaction this substep needs (every boundary
 // particle of a scene with reactions on all spans, the dynamic bodies'
-// particles of a coupled scene), listed by grid slot with one atomic per wave,
-// so the reaction pass below runs on dense waves: a dynamic body's particles
-// are scattered over the fluid's slots.
+// particles of a coupled scene), listed by grid slot so the reaction pass
+// below runs on dense waves: a dynamic body's particles, and the wall
+// particles of the tank's x walls, are scattered over the fluid's slots. The
+// list's order is free (each entry writes its own reaction). Each workgroup
+// takes kListBlocks consecutive pool blocks: a first sweep keeps every wave's
+// ballot per block in LDS, one atomic per workgroup reserves the workgroup's
+// range of the list, and each wave writes its entries into its own part of
+// that range. (One atomic per wave on the single counter serialised about
+// 50,000 atomics per substep on the 10M tank.)
 __global__ __launch_bounds__(kBlock) void k_bforce_list(Pool P) {
+    constexpr uint32_t kWaves = kBlock / 64u;
+    __shared__ uint64_t mk[kListBlocks][kWaves];
+    __shared__ uint32_t wcnt[kWaves];
+    __shared__ uint32_t wbase;
     if (P.ctl->err) return;
     const uint32_t tick_sub = P.ctl->tick_sub, tick_len = P.ctl->tick_len;
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
-    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
+    const uint32_t b0 = blockIdx.x * kListBlocks, nb = min(kListBlocks, P.nblocks - min(b0, P.nblocks));
+    uint32_t cnt = 0;  // this wave's entries (wave-uniform)
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint32_t b = b0 + k;
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
-        if (!sp.react || !substep_active(tick_sub, tick_len, sp.substeps)) continue;  // uniform per block
-        const uint32_t j = b * kBlock + threadIdx.x;
         bool want = false;
-        if (j < sp.pbase + sp.n) {
-            const uint32_t gj = P.sg[j];
-            want = gj >= sp.nf;  // boundary particles follow the fluid prefix
-            if (want && sp.react == kReactDynamic) {  // only dynamic bodies consume a reaction
-                bool dyn = false;
-                for (uint32_t k = sp.rbody0; k < sp.rbody0 + sp.nrbody; ++k)
-                    dyn = dyn || (gj >= P.rigid[k].gstart && gj < P.rigid[k].gend);
-                want = dyn;
+        if (sp.react && substep_active(tick_sub, tick_len, sp.substeps)) {  // uniform per block
+            const uint32_t j = b * kBlock + threadIdx.x;
+            if (j < sp.pbase + sp.n) {
+                const uint32_t gj = P.sg[j];
+                want = gj >= sp.nf;  // boundary particles follow the fluid prefix
+                if (want && sp.react == kReactDynamic) {  // only dynamic bodies consume a reaction
+                    bool dyn = false;
+                    for (uint32_t r = sp.rbody0; r < sp.rbody0 + sp.nrbody; ++r)
+                        dyn = dyn || (gj >= P.rigid[r].gstart && gj < P.rigid[r].gend);
+                    want = dyn;
+                }
             }
         }
         const uint64_t m = __ballot(want);
+        if (lane == 0u) mk[k][wave] = m;
+        cnt += (uint32_t)__popcll(m);
+    }
+    if (lane == 0u) wcnt[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) tot += wcnt[w];
+        wbase = tot ? atomicAdd(&P.ctl->nreact, tot) : 0u;
+    }
+    __syncthreads();
+    if (cnt == 0u) return;
+    uint32_t base = wbase;
+    for (uint32_t w = 0; w < wave; ++w) base += wcnt[w];
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint64_t m = mk[k][wave];
         if (m == 0ull) continue;
-        uint32_t base = 0;
-        if (lane == 0u) base = atomicAdd(&P.ctl->nreact, (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, 0, 64);
-        if (want) P.rlist[base + (uint32_t)__popcll(m & below)] = j;
+        if ((m >> lane) & 1ull) P.rlist[base + (uint32_t)__popcll(m & below)] = (b0 + k) * kBlock + threadIdx.x;
+        base += (uint32_t)__popcll(m);
     }
 }
 
@@ -1764,17 +1778,23 @@ __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
 // (fluid_boundary.rs:10-31), accumulated over the substeps of a tick and
 // scaled by 1/substeps at its end (fluid_step.rs:75-89).
 //
-// One wave per span. The terms (f_k and the cross products) do not depend on
-// the running sums: the lanes load 256 consecutive particles (four each) and
-// write their terms to LDS component by component, then lane c (c < 6) adds
-// component c of the 256 terms in index order (the six sums are independent
-// chains, each folded by its own lane), reading them 64 at a time into
-// registers, while the lanes' loads of the next 256 are in flight. Same
-// operations, same order, same bits as a serial fold; the chain of dependent
-// additions is the kernel's time.
-__global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
-    constexpr uint32_t kPer = 4, kFold = kPer * kReactChunk;  // particles per chunk
-    __shared__ float4 term[6][kFold / 4];                     // term c of particle u at component c, u
+// One workgroup per span. The terms (f_k and the cross products) do not
+// depend on the running sums, so waves 1..3 compute them, kFold particles per
+// chunk, into one of two LDS buffers (component-major rows, padded by one
+// float4 so the six rows start in different banks), while lane c (c < 6) of
+// wave 0 adds component c of the other buffer's terms in index order: six
+// independent chains of dependent additions, fed from LDS 64 terms at a time.
+// Same operations, same order, same bits as a serial fold; the chain is the
+// kernel's time (about 2.9 ns per addition on MI355X, against 2.6 ns for
+// operands already in registers), and the terms' loads and products run
+// beside it. A chunk (1536 particles, ~4.5 us of additions) outlasts the
+// latency of the producers' loads, issued one chunk ahead; with 384 the
+// producers waited on them and a chunk took 1.6 us instead of 1.1.
+__global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
+    constexpr uint32_t kProd = kReactThreads - 64u;          // producer lanes
+    constexpr uint32_t kPer = 8, kFold = kPer * kProd;        // particles per chunk
+    constexpr uint32_t kRow = kFold / 4u + 1u;                // float4 per component row
+    __shared__ float4 term[2][6][kRow];
     if (P.ctl->err) return;
     const uint32_t k = blockIdx.x;
     if (k >= P.nspans) return;
@@ -1784,13 +1804,14 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
     const uint32_t tick_sub = P.ctl->tick_sub, tick_len = max(P.ctl->tick_len, sub);
     if (!substep_active(tick_sub, tick_len, sub)) return;  // the scene does not step in this launch
     const uint32_t phase = tick_sub + sub - tick_len;      // the scene's substep index inside its tick
-    const uint32_t lane = threadIdx.x;
-    float acc = 0.0f;  // component `lane` of (F, tau) for lanes 0..5
+    const bool folder = threadIdx.x < 64u;
+    const uint32_t lane = threadIdx.x, t = threadIdx.x - 64u;  // t: producer index (waves 1..3)
+    float acc = 0.0f;  // component `lane` of (F, tau) for lanes 0..5 of wave 0
     float4 f[kPer], p[kPer];
     auto fetch = [&](uint32_t q) {
 #pragma unroll
         for (uint32_t m = 0; m < kPer; ++m) {
-            const uint32_t j = q + m * kReactChunk + lane;
+            const uint32_t j = q + m * kProd + t;
             f[m] = p[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (j < sd.end) {
                 f[m] = P.bf[j];
@@ -1798,41 +1819,53 @@ __global__ __launch_bounds__(kReactChunk) void k_reactions(Pool P) {
             }
         }
     };
-    fetch(sd.start);
-    float *tc = &term[0][0].x;
-    for (uint32_t q = sd.start; q < sd.end; q += kFold) {
+    auto put = [&](uint32_t buf) {
+        float *tc = &term[buf][0][0].x;
 #pragma unroll
         for (uint32_t m = 0; m < kPer; ++m) {
-            const uint32_t u = m * kReactChunk + lane;
+            const uint32_t u = m * kProd + t;
             const float rx = p[m].x - sd.origin[0], ry = p[m].y - sd.origin[1], rz = p[m].z - sd.origin[2];
-            tc[0 * kFold + u] = f[m].x;
-            tc[1 * kFold + u] = f[m].y;
-            tc[2 * kFold + u] = f[m].z;
-            tc[3 * kFold + u] = ry * f[m].z - rz * f[m].y;
-            tc[4 * kFold + u] = rz * f[m].x - rx * f[m].z;
-            tc[5 * kFold + u] = rx * f[m].y - ry * f[m].x;
+            tc[0 * 4u * kRow + u] = f[m].x;
+            tc[1 * 4u * kRow + u] = f[m].y;
+            tc[2 * 4u * kRow + u] = f[m].z;
+            tc[3 * 4u * kRow + u] = ry * f[m].z - rz * f[m].y;
+            tc[4 * 4u * kRow + u] = rz * f[m].x - rx * f[m].z;
+            tc[5 * 4u * kRow + u] = rx * f[m].y - ry * f[m].x;
         }
-        fetch(q + kFold);
-        __syncthreads();
-        if (lane < 6u) {
-            const uint32_t nq = min(kFold, sd.end - q);
-            const float4 *row = term[lane];
-            uint32_t u = 0;
-            for (; u + 64u <= nq; u += 64u) {
-                float4 t[16];
+    };
+    if (!folder) {
+        fetch(sd.start);
+        put(0);
+        fetch(sd.start + kFold);
+    }
+    __syncthreads();
+    uint32_t buf = 0;
+    for (uint32_t q = sd.start; q < sd.end; q += kFold, buf ^= 1u) {
+        if (folder) {
+            if (lane < 6u) {
+                const uint32_t nq = min(kFold, sd.end - q);
+                const float4 *row = term[buf][lane];
+                uint32_t u = 0;
+                for (; u + 64u <= nq; u += 64u) {
+                    float4 tv[16];
 #pragma unroll
-                for (uint32_t v = 0; v < 16u; ++v) t[v] = row[u / 4u + v];
+                    for (uint32_t v = 0; v < 16u; ++v) tv[v] = row[u / 4u + v];
 #pragma unroll
-                for (uint32_t v = 0; v < 16u; ++v) {
-                    acc = acc + t[v].x;
-                    acc = acc + t[v].y;
-                    acc = acc + t[v].z;
-                    acc = acc + t[v].w;
+                    for (uint32_t v = 0; v < 16u; ++v) {
+                        acc = acc + tv[v].x;
+                        acc = acc + tv[v].y;
+                        acc = acc + tv[v].z;
+                        acc = acc + tv[v].w;
+                    }
                 }
+                const float *rowf = &row[0].x;
+                for (; u < nq; ++u) acc = acc + rowf[u];
             }
-            for (; u < nq; ++u) acc = acc + tc[lane * kFold + u];
+        } else if (q + kFold < sd.end) {
+            put(buf ^ 1u);  // the next chunk, into the buffer folded in the previous iteration
+            fetch(q + 2u * kFold);
         }
-        __syncthreads();  // the terms are read before the next chunk's are written
+        __syncthreads();  // this chunk is folded and the next one written
     }
     if (lane >= 6u) return;
     float *o = P.react + 6ull * k + lane;
