@@ -11,9 +11,11 @@ resident in HBM. Prints ONE JSON line.
 value = fluid particle-updates/s (fluid particles x steps / wall time of the
 timed region, bracketed by device synchronisation). roofline = the dominant
 kernel's algorithmic bytes per launch / its mean HIP-event duration against the
-8 TB/s HBM3E peak. cpu_baseline = the oracle (a strict-f32 C++ restatement of
-the reference CPU step) timed on this machine's host cores on a bounded sample
-of the same scene.
+8 TB/s HBM3E peak; issue_roofline = the two neighbour passes' vector-ALU
+instructions per launch (rocprofv3 counters) / their durations against the
+FP32 issue rate measured on the same device. cpu_baseline = the oracle (a
+strict-f32 C++ restatement of the reference CPU step) timed on this
+machine's host cores on a bounded sample of the same scene.
 """
 import argparse
 import json
@@ -39,6 +41,49 @@ STAGE_BYTES = {
     "density": dict(N=20, C=4),    # R pos 12 + pmass 4, W dens 4; R start
     "force": dict(NF=60, C=4),     # R pos, vel, pmass, dens 32 + item 4, W pos, vel 24; R start
 }
+
+
+def issue_rate(kern, tuning, args):
+    """The neighbour passes against the device's FP32 vector-ALU issue rate:
+    their wave-wide VALU instructions per launch (SQ_INSTS_VALU from the
+    committed rocprofv3 counter pass, default workload only) over their
+    HIP-event times, per SIMD per second, against the rate that
+    rust-wl_amd/valu_rate (independent FMA chains) measures on this device
+    now. Both passes are bound by instruction issue, not by HBM."""
+    path = os.path.join(ROOT, "profiles", "wcsph_10M_valu.json")
+    exe = os.path.join(ROOT, "rust-wl_amd", "valu_rate")
+    if args.block != 215 or args.budget_log2 != 22 or not os.path.exists(path) or not os.path.exists(exe):
+        return None
+    import subprocess
+
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        raise RuntimeError(f"valu_rate failed ({r.returncode}): {r.stderr.strip()}")
+    cal = json.loads(r.stdout.strip().splitlines()[-1])
+    with open(path) as f:
+        counts = json.load(f)
+    entries, probe = counts["entries"], counts["probe"]
+    keys = ("force_kernel", "block_threads", "stage_records")
+    counted = next((e for e in entries if all(e["tuning"][k] == tuning[k] for k in keys)), None)
+    if counted is None:
+        return None  # no counter pass for this launch geometry
+    peak = cal["fma_wave_instr_per_simd_s"]
+    out = {"bound": "valu", "unit": "wave-instr/SIMD/s", "peak": round(peak), "peak_kernel_ms": cal["kernel_ms"],
+           "simds": cal["simds"], "peak_per_cycle": probe["valu_per_simd_per_cycle"],
+           "counted_tuning": counted["tuning"], "kernels": {}}
+    for name, c in counted["kernels"].items():
+        if name in kern and kern[name] > 0:
+            ach = c["valu_wave_instr"] / (kern[name] * 1e-3) / cal["simds"]
+            # per clock cycle (the counter pass's own cycles, kernel and probe
+            # alike): the issue rate without the clock's dependence on load
+            pc = c["valu_per_simd_per_cycle"]
+            out["kernels"][name] = {"kernel": c["kernel"], "valu_wave_instr": c["valu_wave_instr"],
+                                    "achieved": round(ach), "frac": round(ach / peak, 4),
+                                    "per_cycle": pc, "frac_per_cycle": round(pc / probe["valu_per_simd_per_cycle"], 4)}
+    print("# vector-ALU issue: peak %.3e wave-instr/SIMD/s; " % peak
+          + ", ".join(f"{k} {v['frac']:.0%} ({v['frac_per_cycle']:.0%} per cycle)" for k, v in out["kernels"].items()),
+          file=sys.stderr, flush=True)
+    return out
 
 
 def stage_bytes(name, N, NF, C):
@@ -327,6 +372,8 @@ def main():
     ap.add_argument("--stepper-ticks", type=int, default=20)
     ap.add_argument("--walls-steps", type=int, default=200)
     ap.add_argument("--walls-table-ticks", type=int, default=5)
+    ap.add_argument("--no-issue", action="store_true",
+                    help="skip the vector-ALU issue-rate measurement (rust-wl_amd/valu_rate) of the neighbour passes")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run one plain device copy of BYTES (scale for the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -474,6 +521,7 @@ def main():
     served = None if args.no_serve else bench_served(V, args)
     walls = None if args.no_walls else bench_walls(V, args)
 
+    issue = None if args.no_issue else issue_rate(kern, tuning, args)
     cpu = None
     if not args.no_cpu:
         threads = cpu_threads()
@@ -513,6 +561,7 @@ def main():
                      "traffic": measured_traffic(dom, args),
                      "algorithmic_bytes": dom_bytes, "kernel_ms": round(kern[dom], 4)},
         "cpu_baseline": cpu,
+        "issue_roofline": issue,
         "step_latency_ms": {"p50": p50, "p99": p99},
         "step_roofline": {"algorithmic_bytes_per_step": b_step, "achieved_GBs": round(step_gbs, 1),
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
