@@ -163,6 +163,33 @@ def test_reactions_match_oracle():
     assert_state_equal(pos, orc.pos(), "position", cfg.smoothing_radius)
 
 
+def test_reactions_of_a_larger_tank_match_oracle():
+    """Reactions on every span of a tank large enough that the reacting-slot
+    list is filled by many workgroups (each reserving its range of the list
+    with one atomic), and that the span fold runs over several chunks of its
+    producer/consumer pipeline: 48 x 40 x 36 fluid in a 1.2 x 0.9 x 0.9 m
+    tank, ~98k particles, 2 ticks of 2 substeps, bit-exact against the
+    oracle."""
+    cfg, data, tank, d = small_scene(seed=11, block=(48, 40, 36), tank=(1.2, 0.9, 0.9))
+    cfg.substeps = 2
+    cfg.dt_tick = 1e-3
+    assert data.n > 16 * 256 * 4  # more than four workgroups of the list
+    assert max(s[3] - s[2] for s in data.spans) > 2 * 1536  # several fold chunks
+    orc = oracle_dam_break(cfg, data, tank, d, threads=8)
+    st = V.FluidStepper(cfg)
+    for t in range(2):
+        st.step(cfg.dt_tick, data.n_fluid, data.pos, data.vel, data.pmass, data.spans)
+        orc.step(cfg.dt_tick)
+        body, ft = orc.reactions()
+        got = st.reactions()
+        assert [g[0] for g in got] == body.tolist()
+        gft = np.array([np.concatenate([g[1], g[2]]) for g in got], np.float32)
+        np.testing.assert_array_equal(bits(gft), bits(ft), err_msg=f"tick {t} reactions")
+    pos, vel = st.read_state()
+    assert_state_equal(pos, orc.pos(), "position", cfg.smoothing_radius)
+    assert_state_equal(vel, orc.vel(), "velocity", cfg.sound_speed * 1e-3)
+
+
 def test_batched_scenes_equal_alone():
     """Several scenes of uneven size and physics in one pool: each result is
     bit-identical to the scene run alone and to the oracle."""
