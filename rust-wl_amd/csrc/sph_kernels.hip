@@ -530,16 +530,26 @@ __global__ __launch_bounds__(kBlock) void k_canon_gather(Pool P) {
         if (u >= sp.pbase + sp.n) continue;
         const uint2 it = P.items[u];
         const uint32_t m = it.x, g = it.y;
+        // The particle's state loads first: their latency overlaps the chain
+        // key -> cell start -> the cell's members.
+        const float4 a = P.cA[m], v = P.cV[m];
         const uint32_t c = P.key[m];
-        const uint32_t s0 = P.start[c], e0 = P.start[c + 1];
+        const uint32_t s0 = P.start[c], e0 = P.start[c + 1];  // e0 > s0: the cell holds u
         uint32_t rank = 0, nfl = 0;
-        for (uint32_t k = s0; k < e0; ++k) {
-            const uint32_t gk = P.items[k].y;
-            rank += gk < g ? 1u : 0u;
-            nfl += gk < sp.nf ? 1u : 0u;
+        for (uint32_t k = s0; k < e0; k += 4u) {
+            // four members' indices in flight at once (the last index repeats
+            // past the cell's end and is not counted)
+            uint32_t gk[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) gk[q] = P.items[min(k + q, e0 - 1u)].y;
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                const bool in = k + q < e0;
+                rank += in && gk[q] < g ? 1u : 0u;
+                nfl += in && gk[q] < sp.nf ? 1u : 0u;
+            }
         }
         const uint32_t d = s0 + rank;
-        const float4 a = P.cA[m], v = P.cV[m];
         P.sA[d] = a;
         P.sg[d] = g;
         // the position and velocity halves of the density record (the density
