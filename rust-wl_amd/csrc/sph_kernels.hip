@@ -1629,8 +1629,15 @@ __global__ __launch_bounds__(kBlock) void k_bforce_list(Pool P) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
     const uint32_t b0 = blockIdx.x * kListBlocks, nb = min(kListBlocks, P.nblocks - min(b0, P.nblocks));
+    // The workgroup's original indices, all loads in flight at once (every
+    // slot of a pool block is inside the gathered arrays).
+    uint32_t gall[kListBlocks];
+#pragma unroll
+    for (uint32_t k = 0; k < kListBlocks; ++k) gall[k] = k < nb ? P.sg[(b0 + k) * kBlock + threadIdx.x] : 0u;
     uint32_t cnt = 0;  // this wave's entries (wave-uniform)
-    for (uint32_t k = 0; k < nb; ++k) {
+#pragma unroll
+    for (uint32_t k = 0; k < kListBlocks; ++k) {
+        if (k >= nb) break;
         const uint32_t b = b0 + k;
         const uint32_t s = P.block_scene[b];
         const SceneParams &sp = P.params[s];
@@ -1638,7 +1645,7 @@ __global__ __launch_bounds__(kBlock) void k_bforce_list(Pool P) {
         if (sp.react && substep_active(tick_sub, tick_len, sp.substeps)) {  // uniform per block
             const uint32_t j = b * kBlock + threadIdx.x;
             if (j < sp.pbase + sp.n) {
-                const uint32_t gj = P.sg[j];
+                const uint32_t gj = gall[k];
                 want = gj >= sp.nf;  // boundary particles follow the fluid prefix
                 if (want && sp.react == kReactDynamic) {  // only dynamic bodies consume a reaction
                     bool dyn = false;
@@ -1713,8 +1720,18 @@ __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
             // Fluid neighbours only: each cell's fluid head (its members in
             // ascending original index put the fluid prefix first).
             for_row_windows(g, cur, [&](uint32_t, uint32_t cf, uint32_t cl, bool) {
-                for (uint32_t c = cf; c <= cl; ++c) {
-                    const uint32_t i0 = P.start[c], i1 = i0 + min(P.fcount[c], P.start[c + 1u] - i0);
+                // The window's (at most three) cell starts and fluid counts
+                // load together.
+                const uint32_t nc = cl - cf;  // 0..2
+                uint32_t st[4], fc[3];
+#pragma unroll
+                for (uint32_t k = 0; k < 4u; ++k) st[k] = P.start[cf + min(k, nc + 1u)];
+#pragma unroll
+                for (uint32_t k = 0; k < 3u; ++k) fc[k] = P.fcount[cf + min(k, nc)];
+#pragma unroll
+                for (uint32_t w = 0; w < 3u; ++w) {
+                    if (w > nc) break;
+                    const uint32_t i0 = st[w], i1 = i0 + min(fc[w], st[w + 1u] - i0);
                     for (uint32_t ib = i0; ib < i1; ib += 4u) {
                         // four candidates' records in flight at once (the grid-ordered
                         // array is padded past its end)
@@ -1749,12 +1766,25 @@ __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
                 }
             });
             // The batch in ascending original index: d = p_i - p_j as seen
-            // from the fluid particle i.
+            // from the fluid particle i. Each neighbour's records load one
+            // neighbour ahead of its evaluation.
+            float4 an = aj, vn = vj;
+            float cn = cj2;
+            if (cnt != 0u) {
+                const uint32_t i = sslot[0][lane];
+                an = *at(P.sA, i);
+                vn = *at(P.fr, 2u * i + 1u);
+                cn = at(P.fr, 2u * i)->w;
+            }
             for (uint32_t k = 0; k < cnt; ++k) {
-                const uint32_t i = sslot[k][lane];
-                const float4 ai = *at(P.sA, i);
-                const float4 vi = *at(P.fr, 2u * i + 1u);
-                const float ci2 = at(P.fr, 2u * i)->w;
+                const float4 ai = an, vi = vn;
+                const float ci2 = cn;
+                if (k + 1u < cnt) {
+                    const uint32_t i = sslot[k + 1u][lane];
+                    an = *at(P.sA, i);
+                    vn = *at(P.fr, 2u * i + 1u);
+                    cn = at(P.fr, 2u * i)->w;
+                }
                 const float ddx = ai.x - aj.x, ddy = ai.y - aj.y, ddz = ai.z - aj.z;
                 const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
                 const float rho_i = vi.w, mi = ai.w;
