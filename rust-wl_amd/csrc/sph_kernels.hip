@@ -199,6 +199,8 @@ __device__ __forceinline__ void box_merge(SceneBox *o, const uint32_t lo[3], con
 }
 
 __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
+    __shared__ uint32_t wsc[kBlock / 64];  // scene each wave folded (~0u none, ~1u several)
+    __shared__ uint32_t wlo[kBlock / 64][3], whi[kBlock / 64][3];
     if (P.ctl->err) return;
     const uint32_t par = P.ctl->step & 1u;
     SceneBox *out = P.box + par * P.nscenes;
@@ -207,9 +209,20 @@ __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
         uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
         uint32_t cur = 0xffffffffu;  // scene of the running fold (none yet)
         const uint32_t w0 = base + threadIdx.x * kBoxChunk;
-        for (uint32_t k = 0; k < kBoxChunk && w0 + k < npart; ++k) {
-            const uint32_t w = w0 + k;
-            const uint32_t sc = P.block_scene[w / (kBlock / kPartSlots)];
+        // The thread's partial boxes and their scenes, all loads in flight at
+        // once (past the end: no scene, nothing folded).
+        SceneBox qs[kBoxChunk];
+        uint32_t scs[kBoxChunk];
+#pragma unroll
+        for (uint32_t k = 0; k < kBoxChunk; ++k) {
+            const uint32_t w = min(w0 + k, npart - 1u);
+            qs[k] = P.part[w];
+            scs[k] = w0 + k < npart ? P.block_scene[w / (kBlock / kPartSlots)] : 0xffffffffu;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBoxChunk; ++k) {
+            const uint32_t sc = scs[k];
+            if (sc == 0xffffffffu) break;
             if (sc != cur) {
                 if (cur != 0xffffffffu) box_merge(&out[cur], lo, hi);
                 cur = sc;
@@ -218,22 +231,48 @@ __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
                     hi[a] = 0u;
                 }
             }
-            const SceneBox q = P.part[w];
+            const SceneBox &q = qs[k];
             for (int a = 0; a < 3; ++a) {
                 lo[a] = min(lo[a], q.lo[a]);
                 hi[a] = max(hi[a], q.hi[a]);
             }
         }
         const uint32_t s0 = (uint32_t)__shfl((int)cur, 0, 64);
-        if (__all(cur == s0 || cur == 0xffffffffu)) {
+        const bool one = __all(cur == s0 || cur == 0xffffffffu);  // the wave folds one scene
+        if (one) {
             for (int a = 0; a < 3; ++a) {
                 lo[a] = wave_min_u32(lo[a]);
                 hi[a] = wave_max_u32(hi[a]);
             }
-            if ((threadIdx.x & 63u) == 0u && s0 != 0xffffffffu) box_merge(&out[s0], lo, hi);
         } else if (cur != 0xffffffffu) {
             box_merge(&out[cur], lo, hi);
         }
+        // The workgroup's waves that each folded one scene merge through LDS
+        // when it is the same scene: one set of atomics per workgroup instead
+        // of per wave (every workgroup's atomics go to the same few words).
+        const uint32_t wave = threadIdx.x >> 6;
+        if ((threadIdx.x & 63u) == 0u) {
+            wsc[wave] = one ? s0 : 0xfffffffeu;
+            for (int a = 0; a < 3; ++a) {
+                wlo[wave][a] = lo[a];
+                whi[wave][a] = hi[a];
+            }
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63u) == 0u && one && s0 != 0xffffffffu) {
+            uint32_t lead = 0;  // the first wave of this scene merges for all of them
+            while (wsc[lead] != s0) ++lead;
+            if (lead == wave) {
+                for (uint32_t v = wave + 1u; v < kBlock / 64u; ++v)
+                    if (wsc[v] == s0)
+                        for (int a = 0; a < 3; ++a) {
+                            lo[a] = min(lo[a], wlo[v][a]);
+                            hi[a] = max(hi[a], whi[v][a]);
+                        }
+                box_merge(&out[s0], lo, hi);
+            }
+        }
+        __syncthreads();  // the LDS slots are read before the next chunk's
     }
 }
 
