@@ -129,11 +129,11 @@ void device_step(const SceneConfig &cfg, uint32_t n, uint32_t nf, float *pos, fl
     const uint64_t substeps = (uint64_t)ticks * std::max<uint32_t>(cfg.substeps, 1u);
     for (uint64_t k = 0; k < substeps; ++k) {
         hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream, P, 0u, 0u);
-        hipLaunchKernelGGL(k_keys, dim3(nb), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL(k_keys, dim3((nb + kSortBlocks - 1) / kSortBlocks), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_scan_reduce<8>, dim3(grid_scan), dim3(kBlock), 0, stream, P, tile_sums);
         hipLaunchKernelGGL(k_scan_tiles<8>, dim3(1), dim3(1024), 0, stream, P, tile_sums);
         hipLaunchKernelGGL(k_scan_apply<8>, dim3(grid_scan), dim3(kBlock), 0, stream, P, (const uint32_t *)tile_sums);
-        hipLaunchKernelGGL(k_place, dim3(nb), dim3(kBlock), 0, stream, P);
+        hipLaunchKernelGGL(k_place, dim3((nb + kSortBlocks - 1) / kSortBlocks), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL(k_canon_gather, dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL((k_density<256, 256>), dim3(nb), dim3(kBlock), 0, stream, P);
         hipLaunchKernelGGL((k_force_t<4, 256>), dim3(nb), dim3(kBlock), 0, stream, P);

@@ -1022,7 +1022,7 @@ void Engine::enqueue_substep(uint32_t tick_sub, uint32_t tick_len, const std::fu
     };
     hipLaunchKernelGGL(k_grid_box, dim3(1), dim3(1024), 0, stream_, P_, tick_sub, tick_len);
     M("grid_box");
-    hipLaunchKernelGGL(k_keys, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_keys, dim3((grid_slots + kSortBlocks - 1) / kSortBlocks), dim3(kBlock), 0, stream_, P_);
     M("keys");
     switch (tuning_.scan_tile_cells) {
         case 1024:
@@ -1051,7 +1051,7 @@ void Engine::enqueue_substep(uint32_t tick_sub, uint32_t tick_len, const std::fu
             break;
     }
     M("scan_apply");
-    hipLaunchKernelGGL(k_place, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
+    hipLaunchKernelGGL(k_place, dim3((grid_slots + kSortBlocks - 1) / kSortBlocks), dim3(kBlock), 0, stream_, P_);
     M("place");
     hipLaunchKernelGGL(k_canon_gather, dim3(grid_slots), dim3(kBlock), 0, stream_, P_);
     M("canon_gather");

@@ -381,34 +381,60 @@ __global__ __launch_bounds__(1024) void k_grid_box(Pool P, uint32_t tick_sub, ui
 __global__ __launch_bounds__(kBlock) void k_keys(Pool P) {
     if (P.ctl->err) return;
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
-        const uint32_t s = P.block_scene[b];
-        const SceneParams &sp = P.params[s];
-        const uint32_t m = b * kBlock + threadIdx.x;
-        const bool live = m < sp.pbase + sp.n;
-        uint32_t c = 0xffffffffu;
-        if (live) {
-            const SceneGrid &g = P.grid[s];
-            const float4 a = P.cA[m];
-            float f;
-            const uint32_t ux = axis_bin(g.gmin[0], a.x, g.inv, g.dims[0], &f);
-            const uint32_t uy = axis_bin(g.gmin[1], a.y, g.inv, g.dims[1], &f);
-            const uint32_t uz = axis_bin(g.gmin[2], a.z, g.inv, g.dims[2], &f);
-            c = g.cbase + (uz * g.dims[1] + uy) * g.dims[0] + ux;
+    const uint64_t upto = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);
+    // kSortBlocks pool blocks per workgroup iteration: every block's position
+    // loads, then every block's atomics, are in flight together (one block per
+    // iteration waited on a load and then on an atomic, back to back).
+    for (uint32_t bb = blockIdx.x; bb * kSortBlocks < P.nblocks; bb += gridDim.x) {
+        float4 a[kSortBlocks];
+        uint32_t sc[kSortBlocks];
+        bool live[kSortBlocks];
+#pragma unroll
+        for (uint32_t k = 0; k < kSortBlocks; ++k) {
+            const uint32_t b = bb * kSortBlocks + k;
+            live[k] = false;
+            sc[k] = 0u;
+            a[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (b < P.nblocks) {
+                sc[k] = P.block_scene[b];
+                const SceneParams &sp = P.params[sc[k]];
+                const uint32_t m = b * kBlock + threadIdx.x;
+                live[k] = m < sp.pbase + sp.n;
+                if (live[k]) a[k] = P.cA[m];
+            }
         }
-        const uint32_t prev = (uint32_t)__shfl_up((int)c, 1, 64);
-        const bool head = lane == 0u || prev != c;
-        const uint64_t heads = __ballot(head);
-        const uint64_t upto = lane == 63u ? ~0ull : ((1ull << (lane + 1u)) - 1ull);
-        const int hl = 63 - __clzll((long long)(heads & upto));
-        const uint64_t after = heads & ~upto;
-        const int nl = after ? __ffsll((long long)after) - 1 : 64;
-        uint32_t base = 0;
-        if (head && live) base = atomicAdd(&P.counts[c], (uint32_t)(nl - (int)lane));
-        base = (uint32_t)__shfl((int)base, hl, 64);
-        if (live) {
-            P.key[m] = c;
-            P.slot[m] = base + (lane - (uint32_t)hl);
+        uint32_t c[kSortBlocks], base[kSortBlocks];
+        int hl[kSortBlocks];
+#pragma unroll
+        for (uint32_t k = 0; k < kSortBlocks; ++k) {
+            c[k] = 0xffffffffu;
+            if (live[k]) {
+                const SceneGrid &g = P.grid[sc[k]];
+                float f;
+                const uint32_t ux = axis_bin(g.gmin[0], a[k].x, g.inv, g.dims[0], &f);
+                const uint32_t uy = axis_bin(g.gmin[1], a[k].y, g.inv, g.dims[1], &f);
+                const uint32_t uz = axis_bin(g.gmin[2], a[k].z, g.inv, g.dims[2], &f);
+                c[k] = g.cbase + (uz * g.dims[1] + uy) * g.dims[0] + ux;
+            }
+            // one atomic per run of equal keys in the wave (slot order is the
+            // previous grid order, so runs are long)
+            const uint32_t prev = (uint32_t)__shfl_up((int)c[k], 1, 64);
+            const bool head = lane == 0u || prev != c[k];
+            const uint64_t heads = __ballot(head);
+            hl[k] = 63 - __clzll((long long)(heads & upto));
+            const uint64_t after = heads & ~upto;
+            const int nl = after ? __ffsll((long long)after) - 1 : 64;
+            base[k] = 0;
+            if (head && live[k]) base[k] = atomicAdd(&P.counts[c[k]], (uint32_t)(nl - (int)lane));
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kSortBlocks; ++k) {
+            const uint32_t bs = (uint32_t)__shfl((int)base[k], hl[k], 64);
+            if (live[k]) {
+                const uint32_t m = (bb * kSortBlocks + k) * kBlock + threadIdx.x;
+                P.key[m] = c[k];
+                P.slot[m] = bs + (lane - (uint32_t)hl[k]);
+            }
         }
     }
 }
@@ -546,13 +572,34 @@ template __global__ void k_scan_apply<16>(Pool, const uint32_t *);
 // Scatter every cur slot to its (not yet canonical) place inside its cell.
 __global__ __launch_bounds__(kBlock) void k_place(Pool P) {
     if (P.ctl->err) return;
-    for (uint32_t b = blockIdx.x; b < P.nblocks; b += gridDim.x) {
-        const uint32_t s = P.block_scene[b];
-        const SceneParams &sp = P.params[s];
-        const uint32_t m = b * kBlock + threadIdx.x;
-        if (m >= sp.pbase + sp.n) continue;
-        const uint32_t dst = P.start[P.key[m]] + P.slot[m];
-        P.items[dst] = make_uint2(m, P.cg[m]);
+    // kSortBlocks pool blocks per workgroup iteration, loads of all of them
+    // in flight together (key -> cell start -> store is a chain per slot).
+    for (uint32_t bb = blockIdx.x; bb * kSortBlocks < P.nblocks; bb += gridDim.x) {
+        uint32_t key[kSortBlocks], slot[kSortBlocks], gid[kSortBlocks];
+        bool live[kSortBlocks];
+#pragma unroll
+        for (uint32_t k = 0; k < kSortBlocks; ++k) {
+            const uint32_t b = bb * kSortBlocks + k;
+            live[k] = false;
+            key[k] = slot[k] = gid[k] = 0u;
+            if (b < P.nblocks) {
+                const SceneParams &sp = P.params[P.block_scene[b]];
+                const uint32_t m = b * kBlock + threadIdx.x;
+                live[k] = m < sp.pbase + sp.n;
+                if (live[k]) {
+                    key[k] = P.key[m];
+                    slot[k] = P.slot[m];
+                    gid[k] = P.cg[m];
+                }
+            }
+        }
+        uint32_t st[kSortBlocks];
+#pragma unroll
+        for (uint32_t k = 0; k < kSortBlocks; ++k) st[k] = live[k] ? P.start[key[k]] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kSortBlocks; ++k)
+            if (live[k])
+                P.items[st[k] + slot[k]] = make_uint2((bb * kSortBlocks + k) * kBlock + threadIdx.x, gid[k]);
     }
 }
 
