@@ -122,7 +122,7 @@ def cpu_model():
 
 def cpu_threads():
     """The host cores this job may use: the machine's count, capped by the
-    share the GPU box allots one GPU (its OMP_NUM_THREADS, 16)."""
+    host-core share of one GPU (OMP_NUM_THREADS, 16 when unset)."""
     share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
     return max(1, min(os.cpu_count() or 1, share))
 
