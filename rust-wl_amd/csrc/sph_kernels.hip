@@ -874,55 +874,74 @@ __device__ __forceinline__ void density_pass(const Pool &P) {
             if (staged) {
                 const int nx = (int)g.dims[0], ny = (int)g.dims[1], nz = (int)g.dims[2];
                 const float lx = cur.fx * cur.fx, rx = (1.0f - cur.fx) * (1.0f - cur.fx);
-                for (int dz = -1; dz <= 1; ++dz) {
+                // One neighbour row (dz, dy) of the wave: the lane's window and
+                // the union's slot range. Its cell-start loads are issued one
+                // row ahead of the row's scan.
+                struct RowPlan {
+                    bool live;   // the wave has the row
+                    bool has;    // the lane has it
+                    bool left;   // the lane's window starts at dx = -1
+                    uint32_t r, base, n, j0, j1;
+                };
+                auto plan = [&](int q) -> RowPlan {
+                    RowPlan rp{false, false, false, (uint32_t)q, 0u, 0u, 0u, 0u};
+                    const int dz = q / 3 - 1, dy = q % 3 - 1;
                     const int z = cz0 + dz;
-                    if (z < 0 || z >= nz) continue;
+                    if (z < 0 || z >= nz) return rp;
                     const float ez = dz < 0 ? cur.fz : (dz > 0 ? 1.0f - cur.fz : 0.0f);
-                    for (int dy = -1; dy <= 1; ++dy) {
-                        const int y = cur.cy + dy;  // the lane's own row
-                        const uint32_t r = (uint32_t)(3 * (dz + 1) + dy + 1);
-                        // the lane's window, as for_row_windows forms it
-                        const float ey = dy < 0 ? cur.fy : (dy > 0 ? 1.0f - cur.fy : 0.0f);
-                        const float rd = ez * ez + ey * ey;
-                        const bool has = y >= 0 && y < ny && !(rd > cur.cut);  // the skips of for_row_windows
-                        const int xl = (cur.cx > 0 && rd + lx <= cur.cut) ? cur.cx - 1 : cur.cx;
-                        const int xr = (cur.cx + 1 < nx && rd + rx <= cur.cut) ? cur.cx + 1 : cur.cx;
-                        const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
-                        const uint32_t cf = row + (uint32_t)xl, cl = row + (uint32_t)xr;
-                        const uint64_t hm = __ballot(has);
-                        if (hm == 0u) continue;
-                        // The lanes' own cells (row + cx) rise with the lane index
-                        // (grid order), so the union runs from the first lane's cell
-                        // (or the one left of it) to the last lane's cell (or the
-                        // one right of it): at most a cell wider on each side than
-                        // the exact union, which costs a few staged records only.
-                        const int fl = __builtin_ctzll(hm), ll = 63 - __builtin_clzll(hm);
-                        const uint32_t own = row + (uint32_t)cur.cx;
-                        const uint32_t ofl = (uint32_t)__builtin_amdgcn_readlane((int)own, fl);
-                        const int cxf = __builtin_amdgcn_readlane(cur.cx, fl);
-                        const uint32_t oll = (uint32_t)__builtin_amdgcn_readlane((int)own, ll);
-                        const int cxl = __builtin_amdgcn_readlane(cur.cx, ll);
-                        const uint32_t ulo = ofl - (cxf > 0 ? 1u : 0u);
-                        const uint32_t uhi = oll + (cxl + 1 < nx ? 1u : 0u);
-                        const uint32_t base = P.start[ulo], n = P.start[uhi + 1u] - base;
-                        uint32_t j0 = 0, j1 = 0;
-                        if (has) {
-                            j0 = P.start[cf];
-                            j1 = P.start[cl + 1u];
-                        }
-                        if (n + 4u > STAGE) {
-                            if (has) scan_row(da, gsrc, r, xl < cur.cx, j0, j1);
-                            continue;
-                        }
-                        // The wave's LDS operations complete in order; the wave
-                        // barriers keep the compiler from moving the copy and the
-                        // reads of other lanes' records across each other.
-                        __builtin_amdgcn_wave_barrier();
-                        for (uint32_t k = lane; k < n + 4u; k += 64u) stage[k] = *at(sA, base + k);
-                        __builtin_amdgcn_wave_barrier();
-                        const LdsSrc lsrc{stage, base};
-                        if (has) scan_row(da, lsrc, r, xl < cur.cx, j0, j1);
+                    const int y = cur.cy + dy;  // the lane's own row
+                    // the lane's window, as for_row_windows forms it
+                    const float ey = dy < 0 ? cur.fy : (dy > 0 ? 1.0f - cur.fy : 0.0f);
+                    const float rd = ez * ez + ey * ey;
+                    rp.has = y >= 0 && y < ny && !(rd > cur.cut);  // the skips of for_row_windows
+                    const int xl = (cur.cx > 0 && rd + lx <= cur.cut) ? cur.cx - 1 : cur.cx;
+                    const int xr = (cur.cx + 1 < nx && rd + rx <= cur.cut) ? cur.cx + 1 : cur.cx;
+                    rp.left = xl < cur.cx;
+                    const uint32_t row = g.cbase + ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx;
+                    const uint32_t cf = row + (uint32_t)xl, cl = row + (uint32_t)xr;
+                    const uint64_t hm = __ballot(rp.has);
+                    if (hm == 0u) return rp;
+                    rp.live = true;
+                    // The lanes' own cells (row + cx) rise with the lane index
+                    // (grid order), so the union runs from the first lane's
+                    // cell (or the one left of it) to the last lane's cell (or
+                    // the one right of it): at most a cell wider on each side
+                    // than the exact union, which costs a few staged records.
+                    const int fl = __builtin_ctzll(hm), ll = 63 - __builtin_clzll(hm);
+                    const uint32_t own = row + (uint32_t)cur.cx;
+                    const uint32_t ofl = (uint32_t)__builtin_amdgcn_readlane((int)own, fl);
+                    const int cxf = __builtin_amdgcn_readlane(cur.cx, fl);
+                    const uint32_t oll = (uint32_t)__builtin_amdgcn_readlane((int)own, ll);
+                    const int cxl = __builtin_amdgcn_readlane(cur.cx, ll);
+                    const uint32_t ulo = ofl - (cxf > 0 ? 1u : 0u);
+                    const uint32_t uhi = oll + (cxl + 1 < nx ? 1u : 0u);
+                    rp.base = P.start[ulo];
+                    rp.n = P.start[uhi + 1u];  // the range's end until the row is used
+                    if (rp.has) {
+                        rp.j0 = P.start[cf];
+                        rp.j1 = P.start[cl + 1u];
                     }
+                    return rp;
+                };
+                RowPlan nxt = plan(0);
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    RowPlan rp = nxt;
+                    if (q + 1 < 9) nxt = plan(q + 1);
+                    if (!rp.live) continue;
+                    const uint32_t base = rp.base, n = rp.n - rp.base;
+                    if (n + 4u > STAGE) {
+                        if (rp.has) scan_row(da, gsrc, rp.r, rp.left, rp.j0, rp.j1);
+                        continue;
+                    }
+                    // The wave's LDS operations complete in order; the wave
+                    // barriers keep the compiler from moving the copy and the
+                    // reads of other lanes' records across each other.
+                    __builtin_amdgcn_wave_barrier();
+                    for (uint32_t k = lane; k < n + 4u; k += 64u) stage[k] = *at(sA, base + k);
+                    __builtin_amdgcn_wave_barrier();
+                    const LdsSrc lsrc{stage, base};
+                    if (rp.has) scan_row(da, lsrc, rp.r, rp.left, rp.j0, rp.j1);
                 }
             } else {
                 for_rows(g, P.start, cur, [&](uint32_t r, bool left, uint32_t j0, uint32_t j1) {
