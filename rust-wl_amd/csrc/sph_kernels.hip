@@ -244,8 +244,22 @@ __global__ __launch_bounds__(kBlock) void k_box_reduce(Pool P) {
                 lo[a] = wave_min_u32(lo[a]);
                 hi[a] = wave_max_u32(hi[a]);
             }
-        } else if (cur != 0xffffffffu) {
-            box_merge(&out[cur], lo, hi);
+        } else {
+            // A wave across scene boundaries: one reduction and one set of
+            // atomics per scene it holds (lanes of other scenes contribute
+            // the empty box).
+            uint64_t left = __ballot(cur != 0xffffffffu);
+            while (left) {
+                const uint32_t sc = (uint32_t)__builtin_amdgcn_readlane((int)cur, __builtin_ctzll(left));
+                const bool mine = cur == sc;
+                uint32_t l2[3], h2[3];
+                for (int a = 0; a < 3; ++a) {
+                    l2[a] = wave_min_u32(mine ? lo[a] : 0xffffffffu);
+                    h2[a] = wave_max_u32(mine ? hi[a] : 0u);
+                }
+                if ((threadIdx.x & 63u) == 0u) box_merge(&out[sc], l2, h2);
+                left &= ~__ballot(mine);
+            }
         }
         // The workgroup's waves that each folded one scene merge through LDS
         // when it is the same scene: one set of atomics per workgroup instead
