@@ -1939,21 +1939,27 @@ __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
 //
 // One workgroup per span. The terms (f_k and the cross products) do not
 // depend on the running sums, so waves 1..3 compute them, kFold particles per
-// chunk, into one of two LDS buffers (component-major rows, padded by one
-// float4 so the six rows start in different banks), while lane c (c < 6) of
-// wave 0 adds component c of the other buffer's terms in index order: six
-// independent chains of dependent additions, fed from LDS 64 terms at a time.
-// Same operations, same order, same bits as a serial fold; the chain is the
-// kernel's time (about 2.9 ns per addition on MI355X, against 2.6 ns for
+// chunk (each wave a contiguous segment of kSeg), into one of two LDS buffers
+// (component-major rows, padded by one float4 so the six rows start in
+// different banks), while lane c (c < 6) of wave 0 adds component c of the
+// other buffer's terms in index order: six independent chains of dependent
+// additions, fed from LDS 64 terms at a time. A particle whose six terms are
+// all zero (a tank particle without fluid neighbours: bforce = +0, so the
+// cross products are +-0) is left out of its segment, order kept: adding +-0
+// leaves a sum unchanged, and no sum is ever -0 (it starts at +0, and x + y
+// is -0 in round-to-nearest only when both are -0). Same values added in the
+// same order as a serial fold over every particle, same bits; the chain is
+// the kernel's time (about 2.9 ns per addition on MI355X, against 2.6 ns for
 // operands already in registers), and the terms' loads and products run
-// beside it. A chunk (1536 particles, ~4.5 us of additions) outlasts the
-// latency of the producers' loads, issued one chunk ahead; with 384 the
-// producers waited on them and a chunk took 1.6 us instead of 1.1.
+// beside it. A chunk (1536 particles, ~4.5 us of additions when all are wet)
+// outlasts the latency of the producers' loads, issued one chunk ahead.
 __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
-    constexpr uint32_t kProd = kReactThreads - 64u;          // producer lanes
-    constexpr uint32_t kPer = 8, kFold = kPer * kProd;        // particles per chunk
-    constexpr uint32_t kRow = kFold / 4u + 1u;                // float4 per component row
+    constexpr uint32_t kProd = kReactThreads - 64u;           // producer lanes
+    constexpr uint32_t kPer = 8, kFold = kPer * kProd;         // particles per chunk
+    constexpr uint32_t kWaves = kProd / 64u, kSeg = kFold / kWaves;  // one segment per producer wave
+    constexpr uint32_t kRow = kFold / 4u + 1u;                 // float4 per component row
     __shared__ float4 term[2][6][kRow];
+    __shared__ uint32_t segn[2][kWaves];  // terms kept in each segment
     if (P.ctl->err) return;
     const uint32_t k = blockIdx.x;
     if (k >= P.nspans) return;
@@ -1964,13 +1970,15 @@ __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
     if (!substep_active(tick_sub, tick_len, sub)) return;  // the scene does not step in this launch
     const uint32_t phase = tick_sub + sub - tick_len;      // the scene's substep index inside its tick
     const bool folder = threadIdx.x < 64u;
-    const uint32_t lane = threadIdx.x, t = threadIdx.x - 64u;  // t: producer index (waves 1..3)
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t pw = (threadIdx.x >> 6) - 1u;  // producer wave (waves 1..3)
+    const uint64_t below = lane ? (~0ull >> (64u - lane)) : 0ull;
     float acc = 0.0f;  // component `lane` of (F, tau) for lanes 0..5 of wave 0
     float4 f[kPer], p[kPer];
     auto fetch = [&](uint32_t q) {
 #pragma unroll
         for (uint32_t m = 0; m < kPer; ++m) {
-            const uint32_t j = q + m * kProd + t;
+            const uint32_t j = q + pw * kSeg + m * 64u + lane;
             f[m] = p[m] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (j < sd.end) {
                 f[m] = P.bf[j];
@@ -1978,19 +1986,32 @@ __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
             }
         }
     };
+    // The segment's nonzero terms, in index order (past the span's end the
+    // terms are zero too: nothing is written for them).
     auto put = [&](uint32_t buf) {
         float *tc = &term[buf][0][0].x;
+        uint32_t n = 0;  // terms kept so far (wave-uniform)
 #pragma unroll
         for (uint32_t m = 0; m < kPer; ++m) {
-            const uint32_t u = m * kProd + t;
             const float rx = p[m].x - sd.origin[0], ry = p[m].y - sd.origin[1], rz = p[m].z - sd.origin[2];
-            tc[0 * 4u * kRow + u] = f[m].x;
-            tc[1 * 4u * kRow + u] = f[m].y;
-            tc[2 * 4u * kRow + u] = f[m].z;
-            tc[3 * 4u * kRow + u] = ry * f[m].z - rz * f[m].y;
-            tc[4 * 4u * kRow + u] = rz * f[m].x - rx * f[m].z;
-            tc[5 * 4u * kRow + u] = rx * f[m].y - ry * f[m].x;
+            const float t3 = ry * f[m].z - rz * f[m].y;
+            const float t4 = rz * f[m].x - rx * f[m].z;
+            const float t5 = rx * f[m].y - ry * f[m].x;
+            const bool keep = f[m].x != 0.0f || f[m].y != 0.0f || f[m].z != 0.0f || t3 != 0.0f || t4 != 0.0f ||
+                              t5 != 0.0f;
+            const uint64_t kb = __ballot(keep);
+            if (keep) {
+                const uint32_t u = pw * kSeg + n + (uint32_t)__popcll(kb & below);
+                tc[0 * 4u * kRow + u] = f[m].x;
+                tc[1 * 4u * kRow + u] = f[m].y;
+                tc[2 * 4u * kRow + u] = f[m].z;
+                tc[3 * 4u * kRow + u] = t3;
+                tc[4 * 4u * kRow + u] = t4;
+                tc[5 * 4u * kRow + u] = t5;
+            }
+            n += (uint32_t)__popcll(kb);
         }
+        if (lane == 0u) segn[buf][pw] = n;
     };
     if (!folder) {
         fetch(sd.start);
@@ -2002,23 +2023,26 @@ __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
     for (uint32_t q = sd.start; q < sd.end; q += kFold, buf ^= 1u) {
         if (folder) {
             if (lane < 6u) {
-                const uint32_t nq = min(kFold, sd.end - q);
-                const float4 *row = term[buf][lane];
-                uint32_t u = 0;
-                for (; u + 64u <= nq; u += 64u) {
-                    float4 tv[16];
+#pragma unroll 1
+                for (uint32_t w = 0; w < kWaves; ++w) {
+                    const uint32_t nq = segn[buf][w];
+                    const float4 *row = &term[buf][lane][w * (kSeg / 4u)];
+                    uint32_t u = 0;
+                    for (; u + 64u <= nq; u += 64u) {
+                        float4 tv[16];
 #pragma unroll
-                    for (uint32_t v = 0; v < 16u; ++v) tv[v] = row[u / 4u + v];
+                        for (uint32_t v = 0; v < 16u; ++v) tv[v] = row[u / 4u + v];
 #pragma unroll
-                    for (uint32_t v = 0; v < 16u; ++v) {
-                        acc = acc + tv[v].x;
-                        acc = acc + tv[v].y;
-                        acc = acc + tv[v].z;
-                        acc = acc + tv[v].w;
+                        for (uint32_t v = 0; v < 16u; ++v) {
+                            acc = acc + tv[v].x;
+                            acc = acc + tv[v].y;
+                            acc = acc + tv[v].z;
+                            acc = acc + tv[v].w;
+                        }
                     }
+                    const float *rowf = &row[0].x;
+                    for (; u < nq; ++u) acc = acc + rowf[u];
                 }
-                const float *rowf = &row[0].x;
-                for (; u < nq; ++u) acc = acc + rowf[u];
             }
         } else if (q + kFold < sd.end) {
             put(buf ^ 1u);  // the next chunk, into the buffer folded in the previous iteration
@@ -2026,7 +2050,7 @@ __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
         }
         __syncthreads();  // this chunk is folded and the next one written
     }
-    if (lane >= 6u) return;
+    if (!folder || lane >= 6u) return;
     float *o = P.react + 6ull * k + lane;
     *o = phase == 0 ? acc : *o + acc;
     if (sub > 1 && phase == sub - 1) *o = *o * (1.0f / (float)sub);
