@@ -1943,16 +1943,18 @@ __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {
 // (component-major rows, padded by one float4 so the six rows start in
 // different banks), while lane c (c < 6) of wave 0 adds component c of the
 // other buffer's terms in index order: six independent chains of dependent
-// additions, fed from LDS 64 terms at a time. A particle whose six terms are
-// all zero (a tank particle without fluid neighbours: bforce = +0, so the
-// cross products are +-0) is left out of its segment, order kept: adding +-0
-// leaves a sum unchanged, and no sum is ever -0 (it starts at +0, and x + y
-// is -0 in round-to-nearest only when both are -0). Same values added in the
-// same order as a serial fold over every particle, same bits; the chain is
-// the kernel's time (about 2.9 ns per addition on MI355X, against 2.6 ns for
-// operands already in registers), and the terms' loads and products run
-// beside it. A chunk (1536 particles, ~4.5 us of additions when all are wet)
-// outlasts the latency of the producers' loads, issued one chunk ahead.
+// additions, fed from LDS 64 (then 16) terms at a time. A particle whose six
+// terms are all zero (a tank particle without fluid neighbours: bforce = +0,
+// so the cross products are +-0) is left out of its segment, order kept:
+// adding +-0 leaves a sum unchanged, and no sum is ever -0 (it starts at +0,
+// and x + y is -0 in round-to-nearest only when both are -0); for the same
+// reason each segment is padded with +0 terms to a multiple of 16. Same
+// values added in the same order as a serial fold over every particle, same
+// bits; the chain is the kernel's time (about 2.9 ns per addition on MI355X,
+// against 2.6 ns for operands already in registers), and the terms' loads
+// and products run beside it. A chunk (1536 particles, ~4.5 us of additions
+// when all are wet) outlasts the latency of the producers' loads, issued one
+// chunk ahead.
 __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
     constexpr uint32_t kProd = kReactThreads - 64u;           // producer lanes
     constexpr uint32_t kPer = 8, kFold = kPer * kProd;         // particles per chunk
@@ -2011,7 +2013,15 @@ __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
             }
             n += (uint32_t)__popcll(kb);
         }
-        if (lane == 0u) segn[buf][pw] = n;
+        // Pad the segment with +0 terms to a multiple of 16 (a no-op for the
+        // sums, as the left-out ones), so the fold reads whole float4 groups.
+        const uint32_t npad = (n + 15u) & ~15u;
+        if (n + lane < npad) {
+            const uint32_t u = pw * kSeg + n + lane;
+#pragma unroll
+            for (uint32_t c = 0; c < 6u; ++c) tc[c * 4u * kRow + u] = 0.0f;
+        }
+        if (lane == 0u) segn[buf][pw] = npad;
     };
     if (!folder) {
         fetch(sd.start);
@@ -2040,8 +2050,18 @@ __global__ __launch_bounds__(kReactThreads) void k_reactions(Pool P) {
                             acc = acc + tv[v].w;
                         }
                     }
-                    const float *rowf = &row[0].x;
-                    for (; u < nq; ++u) acc = acc + rowf[u];
+                    for (; u < nq; u += 16u) {  // nq is a multiple of 16
+                        float4 tv[4];
+#pragma unroll
+                        for (uint32_t v = 0; v < 4u; ++v) tv[v] = row[u / 4u + v];
+#pragma unroll
+                        for (uint32_t v = 0; v < 4u; ++v) {
+                            acc = acc + tv[v].x;
+                            acc = acc + tv[v].y;
+                            acc = acc + tv[v].z;
+                            acc = acc + tv[v].w;
+                        }
+                    }
                 }
             }
         } else if (q + kFold < sd.end) {
