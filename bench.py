@@ -63,15 +63,17 @@ def issue_rate(kern, tuning, args):
     with open(path) as f:
         counts = json.load(f)
     entries, probe = counts["entries"], counts["probe"]
-    keys = ("force_kernel", "block_threads", "stage_records")
-    counted = next((e for e in entries if all(e["tuning"][k] == tuning[k] for k in keys)), None)
-    if counted is None:
-        return None  # no counter pass for this launch geometry
+    # The launch-geometry fields each pass's instruction count depends on.
+    keys = {"force": ("force_kernel", "block_threads"), "density": ("block_threads", "stage_records")}
     peak = cal["fma_wave_instr_per_simd_s"]
     out = {"bound": "valu", "unit": "wave-instr/SIMD/s", "peak": round(peak), "peak_kernel_ms": cal["kernel_ms"],
            "simds": cal["simds"], "peak_per_cycle": probe["valu_per_simd_per_cycle"],
-           "counted_tuning": counted["tuning"], "kernels": {}}
-    for name, c in counted["kernels"].items():
+           "tuning": {k: tuning[k] for k in ("force_kernel", "block_threads", "stage_records")}, "kernels": {}}
+    for name, ks in keys.items():
+        counted = next((e for e in entries if all(e["tuning"][k] == tuning[k] for k in ks)), None)
+        if counted is None:
+            continue  # no counter pass for this launch geometry
+        c = counted["kernels"][name]
         if name in kern and kern[name] > 0:
             ach = c["valu_wave_instr"] / (kern[name] * 1e-3) / cal["simds"]
             # per clock cycle (the counter pass's own cycles, kernel and probe
@@ -80,6 +82,8 @@ def issue_rate(kern, tuning, args):
             out["kernels"][name] = {"kernel": c["kernel"], "valu_wave_instr": c["valu_wave_instr"],
                                     "achieved": round(ach), "frac": round(ach / peak, 4),
                                     "per_cycle": pc, "frac_per_cycle": round(pc / probe["valu_per_simd_per_cycle"], 4)}
+    if not out["kernels"]:
+        return None
     print("# vector-ALU issue: peak %.3e wave-instr/SIMD/s; " % peak
           + ", ".join(f"{k} {v['frac']:.0%} ({v['frac_per_cycle']:.0%} per cycle)" for k, v in out["kernels"].items()),
           file=sys.stderr, flush=True)
