@@ -370,7 +370,8 @@ def main():
                     help='launch geometry as JSON instead of the autotuner (e.g. \'{"force_kernel": 3, '
                          '"block_threads": 128}\'), for profiles of a fixed kernel set')
     ap.add_argument("--late-steps", type=int, default=480,
-                    help="step the headline scene on to this step and time 20 more (0 skips)")
+                    help="step the headline scene (afresh, with the headline's tuning) to this step and time 20 "
+                         "more (0 skips)")
     ap.add_argument("--no-ref-rule", action="store_true", help="skip the 2^21 grid-budget figure")
     ap.add_argument("--no-stepper", action="store_true", help="skip the FluidStepper drop-in timings")
     ap.add_argument("--stepper-ticks", type=int, default=20)
@@ -462,19 +463,23 @@ def main():
     prof = eng.profile(args.profile_steps)
     tuning = eng.get_tuning()
     g = eng.read_grid(sid)
-    # Late in the run: the same scene stepped on to step 480, then 20 steps
-    # timed (the flow is far from the initial lattice by then).
+    # Late in the run: the same scene from its start, with the tuning above,
+    # stepped to step late_steps, then 20 steps timed (the flow is far from the
+    # initial lattice by then). A fresh engine, so the window is exact (the
+    # autotuner's own substeps advanced the first one).
     late = None
     if args.late_steps:
-        done = args.warmup + 2 * args.steps + args.profile_steps + 4
-        if done < args.late_steps:
-            eng.step(args.late_steps - done)
-        eng.sync()
+        el = V.Engine()
+        el.add(cfg, data, reactions=False)
+        el.set_tuning(**tuning)
+        el.step(args.late_steps)
+        el.sync()
         t0 = time.perf_counter()
-        eng.step(20)
-        eng.sync()
+        el.step(20)
+        el.sync()
         lw = time.perf_counter() - t0
-        late = {"steps": f"{max(done, args.late_steps)}-{max(done, args.late_steps) + 20}",
+        el.close()
+        late = {"steps": f"{args.late_steps}-{args.late_steps + 20}",
                 "ms_per_step": 1e3 * lw / 20, "value": NF * 20 / lw, "unit": "particle-updates/s"}
         print(f"# late run (steps {late['steps']}): {late['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     C = g["total"] + 1

@@ -1213,7 +1213,9 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
     // Coordinate descent: each knob in turn over its values with the others
     // held, until a pass changes nothing (each knob acts mostly on one
     // kernel, so the knobs hardly interact). A change must win by 1 % to
-    // stand over run-to-run noise.
+    // stand over run-to-run noise, against the current choice timed right
+    // after it: the scene moves on while it is tuned, and early in a run its
+    // cost per substep drifts by more than that.
     struct Axis {
         uint32_t Tuning::*field;
         std::vector<uint32_t> values;
@@ -1228,7 +1230,6 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
         {&Tuning::persist_blocks, {0u, 8u * cus}},
     };
     Tuning best = tuning_;
-    float best_ms = cost(best);
     for (int pass = 0; pass < 3; ++pass) {
         bool changed = false;
         for (const Axis &ax : axes)
@@ -1237,8 +1238,8 @@ Engine::Tuning Engine::autotune(uint32_t substeps) {
                 Tuning t = best;
                 t.*(ax.field) = v;
                 const float ms = cost(t);
-                if (ms < 0.99f * best_ms) {
-                    best_ms = ms;
+                const float ref = cost(best);
+                if (ms < 0.99f * ref) {
                     best = t;
                     changed = true;
                 }
