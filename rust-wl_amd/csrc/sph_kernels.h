@@ -134,7 +134,7 @@ __global__ void k_wall_index_clear(Pool P, uint32_t s, uint32_t side);
 __global__ void k_wall_index_set(Pool P, uint32_t s, uint32_t side, const uint32_t *ids, uint32_t ne);
 constexpr uint32_t kSortBlocks = 4;      // k_keys, k_place: pool blocks per workgroup iteration
 constexpr uint32_t kListBlocks = 16;     // k_bforce_list: pool blocks per workgroup
-constexpr uint32_t kBforceThreads = 128;  // k_bforce: slots per workgroup (LDS: 256 B per slot)
+constexpr uint32_t kBforceThreads = 128;  // k_bforce: slots per workgroup (LDS: 8 B per slot and batch entry)
 constexpr uint32_t kReactThreads = 256;  // k_reactions: one workgroup per span (1 folding wave, 3 producers)
 __global__ void k_reactions(Pool P);
 // Rigid leg, once per tick: integrate the bodies (before the first substep),

@@ -1807,8 +1807,10 @@ __global__ __launch_bounds__(kBlock) void k_bforce_list(Pool P) {
 // subtracted (insertion into the lane's column of an LDS table of index and
 // slot), then evaluates and subtracts them in that order, and passes again
 // while any were left out. Any neighbour count gives the reference's sum bit
-// for bit; one pass covers the usual case (a wall particle sees about half a
-// neighbourhood). Only each cell's fluid head is scanned, so a dry tank
+// for bit; one or two passes cover the usual case (a wall particle sees about
+// half a neighbourhood, ~16 fluid particles), and the small table keeps the
+// workgroup's LDS low enough for five waves per SIMD. Only each cell's fluid
+// head is scanned, so a dry tank
 // particle tests no candidate. A grid-stride loop over the list
 // (kBforceThreads entries per workgroup and round).
 __global__ __launch_bounds__(kBforceThreads) void k_bforce(Pool P) {

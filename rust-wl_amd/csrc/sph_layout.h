@@ -102,7 +102,7 @@ struct SpanDev {
 constexpr uint32_t kReactAll = 1;      // every span (FluidStepper semantics)
 constexpr uint32_t kReactDynamic = 2;  // only spans of dynamic rigid bodies (coupled scenes)
 
-constexpr uint32_t kReactBatch = 32;  // fluid neighbours a boundary particle's reaction pass holds at once
+constexpr uint32_t kReactBatch = 16;  // fluid neighbours a boundary particle's reaction pass holds at once
 
 // Provider walls of a scene (crates/vxl-phys-core/src/interop.rs:166-224).
 //

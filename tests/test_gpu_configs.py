@@ -149,7 +149,7 @@ def test_stepper_moving_body_matches_oracle_with_bounds():
 def test_boundary_reactions_past_one_batch():
     """A plate of boundary particles inside fluid sampled at a third of h:
     each plate particle has about 100 fluid neighbours, more than one batch of
-    the reaction pass (32), so its sum takes several passes in index order.
+    the reaction pass (16), so its sum takes several passes in index order.
     Reactions and state equal the oracle bit for bit."""
     s = F(0.02)
     cfg = V.fluid_config(smoothing_radius=0.06, particle_mass=float(F(1000.0) * s * s * s), spacing=float(s),
