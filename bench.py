@@ -396,7 +396,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box here has one MI355X (asked for --gpus {args.gpus}).",
+        print(f"bench.py measures one GPU (one MI355X); --gpus {args.gpus} is not supported.",
               file=sys.stderr)
         sys.exit(2)
 
